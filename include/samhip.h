@@ -1,0 +1,152 @@
+/*
+ * samhip.h -- C ABI of libsamhip, the MI355X (gfx950) implementation of
+ * SAM's RK3 pseudo-spectral time step.
+ *
+ * All entry points are extern "C", take plain pointers and sizes, and
+ * return an int status (SAM_OK = 0).  Complex arrays are interleaved
+ * (re, im) doubles, i.e. Fortran complex(8).  Array shapes are written
+ * Fortran-style, first index fastest, exactly as the reference declares
+ * them.  Host arrays are copied to/from the device by the call; the step
+ * itself keeps its state resident in HBM behind the handle.
+ *
+ * One handle = one grid = one HIP stream on the current device.  Calls on
+ * one handle must come from one host thread at a time.  Every call that
+ * returns host data synchronises the handle's stream.
+ *
+ * Notation: kx_max,ky_max,kz_max per init.f:49-62; Nx_min = kx_max+1,
+ * Ny_min = 2*ky_max+1, Nz_min = 2*kz_max+1 (init.f:71-73); Nze = Nz for the
+ * periodic z case (init.f:78-80); Lu = 3 (4 if i_strat), Lr = 6 (9).
+ */
+#ifndef SAMHIP_H
+#define SAMHIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+#define SAM_OK               0
+#define SAM_ERR_REQUIRED     6   /* a required input is missing (check_inputs.f:222-226) */
+#define SAM_ERR_VALUE        9   /* an input has an invalid value (check_inputs.f:227-230) */
+#define SAM_ERR_SIZE        10   /* a buffer/file size does not match the grid (read_field.f:26-32) */
+#define SAM_ERR_UNSUPPORTED 20   /* valid for the reference, not built here (e.g. iubc_z != 0) */
+#define SAM_ERR_STATE       21   /* call out of order (e.g. spectra before any RHS) */
+#define SAM_ERR_HIP         30   /* a HIP runtime call failed; see sam_last_error() */
+#define SAM_ERR_NOMEM       31   /* device allocation failed */
+#define SAM_ERR_NODEVICE    32   /* no usable gfx950 device */
+
+/* precision of the transforms / physical-space products */
+#define SAM_PREC_FP64        0   /* plain FP64 drop-in */
+#define SAM_PREC_SPLIT       1   /* FP32 transforms+products, FP64 state/RK/reductions */
+
+/* subgrid-scale model coupled into the step */
+#define SAM_SGS_NONE         0
+#define SAM_SGS_SMAG         1   /* sm.f: nu_t = c_smag * Delta^2 * |S| */
+#define SAM_SGS_DYN          2   /* dm.f: Germano-Lilly, volume-averaged coefficient */
+#define SAM_SGS_MLP          3   /* learned per-point closure (no reference counterpart) */
+
+/* Run parameters: the subset of input.dat (set_labels.f:29-102) and of the
+ * COMMON blocks of sam.h:13-55 that the step reads. */
+typedef struct sam_params {
+    int    nx, ny, nz;          /* Nx Ny Nz: powers of two, 16..1024 */
+    int    n_dealias;           /* 0, 1 (2/3 rule, default) or 2 (init.f:49-62) */
+    int    i_strat;             /* 0: u,v,w; 1: + temperature T' (Boussinesq) */
+    int    nrk_max;             /* 1, 2 or 3 (init.f:117-145) */
+    int    i_cfl;               /* 1: dt = cfl0*adv_cfl_limit/max(u_i,max k_i,max) (vel_max.f:43-46) */
+    int    iubc_z;              /* must be 0 (periodic z); others -> SAM_ERR_UNSUPPORTED */
+    int    precision;           /* SAM_PREC_* */
+    int    sgs;                 /* SAM_SGS_* */
+    double xl, yl, zl;          /* box lengths */
+    double vis;                 /* kinematic viscosity */
+    double pr;                  /* Prandtl number (T uses vis/pr) */
+    double grav, to;            /* buoy_fac_z = grav/To (init.f:101-105) */
+    double lapse;               /* background gradient in the T equation (rhs.f:133);
+                                   lapse0 for a fresh run, 0 reproduces the restart quirk */
+    double cfl0, dt0;           /* CFL target / fixed dt */
+    double c_smag;              /* Smagorinsky c_smag (sm.f:32) */
+    double pr_t;                /* turbulent Prandtl number of the SGS scalar flux */
+} sam_params;
+
+typedef struct sam_dims {
+    int kx_max, ky_max, kz_max;
+    int nx_min, ny_min, nz_min, nze;
+    int lu, lr, lm;             /* lm = number of spectral shells (sam.f:72-73) */
+} sam_dims;
+
+/* Diagnostics of the most recent first substep (vel_max.f:28-49) plus the
+ * run clock.  u_* are per field (u, v, w, T'). */
+typedef struct sam_diag {
+    long long nt;
+    double time, dt;
+    double u_avg[4], u_var[4], u_max[4];
+    double energy;              /* 0.5*(u_var(1)+u_var(2)+u_var(3)) */
+    double cfl_x, cfl_y, cfl_z;
+    double c_delta_sq;          /* C*Delta^2 of the dynamic model (dm.f:176), else 0 */
+    int    sgs_active;          /* SGS model actually used by the last step */
+    int    pad_;
+} sam_diag;
+
+typedef struct sam_handle sam_handle;
+
+/* ---- lifecycle ------------------------------------------------------- */
+/* Replaces init.f + fft_init.f + the allocations of sam.f:80-98. */
+int  sam_create(const sam_params *p, sam_handle **h);
+int  sam_destroy(sam_handle *h);
+int  sam_get_dims(const sam_handle *h, sam_dims *d);
+const char *sam_last_error(void);
+
+/* ---- state (read_field.f / write_field.f record layout) --------------- */
+/* u: (ldz, Ny_min, Nx_min, Lu) complex, ldz >= Nz_min (Nz_min = the vel.*
+ * file layout, Nze = sam.f's in-memory u).  Rows beyond Nz_min ignored on
+ * set, zero-filled on get. */
+int  sam_set_state(sam_handle *h, const double *u, int ldz);
+int  sam_get_state(sam_handle *h, double *u, int ldz);
+/* clock carried in header.* (time, dt, nt; write_header.f:19-25) */
+int  sam_set_clock(sam_handle *h, double time, double dt, long long nt);
+
+/* ---- the time step (sam.f:122-326) ----------------------------------- */
+/* nsteps full RK steps.  diag (may be NULL) receives the statistics of the
+ * last step's first substep and the clock after the last step.  With
+ * want_spectra != 0 the shell spectra of the last step's first substep
+ * (state and RHS at the start of that step, as spectra.f is called in
+ * sam.f:248-250) are kept for sam_get_spectra. */
+int  sam_step(sam_handle *h, int nsteps, int want_spectra, sam_diag *diag);
+/* RHS-only pass of the current state (the nt_end pass, sam.f:288).
+ * r (may be NULL): (ldz, Ny_min, Nx_min, Lu+1) complex, last field = p. */
+int  sam_rhs(sam_handle *h, double *r, int ldz, int want_spectra, sam_diag *diag);
+/* spectra.f:13-111 of the state/RHS pair captured by the last call with
+ * want_spectra.  out: ek(lm) tk(lm) dk(lm) vt(lm) sample(lm) energy_w
+ * div_rms div_max  (5*lm + 3 doubles). */
+int  sam_get_spectra(sam_handle *h, double *out);
+
+/* ---- stage entry points (parity with the reference's routines) -------- */
+/* z_trans_b.f (case 0): u (ldz,Ny_min,Nx_min,nvar) -> out (Nz,Ny_min,Nx_min,nvar).
+ * The reference scatters to (Ny_min,Nx_min,Nz,nvar); this is the same data
+ * with z leading, the layout the next stage reads. */
+int  sam_z_trans_b(sam_handle *h, const double *u, int ldz, double *out, int nvar);
+/* x2z_decomp.f + xy_trans_b.f: (Nz,Ny_min,Nx_min,nvar) -> physical (Nx,Ny,Nz,nvar) real */
+int  sam_xy_trans_b(sam_handle *h, const double *in, double *out, int nvar);
+/* xy_trans_f.f + z2x_decomp.f: physical (Nx,Ny,Nz,nvar) -> (Nz,Ny_min,Nx_min,nvar) */
+int  sam_xy_trans_f(sam_handle *h, const double *in, double *out, int nvar);
+/* z_trans_f.f (case 0): (Nz,Ny_min,Nx_min,nvar) -> (Nz_min,Ny_min,Nx_min,nvar) */
+int  sam_z_trans_f(sam_handle *h, const double *in, double *out, int nvar);
+/* rhs.f:32-61 products: physical (Nx,Ny,Nz,Lu) -> (Nx,Ny,Nz,Lr) */
+int  sam_products(sam_handle *h, const double *phys, double *prod);
+/* vel_max.f: stats[0..3] u_avg, [4..7] u_var, [8..11] u_max, [12] energy,
+ * [13..15] cfl_{x,y,z}_by_dt, [16] dt (dt_in unless i_cfl=1) */
+int  sam_vel_max(sam_handle *h, const double *phys, double dt_in, double *stats);
+/* rhs.f:74-142: F (Nz_min,Ny_min,Nx_min,Lr), usav (Nz_min,Ny_min,Nx_min,Lu)
+ * -> r (Nz_min,Ny_min,Nx_min,Lu+1), last field = pressure */
+int  sam_project(sam_handle *h, const double *F, const double *usav, double *r);
+
+/* ---- subgrid-scale pieces (sm.f, dm.f, filter.f) ---------------------- */
+/* eddy viscosity of a physical velocity (Nx,Ny,Nz,Lu) with the handle's
+ * sgs model (SMAG or DYN); vt (Nx,Ny,Nz); c_delta_sq may be NULL */
+int  sam_eddy_visc(sam_handle *h, const double *phys, double *vt, double *c_delta_sq);
+/* (1,4,1)/6 periodic test filter of nvar physical fields (filter.f:9-42) */
+int  sam_test_filter(sam_handle *h, const double *in, double *out, int nvar);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SAMHIP_H */

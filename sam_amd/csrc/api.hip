@@ -1,0 +1,630 @@
+// api.hip -- the extern "C" surface of libsamhip (include/samhip.h).
+//
+// One handle owns the device-resident state of one grid:
+//   U[2]   spectral state, ping-pong          (Nz_min,Ny_min,Nx_min,Lu) c16
+//   R      RHS of the last substep (+ pressure) (Nz_min,Ny_min,Nx_min,Lu+1) c16
+//   Zb     z-physical work, Lr fields          (Nz,Ny_min,Nx_min,Lr) complex T
+//   Yb     y,z-physical work, Lr fields        (Ny,Nx_min,Nz,Lr) complex T
+//   Pb     physical fields (LES path and stage calls only), allocated on demand
+// T = double (SAM_PREC_FP64) or float (SAM_PREC_SPLIT).
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <vector>
+
+#include "internal.hpp"
+
+using namespace samhip;
+
+static thread_local char g_err[512] = "";
+
+static int set_err(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIPCHK(call)                                                                                  \
+    do {                                                                                              \
+        hipError_t e_ = (call);                                                                       \
+        if (e_ != hipSuccess)                                                                         \
+            return set_err(SAM_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),        \
+                           __FILE__, __LINE__);                                                       \
+    } while (0)
+
+#define RCHK(call)                                                                                    \
+    do {                                                                                              \
+        int r_ = (call);                                                                              \
+        if (r_ != SAM_OK) {                                                                           \
+            if (r_ == SAM_ERR_HIP)                                                                    \
+                return set_err(r_, "kernel launch failed in %s: %s", #call,                           \
+                               hipGetErrorString(hipGetLastError()));                                 \
+            return set_err(r_, "%s returned %d", #call, r_);                                          \
+        }                                                                                             \
+    } while (0)
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+struct sam_handle {
+    sam_params p;
+    Dims d;
+    StepConsts k;
+    int dev = 0;
+    hipStream_t st = nullptr;
+    int prec = 0;
+    double *U[2] = {nullptr, nullptr};
+    int cur = 0;
+    double *R = nullptr;
+    void *Zb = nullptr, *Yb = nullptr;
+    DevBuf Pb, stage0, stage1, sgsw;
+    double *partials = nullptr;
+    double *specout = nullptr;
+    DevDiag *diag = nullptr;     // [0] step, [1] stage calls
+    void *tw[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
+    int have_spectra = 0;
+    int sgs_active = 0;
+};
+
+static bool pow2_in(int n, int lo, int hi) { return n >= lo && n <= hi && (n & (n - 1)) == 0; }
+
+static int kmax_of(int n, int mode, int xdir)
+{
+    if (mode == 0) return n / 2 - 1;
+    if (mode == 2 && xdir) return n / 2 - 1;
+    return (int)(((double)n - 0.1) / 3.0);
+}
+
+static int ensure(DevBuf &b, size_t bytes)
+{
+    if (b.bytes >= bytes) return SAM_OK;
+    if (b.p) hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+    if (hipMalloc(&b.p, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return set_err(SAM_ERR_NOMEM, "hipMalloc(%zu) failed", bytes);
+    }
+    b.bytes = bytes;
+    return SAM_OK;
+}
+
+static int make_twiddles(sam_handle *h, int dir, int n)
+{
+    std::vector<double> td(2 * (size_t)n);
+    std::vector<float> tf(2 * (size_t)n);
+    const long double pi = 3.14159265358979323846264338327950288L;
+    for (int k = 0; k < n; k++) {
+        const long double a = -2.0L * pi * (long double)k / (long double)n;
+        const long double c = cosl(a), s = sinl(a);
+        td[2 * k] = (double)c;
+        td[2 * k + 1] = (double)s;
+        tf[2 * k] = (float)c;
+        tf[2 * k + 1] = (float)s;
+    }
+    HIPCHK(hipMalloc(&h->tw[dir][0], sizeof(double) * 2 * n));
+    HIPCHK(hipMalloc(&h->tw[dir][1], sizeof(float) * 2 * n));
+    HIPCHK(hipMemcpy(h->tw[dir][0], td.data(), sizeof(double) * 2 * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->tw[dir][1], tf.data(), sizeof(float) * 2 * n, hipMemcpyHostToDevice));
+    return SAM_OK;
+}
+
+static size_t tsize(const sam_handle *h) { return h->prec ? sizeof(float) : sizeof(double); }
+
+extern "C" const char *sam_last_error(void) { return g_err; }
+
+extern "C" int sam_destroy(sam_handle *h)
+{
+    if (!h) return SAM_OK;
+    if (h->st) hipStreamSynchronize(h->st);
+    hipFree(h->U[0]);
+    hipFree(h->U[1]);
+    hipFree(h->R);
+    hipFree(h->Zb);
+    hipFree(h->Yb);
+    hipFree(h->Pb.p);
+    hipFree(h->stage0.p);
+    hipFree(h->stage1.p);
+    hipFree(h->sgsw.p);
+    hipFree(h->partials);
+    hipFree(h->specout);
+    hipFree(h->diag);
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 2; j++) hipFree(h->tw[i][j]);
+    if (h->st) hipStreamDestroy(h->st);
+    delete h;
+    return SAM_OK;
+}
+
+static int validate(const sam_params *p)
+{
+    if (!p) return set_err(SAM_ERR_VALUE, "null sam_params");
+    if (!pow2_in(p->nx, 16, 1024) || !pow2_in(p->ny, 16, 1024) || !pow2_in(p->nz, 16, 1024))
+        return set_err(SAM_ERR_UNSUPPORTED, "Nx, Ny, Nz must be powers of two in 16..1024 (got %d %d %d)", p->nx,
+                       p->ny, p->nz);
+    if (p->n_dealias < 0 || p->n_dealias > 2)
+        return set_err(SAM_ERR_VALUE, "n_dealias must be 0, 1 or 2 (got %d)", p->n_dealias);
+    if (p->nrk_max < 1 || p->nrk_max > 3) return set_err(SAM_ERR_VALUE, "nrk_max must be 1..3 (got %d)", p->nrk_max);
+    if (p->iubc_z != 0) return set_err(SAM_ERR_UNSUPPORTED, "only periodic z (iubc_z = 0) is built");
+    if (p->i_strat != 0 && p->i_strat != 1) return set_err(SAM_ERR_VALUE, "i_strat must be 0 or 1");
+    if (p->precision != SAM_PREC_FP64 && p->precision != SAM_PREC_SPLIT)
+        return set_err(SAM_ERR_VALUE, "precision must be SAM_PREC_FP64 or SAM_PREC_SPLIT");
+    if (p->sgs < SAM_SGS_NONE || p->sgs > SAM_SGS_MLP) return set_err(SAM_ERR_VALUE, "unknown sgs model %d", p->sgs);
+    if (!(p->xl > 0) || !(p->yl > 0) || !(p->zl > 0)) return set_err(SAM_ERR_VALUE, "box lengths must be > 0");
+    if (!(p->vis >= 0)) return set_err(SAM_ERR_VALUE, "vis must be >= 0");
+    if (!(p->pr > 0)) return set_err(SAM_ERR_VALUE, "pr must be > 0");
+    if (p->i_strat && !(p->to > 0)) return set_err(SAM_ERR_VALUE, "To must be > 0");
+    if (p->i_cfl == 1 && !(p->cfl0 > 0)) return set_err(SAM_ERR_VALUE, "cfl0 must be > 0");
+    if (p->i_cfl != 1 && !(p->dt0 > 0)) return set_err(SAM_ERR_REQUIRED, "dt0 must be > 0 when i_cfl = 0");
+    if (p->sgs != SAM_SGS_NONE && !(p->pr_t > 0)) return set_err(SAM_ERR_VALUE, "pr_t must be > 0");
+    return SAM_OK;
+}
+
+extern "C" int sam_create(const sam_params *p, sam_handle **out)
+{
+    if (!out) return set_err(SAM_ERR_VALUE, "null handle pointer");
+    *out = nullptr;
+    int rc = validate(p);
+    if (rc) return rc;
+
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        (void)hipGetLastError();
+        return set_err(SAM_ERR_NODEVICE, "no HIP device visible");
+    }
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, dev));
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return set_err(SAM_ERR_NODEVICE, "libsamhip is built for gfx950, device is %s", prop.gcnArchName);
+
+    sam_handle *h = new sam_handle();
+    h->p = *p;
+    h->dev = dev;
+    h->prec = p->precision;
+    Dims &d = h->d;
+    d.nx = p->nx;
+    d.ny = p->ny;
+    d.nz = p->nz;
+    d.kxm = kmax_of(p->nx, p->n_dealias, 1);
+    d.kym = kmax_of(p->ny, p->n_dealias, 0);
+    d.kzm = kmax_of(p->nz, p->n_dealias, 0);
+    d.nxm = d.kxm + 1;
+    d.nym = 2 * d.kym + 1;
+    d.nzm = 2 * d.kzm + 1;
+    d.lu = p->i_strat ? 4 : 3;
+    d.lr = p->i_strat ? 9 : 6;
+    const int ksq = d.kxm * d.kxm + d.kym * d.kym + d.kzm * d.kzm;
+    d.lm = (int)(sqrt((double)ksq) + 0.5) + 1;
+    d.ss = (long long)d.nxm * d.nym * d.nzm;
+    d.zfield = (long long)d.nxm * d.nym * d.nz;
+    d.yfield = (long long)d.nz * d.nxm * d.ny;
+    d.pfield = (long long)d.nx * d.ny * d.nz;
+    const double pi = acos(-1.0);
+    d.wave_x = 2.0 * pi / p->xl;
+    d.wave_y = 2.0 * pi / p->yl;
+    d.wave_z = 2.0 * pi / p->zl;
+
+    StepConsts &k = h->k;
+    memset(&k, 0, sizeof k);
+    if (p->nrk_max == 1) { k.gamma[0] = 1.0; k.adv_cfl_limit = 1.0; }
+    if (p->nrk_max == 2) { k.gamma[0] = 1.0; k.gamma[1] = 0.5; k.zeta[1] = -0.5; k.adv_cfl_limit = 1.0; }
+    if (p->nrk_max == 3) {
+        k.gamma[0] = 8.0 / 15.0; k.gamma[1] = 5.0 / 12.0; k.gamma[2] = 3.0 / 4.0;
+        k.zeta[1] = -17.0 / 60.0; k.zeta[2] = -5.0 / 12.0;
+        k.adv_cfl_limit = sqrt(3.0);
+    }
+    k.vis = p->vis;
+    k.pr_inv = 1.0 / p->pr;
+    k.buoy_fac_x = 0.0;
+    k.buoy_fac_z = p->i_strat ? p->grav / p->to : 0.0;
+    k.lapse = p->i_strat ? p->lapse : 0.0;
+    k.cfl0 = p->cfl0;
+    k.dt0 = p->dt0;
+    k.i_cfl = p->i_cfl;
+    k.strat = p->i_strat;
+
+    auto fail = [&](int code) { sam_destroy(h); return code; };
+    if (hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking) != hipSuccess)
+        return fail(set_err(SAM_ERR_HIP, "hipStreamCreate failed"));
+    const size_t ts = tsize(h);
+    const size_t su = sizeof(double) * 2 * d.ss * d.lu;
+    struct { void **pp; size_t bytes; } allocs[] = {
+        {(void **)&h->U[0], su},
+        {(void **)&h->U[1], su},
+        {(void **)&h->R, sizeof(double) * 2 * d.ss * (d.lu + 1)},
+        {&h->Zb, ts * 2 * d.zfield * d.lr},
+        {&h->Yb, ts * 2 * d.yfield * d.lr},
+        {(void **)&h->partials, sizeof(double) * (size_t)partials_capacity(d)},
+        {(void **)&h->specout, sizeof(double) * (5 * d.lm + 8)},
+        {(void **)&h->diag, sizeof(DevDiag) * 2},
+    };
+    for (auto &a : allocs) {
+        if (hipMalloc(a.pp, a.bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(set_err(SAM_ERR_NOMEM, "hipMalloc(%zu) failed", a.bytes));
+        }
+    }
+    if (hipMemset(h->U[0], 0, su) != hipSuccess || hipMemset(h->R, 0, sizeof(double) * 2 * d.ss * (d.lu + 1)) != hipSuccess)
+        return fail(set_err(SAM_ERR_HIP, "hipMemset failed"));
+    DevDiag dg[2];
+    memset(dg, 0, sizeof dg);
+    dg[0].dt = dg[1].dt = p->dt0;
+    if (hipMemcpy(h->diag, dg, sizeof dg, hipMemcpyHostToDevice) != hipSuccess)
+        return fail(set_err(SAM_ERR_HIP, "hipMemcpy failed"));
+    if ((rc = make_twiddles(h, 0, d.nx)) || (rc = make_twiddles(h, 1, d.ny)) || (rc = make_twiddles(h, 2, d.nz)))
+        return fail(rc);
+    h->sgs_active = p->sgs;
+    *out = h;
+    return SAM_OK;
+}
+
+extern "C" int sam_get_dims(const sam_handle *h, sam_dims *o)
+{
+    if (!h || !o) return set_err(SAM_ERR_VALUE, "null argument");
+    o->kx_max = h->d.kxm;
+    o->ky_max = h->d.kym;
+    o->kz_max = h->d.kzm;
+    o->nx_min = h->d.nxm;
+    o->ny_min = h->d.nym;
+    o->nz_min = h->d.nzm;
+    o->nze = h->d.nz;
+    o->lu = h->d.lu;
+    o->lr = h->d.lr;
+    o->lm = h->d.lm;
+    return SAM_OK;
+}
+
+// host (ldz, ny_min, nx_min, nvar) <-> device packed (nz_min, ...)
+static int h2d_spec(sam_handle *h, double *dst, const double *src, int ldz, int nvar)
+{
+    const Dims &d = h->d;
+    if (ldz < d.nzm) return set_err(SAM_ERR_SIZE, "ldz %d < Nz_min %d", ldz, d.nzm);
+    HIPCHK(hipMemcpy2DAsync(dst, sizeof(double) * 2 * d.nzm, src, sizeof(double) * 2 * ldz, sizeof(double) * 2 * d.nzm,
+                            (size_t)d.nym * d.nxm * nvar, hipMemcpyHostToDevice, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    return SAM_OK;
+}
+
+static int d2h_spec(sam_handle *h, double *dst, const double *src, int ldz, int nvar)
+{
+    const Dims &d = h->d;
+    if (ldz < d.nzm) return set_err(SAM_ERR_SIZE, "ldz %d < Nz_min %d", ldz, d.nzm);
+    if (ldz > d.nzm) memset(dst, 0, sizeof(double) * 2 * (size_t)ldz * d.nym * d.nxm * nvar);
+    HIPCHK(hipMemcpy2DAsync(dst, sizeof(double) * 2 * ldz, src, sizeof(double) * 2 * d.nzm, sizeof(double) * 2 * d.nzm,
+                            (size_t)d.nym * d.nxm * nvar, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    return SAM_OK;
+}
+
+extern "C" int sam_set_state(sam_handle *h, const double *u, int ldz)
+{
+    if (!h || !u) return set_err(SAM_ERR_VALUE, "null argument");
+    h->have_spectra = 0;
+    return h2d_spec(h, h->U[h->cur], u, ldz, h->d.lu);
+}
+
+extern "C" int sam_get_state(sam_handle *h, double *u, int ldz)
+{
+    if (!h || !u) return set_err(SAM_ERR_VALUE, "null argument");
+    return d2h_spec(h, u, h->U[h->cur], ldz, h->d.lu);
+}
+
+extern "C" int sam_set_clock(sam_handle *h, double time, double dt, long long nt)
+{
+    if (!h) return set_err(SAM_ERR_VALUE, "null handle");
+    DevDiag dg;
+    HIPCHK(hipMemcpy(&dg, h->diag, sizeof dg, hipMemcpyDeviceToHost));
+    dg.time = time;
+    dg.dt = dt;
+    dg.nt = nt;
+    HIPCHK(hipMemcpy(h->diag, &dg, sizeof dg, hipMemcpyHostToDevice));
+    return SAM_OK;
+}
+
+static int fetch_diag(sam_handle *h, sam_diag *o)
+{
+    HIPCHK(hipStreamSynchronize(h->st));
+    if (!o) return SAM_OK;
+    DevDiag dg;
+    HIPCHK(hipMemcpy(&dg, h->diag, sizeof dg, hipMemcpyDeviceToHost));
+    memset(o, 0, sizeof *o);
+    o->nt = dg.nt;
+    o->time = dg.time;
+    o->dt = dg.dt;
+    for (int i = 0; i < 4; i++) {
+        o->u_avg[i] = dg.u_avg[i];
+        o->u_var[i] = dg.u_var[i];
+        o->u_max[i] = dg.u_max[i];
+    }
+    o->energy = dg.energy;
+    o->cfl_x = dg.cfl_x;
+    o->cfl_y = dg.cfl_y;
+    o->cfl_z = dg.cfl_z;
+    o->c_delta_sq = dg.c_delta_sq;
+    o->sgs_active = h->sgs_active;
+    return SAM_OK;
+}
+
+// Everything up to (and including) the forward y transform: leaves the
+// x,y-transformed fluxes of the state U[cur] in Zb (Lr fields, Z layout).
+template <typename T> static int fluxes_to_z(sam_handle *h, int stats)
+{
+    const Dims &d = h->d;
+    T *Z = (T *)h->Zb, *Y = (T *)h->Yb;
+    const int pi = h->prec;
+    RCHK(launch_zb<T>(d, h->U[h->cur], d.nzm, d.lu, Z, h->tw[2][pi], h->st));
+    RCHK(launch_yb<T>(d, Z, d.lu, Y, h->tw[1][pi], h->st));
+    if (h->p.sgs == SAM_SGS_NONE) {
+        int nb = 0;
+        RCHK(launch_xrow<T>(d, Y, stats, h->partials, h->diag, h->tw[0][pi], h->st, &nb));
+        if (stats) RCHK(launch_finalize(d, h->k, h->partials, nb, h->diag, 1, h->st));
+    } else {
+        return set_err(SAM_ERR_UNSUPPORTED, "SGS path not built yet");
+    }
+    RCHK(launch_yf<T>(d, Y, d.lr, Z, h->tw[1][pi], h->st));
+    return SAM_OK;
+}
+
+template <typename T> static int substep(sam_handle *h, int m, int spectra)
+{
+    const Dims &d = h->d;
+    int rc = fluxes_to_z<T>(h, m == 0);
+    if (rc) return rc;
+    RCHK(launch_zrhs<T>(d, h->k, (const T *)h->Zb, h->U[h->cur], h->R, h->U[1 - h->cur], h->diag, m, 0,
+                        h->tw[2][h->prec], h->st));
+    if (spectra) {
+        RCHK(launch_spectra(d, h->p.vis, h->U[h->cur], h->R, h->partials, h->specout, h->diag, h->st));
+        h->have_spectra = 1;
+    }
+    h->cur = 1 - h->cur;
+    return SAM_OK;
+}
+
+extern "C" int sam_step(sam_handle *h, int nsteps, int want_spectra, sam_diag *diag)
+{
+    if (!h) return set_err(SAM_ERR_VALUE, "null handle");
+    if (nsteps < 0) return set_err(SAM_ERR_VALUE, "nsteps < 0");
+    for (int n = 0; n < nsteps; n++) {
+        for (int m = 0; m < h->p.nrk_max; m++) {
+            const int spec = want_spectra && n == nsteps - 1 && m == 0;
+            int rc = h->prec ? substep<float>(h, m, spec) : substep<double>(h, m, spec);
+            if (rc) return rc;
+        }
+        RCHK(launch_advance_clock(h->diag, h->st));
+    }
+    return fetch_diag(h, diag);
+}
+
+extern "C" int sam_rhs(sam_handle *h, double *r, int ldz, int want_spectra, sam_diag *diag)
+{
+    if (!h) return set_err(SAM_ERR_VALUE, "null handle");
+    const Dims &d = h->d;
+    int rc = h->prec ? fluxes_to_z<float>(h, 1) : fluxes_to_z<double>(h, 1);
+    if (rc) return rc;
+    if (h->prec)
+        RCHK(launch_zrhs<float>(d, h->k, (const float *)h->Zb, h->U[h->cur], h->R, nullptr, h->diag, 0, 1,
+                                h->tw[2][1], h->st));
+    else
+        RCHK(launch_zrhs<double>(d, h->k, (const double *)h->Zb, h->U[h->cur], h->R, nullptr, h->diag, 0, 1,
+                                 h->tw[2][0], h->st));
+    if (want_spectra) {
+        RCHK(launch_spectra(d, h->p.vis, h->U[h->cur], h->R, h->partials, h->specout, h->diag, h->st));
+        h->have_spectra = 1;
+    }
+    if (r) {
+        rc = d2h_spec(h, r, h->R, ldz, d.lu + 1);
+        if (rc) return rc;
+    }
+    return fetch_diag(h, diag);
+}
+
+extern "C" int sam_get_spectra(sam_handle *h, double *out)
+{
+    if (!h || !out) return set_err(SAM_ERR_VALUE, "null argument");
+    if (!h->have_spectra) return set_err(SAM_ERR_STATE, "no spectra captured (call sam_step/sam_rhs with want_spectra)");
+    HIPCHK(hipStreamSynchronize(h->st));
+    HIPCHK(hipMemcpy(out, h->specout, sizeof(double) * (5 * h->d.lm + 3), hipMemcpyDeviceToHost));
+    return SAM_OK;
+}
+
+// ---------------------------------------------------------------------
+// stage entry points
+// ---------------------------------------------------------------------
+// copy nelem doubles from host into a device buffer of precision T
+template <typename T> static int upload_T(sam_handle *h, T *dst, const double *src, long long nelem)
+{
+    if (sizeof(T) == sizeof(double)) {
+        HIPCHK(hipMemcpyAsync(dst, src, sizeof(double) * nelem, hipMemcpyHostToDevice, h->st));
+        return SAM_OK;
+    }
+    int rc = ensure(h->stage1, sizeof(double) * nelem);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(h->stage1.p, src, sizeof(double) * nelem, hipMemcpyHostToDevice, h->st));
+    RCHK(launch_convert_to<T>((const double *)h->stage1.p, dst, nelem, h->st));
+    return SAM_OK;
+}
+
+template <typename T> static int download_T(sam_handle *h, double *dst, const T *src, long long nelem)
+{
+    if (sizeof(T) == sizeof(double)) {
+        HIPCHK(hipMemcpyAsync(dst, src, sizeof(double) * nelem, hipMemcpyDeviceToHost, h->st));
+    } else {
+        int rc = ensure(h->stage1, sizeof(double) * nelem);
+        if (rc) return rc;
+        RCHK(launch_convert<T>(src, (double *)h->stage1.p, nelem, h->st));
+        HIPCHK(hipMemcpyAsync(dst, h->stage1.p, sizeof(double) * nelem, hipMemcpyDeviceToHost, h->st));
+    }
+    HIPCHK(hipStreamSynchronize(h->st));
+    return SAM_OK;
+}
+
+static int check_nvar(const sam_handle *h, int nvar)
+{
+    if (nvar < 1 || nvar > h->d.lr) return set_err(SAM_ERR_VALUE, "nvar must be 1..%d (got %d)", h->d.lr, nvar);
+    return SAM_OK;
+}
+
+template <typename T> static int zb_stage(sam_handle *h, const double *u, int ldz, double *out, int nvar)
+{
+    const Dims &d = h->d;
+    int rc = ensure(h->stage0, sizeof(double) * 2 * d.ss * nvar);
+    if (rc) return rc;
+    rc = h2d_spec(h, (double *)h->stage0.p, u, ldz, nvar);
+    if (rc) return rc;
+    RCHK(launch_zb<T>(d, (const double *)h->stage0.p, d.nzm, nvar, (T *)h->Zb, h->tw[2][h->prec], h->st));
+    return download_T<T>(h, out, (const T *)h->Zb, 2 * d.zfield * nvar);
+}
+
+extern "C" int sam_z_trans_b(sam_handle *h, const double *u, int ldz, double *out, int nvar)
+{
+    if (!h || !u || !out) return set_err(SAM_ERR_VALUE, "null argument");
+    int rc = check_nvar(h, nvar);
+    if (rc) return rc;
+    return h->prec ? zb_stage<float>(h, u, ldz, out, nvar) : zb_stage<double>(h, u, ldz, out, nvar);
+}
+
+template <typename T> static int xyb_stage(sam_handle *h, const double *in, double *out, int nvar)
+{
+    const Dims &d = h->d;
+    int rc = ensure(h->Pb, sizeof(T) * d.pfield * d.lr);
+    if (rc) return rc;
+    rc = upload_T<T>(h, (T *)h->Zb, in, 2 * d.zfield * nvar);
+    if (rc) return rc;
+    RCHK(launch_yb<T>(d, (const T *)h->Zb, nvar, (T *)h->Yb, h->tw[1][h->prec], h->st));
+    RCHK(launch_xb<T>(d, (const T *)h->Yb, nvar, (T *)h->Pb.p, h->tw[0][h->prec], h->st));
+    return download_T<T>(h, out, (const T *)h->Pb.p, d.pfield * nvar);
+}
+
+extern "C" int sam_xy_trans_b(sam_handle *h, const double *in, double *out, int nvar)
+{
+    if (!h || !in || !out) return set_err(SAM_ERR_VALUE, "null argument");
+    int rc = check_nvar(h, nvar);
+    if (rc) return rc;
+    return h->prec ? xyb_stage<float>(h, in, out, nvar) : xyb_stage<double>(h, in, out, nvar);
+}
+
+template <typename T> static int xyf_stage(sam_handle *h, const double *in, double *out, int nvar)
+{
+    const Dims &d = h->d;
+    int rc = ensure(h->Pb, sizeof(T) * d.pfield * d.lr);
+    if (rc) return rc;
+    rc = upload_T<T>(h, (T *)h->Pb.p, in, d.pfield * nvar);
+    if (rc) return rc;
+    RCHK(launch_xf<T>(d, (const T *)h->Pb.p, nvar, (T *)h->Yb, h->tw[0][h->prec], h->st));
+    RCHK(launch_yf<T>(d, (const T *)h->Yb, nvar, (T *)h->Zb, h->tw[1][h->prec], h->st));
+    return download_T<T>(h, out, (const T *)h->Zb, 2 * d.zfield * nvar);
+}
+
+extern "C" int sam_xy_trans_f(sam_handle *h, const double *in, double *out, int nvar)
+{
+    if (!h || !in || !out) return set_err(SAM_ERR_VALUE, "null argument");
+    int rc = check_nvar(h, nvar);
+    if (rc) return rc;
+    return h->prec ? xyf_stage<float>(h, in, out, nvar) : xyf_stage<double>(h, in, out, nvar);
+}
+
+template <typename T> static int zf_stage(sam_handle *h, const double *in, double *out, int nvar)
+{
+    const Dims &d = h->d;
+    int rc = ensure(h->stage0, sizeof(double) * 2 * d.ss * nvar);
+    if (rc) return rc;
+    rc = upload_T<T>(h, (T *)h->Zb, in, 2 * d.zfield * nvar);
+    if (rc) return rc;
+    RCHK(launch_zf<T>(d, (const T *)h->Zb, nvar, (double *)h->stage0.p, h->tw[2][h->prec], h->st));
+    HIPCHK(hipMemcpyAsync(out, h->stage0.p, sizeof(double) * 2 * d.ss * nvar, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    return SAM_OK;
+}
+
+extern "C" int sam_z_trans_f(sam_handle *h, const double *in, double *out, int nvar)
+{
+    if (!h || !in || !out) return set_err(SAM_ERR_VALUE, "null argument");
+    int rc = check_nvar(h, nvar);
+    if (rc) return rc;
+    return h->prec ? zf_stage<float>(h, in, out, nvar) : zf_stage<double>(h, in, out, nvar);
+}
+
+extern "C" int sam_products(sam_handle *h, const double *phys, double *prod)
+{
+    if (!h || !phys || !prod) return set_err(SAM_ERR_VALUE, "null argument");
+    const Dims &d = h->d;
+    int rc = ensure(h->stage0, sizeof(double) * d.pfield * (d.lu + d.lr));
+    if (rc) return rc;
+    double *P = (double *)h->stage0.p, *Q = P + d.pfield * d.lu;
+    HIPCHK(hipMemcpyAsync(P, phys, sizeof(double) * d.pfield * d.lu, hipMemcpyHostToDevice, h->st));
+    RCHK(launch_products(d, P, 1, Q, h->st));
+    HIPCHK(hipMemcpyAsync(prod, Q, sizeof(double) * d.pfield * d.lr, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    return SAM_OK;
+}
+
+extern "C" int sam_vel_max(sam_handle *h, const double *phys, double dt_in, double *stats)
+{
+    if (!h || !phys || !stats) return set_err(SAM_ERR_VALUE, "null argument");
+    const Dims &d = h->d;
+    int rc = ensure(h->stage0, sizeof(double) * d.pfield * d.lu);
+    if (rc) return rc;
+    DevDiag dg;
+    memset(&dg, 0, sizeof dg);
+    dg.dt = dt_in;
+    DevDiag *sd = h->diag + 1;
+    HIPCHK(hipMemcpyAsync(sd, &dg, sizeof dg, hipMemcpyHostToDevice, h->st));
+    HIPCHK(hipMemcpyAsync(h->stage0.p, phys, sizeof(double) * d.pfield * d.lu, hipMemcpyHostToDevice, h->st));
+    int nb = 0;
+    RCHK(launch_stats_phys(d, h->stage0.p, 1, d.lu, h->partials, sd, h->st, &nb));
+    RCHK(launch_finalize(d, h->k, h->partials, nb, sd, 1, h->st));
+    HIPCHK(hipMemcpyAsync(&dg, sd, sizeof dg, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    for (int i = 0; i < 4; i++) {
+        stats[i] = dg.u_avg[i];
+        stats[4 + i] = dg.u_var[i];
+        stats[8 + i] = dg.u_max[i];
+    }
+    stats[12] = dg.energy;
+    stats[13] = dg.spare[0];
+    stats[14] = dg.spare[1];
+    stats[15] = dg.spare[2];
+    stats[16] = dg.dt;
+    return SAM_OK;
+}
+
+extern "C" int sam_project(sam_handle *h, const double *F, const double *usav, double *r)
+{
+    if (!h || !F || !usav || !r) return set_err(SAM_ERR_VALUE, "null argument");
+    const Dims &d = h->d;
+    const size_t fb = sizeof(double) * 2 * d.ss;
+    int rc = ensure(h->stage0, fb * (d.lr + d.lu + d.lu + 1));
+    if (rc) return rc;
+    double *dF = (double *)h->stage0.p;
+    double *dU = dF + 2 * d.ss * d.lr;
+    double *dR = dU + 2 * d.ss * d.lu;
+    HIPCHK(hipMemcpyAsync(dF, F, fb * d.lr, hipMemcpyHostToDevice, h->st));
+    HIPCHK(hipMemcpyAsync(dU, usav, fb * d.lu, hipMemcpyHostToDevice, h->st));
+    RCHK(launch_project(d, h->k, dF, dU, dR, h->st));
+    HIPCHK(hipMemcpyAsync(r, dR, fb * (d.lu + 1), hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    return SAM_OK;
+}
+
+extern "C" int sam_eddy_visc(sam_handle *h, const double *phys, double *vt, double *c_delta_sq)
+{
+    (void)phys; (void)vt; (void)c_delta_sq;
+    if (!h) return set_err(SAM_ERR_VALUE, "null handle");
+    return set_err(SAM_ERR_UNSUPPORTED, "SGS path not built yet");
+}
+
+extern "C" int sam_test_filter(sam_handle *h, const double *in, double *out, int nvar)
+{
+    (void)in; (void)out; (void)nvar;
+    if (!h) return set_err(SAM_ERR_VALUE, "null handle");
+    return set_err(SAM_ERR_UNSUPPORTED, "SGS path not built yet");
+}

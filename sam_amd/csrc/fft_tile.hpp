@@ -1,0 +1,229 @@
+// fft_tile.hpp -- batched power-of-two complex FFTs for gfx950, one tile of
+// C columns per workgroup, Stockham autosort with radix-8/4/2 passes.
+//
+// Each thread owns EPT (8 or 16) elements of one column per pass and keeps
+// them in VGPRs; passes exchange data through an LDS tile of C*N complex
+// elements.  The LDS address of element p of column c is XOR-swizzled in
+// 128-byte groups so that the strided Stockham reads/writes and the
+// column-interleaved cooperative loads are free of bank conflicts without
+// padding (the tile stays exactly C*N elements).
+//
+// Sign convention (FFTPACK, fft.f:57-59,171-173): forward = exp(-i..),
+// backward = exp(+i..), both unnormalised.  Twiddles come from a table
+// tw[k] = exp(-2 pi i k/N) built on the host in long double.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace samhip {
+
+template <typename T> struct cpx { T x, y; };
+
+template <typename T> __device__ __forceinline__ cpx<T> operator+(cpx<T> a, cpx<T> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename T> __device__ __forceinline__ cpx<T> operator-(cpx<T> a, cpx<T> b) { return {a.x - b.x, a.y - b.y}; }
+template <typename T> __device__ __forceinline__ cpx<T> cmul(cpx<T> a, cpx<T> b)
+{
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename T> __device__ __forceinline__ cpx<T> conjg(cpx<T> a) { return {a.x, -a.y}; }
+template <typename T> __device__ __forceinline__ cpx<T> scal(cpx<T> a, T s) { return {a.x * s, a.y * s}; }
+// a * (S i), S = +1 or -1
+template <int S, typename T> __device__ __forceinline__ cpx<T> mul_si(cpx<T> a)
+{
+    if constexpr (S > 0) return {-a.y, a.x};
+    else return {a.y, -a.x};
+}
+
+__host__ __device__ constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n >> 1); }
+
+// Radix schedule, symmetric where possible so that the last pass of one
+// transform leaves data exactly where the first pass of the next reads it.
+template <int N> struct Sched {
+    static constexpr int L = ilog2(N);
+    static_assert((1 << L) == N && N >= 16 && N <= 4096, "power of two 16..4096");
+    static constexpr int P = (L == 4) ? 2 : (L == 5) ? 2 : (L == 6) ? 2 : (L == 7) ? 3 : (L == 8) ? 3 : (L == 9) ? 3
+                           : (L == 10) ? 4 : (L == 11) ? 4 : 4;
+    __host__ __device__ static constexpr int radix(int p)
+    {
+        return (L == 4) ? 4
+             : (L == 5) ? (p == 0 ? 8 : 4)
+             : (L == 6) ? 8
+             : (L == 7) ? (p == 1 ? 2 : 8)
+             : (L == 8) ? (p == 1 ? 4 : 8)
+             : (L == 9) ? 8
+             : (L == 10) ? ((p == 1 || p == 2) ? 4 : 8)
+             : (L == 11) ? (p == 1 ? 4 : 8)
+             : 8; /* L == 12 */
+    }
+    __host__ __device__ static constexpr int ns(int p) { return p == 0 ? 1 : ns(p - 1) * radix(p - 1); }
+};
+
+template <typename T> struct Swz;
+template <> struct Swz<double> { static constexpr int SW = 8, LOG = 3; };   // 16-byte elements
+template <> struct Swz<float> { static constexpr int SW = 16, LOG = 4; };   // 8-byte elements
+
+// LDS address of element p of column c in a tile with column length N.
+template <typename T, int N>
+__device__ __forceinline__ int lds_idx(int c, int p)
+{
+    constexpr int SW = Swz<T>::SW, LOG = Swz<T>::LOG;
+    return c * N + (p ^ (((p >> LOG) ^ c) & (SW - 1)));
+}
+
+template <int S, typename T> __device__ __forceinline__ void dft2(cpx<T> &a, cpx<T> &b)
+{
+    cpx<T> t = a - b;
+    a = a + b;
+    b = t;
+}
+
+template <int S, typename T> __device__ __forceinline__ void dft4(cpx<T> &a, cpx<T> &b, cpx<T> &c, cpx<T> &d)
+{
+    cpx<T> t0 = a + c, t1 = a - c, t2 = b + d, t3 = mul_si<S>(b - d);
+    a = t0 + t2;
+    c = t0 - t2;
+    b = t1 + t3;
+    d = t1 - t3;
+}
+
+template <int S, typename T> __device__ __forceinline__ void dft8(cpx<T> *v)
+{
+    // even/odd split: E = DFT4(v0,v2,v4,v6), O = DFT4(v1,v3,v5,v7)
+    cpx<T> e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+    cpx<T> o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+    dft4<S>(e0, e1, e2, e3);
+    dft4<S>(o0, o1, o2, o3);
+    const T h = (T)0.70710678118654752440084436210484903928483593768847;
+    // w^1 = h(1 + S i), w^2 = S i, w^3 = h(-1 + S i)
+    cpx<T> w1o1 = {h * (o1.x - (T)S * o1.y), h * (o1.y + (T)S * o1.x)};
+    cpx<T> w2o2 = mul_si<S>(o2);
+    cpx<T> w3o3 = {h * (-o3.x - (T)S * o3.y), h * (-o3.y + (T)S * o3.x)};
+    v[0] = e0 + o0;
+    v[4] = e0 - o0;
+    v[1] = e1 + w1o1;
+    v[5] = e1 - w1o1;
+    v[2] = e2 + w2o2;
+    v[6] = e2 - w2o2;
+    v[3] = e3 + w3o3;
+    v[7] = e3 - w3o3;
+}
+
+// Butterfly of radix R on v[0..R-1] (inputs in stride order r)
+template <int R, int S, typename T> __device__ __forceinline__ void dftR(cpx<T> *v)
+{
+    if constexpr (R == 8) dft8<S>(v);
+    else if constexpr (R == 4) dft4<S>(v[0], v[1], v[2], v[3]);
+    else dft2<S>(v[0], v[1]);
+}
+
+// Tile FFT engine.  N: length, EPT: elements per thread (8 or 16),
+// threads per column TPC = N/EPT.  Thread (c, j0) owns butterflies
+// b_m = j0 + m*TPC, m < EPT/R, in each pass.  Register slot s = m*R + r
+// holds element b_m + r*N/R at the start of a pass.
+template <typename T, int N, int EPT> struct Tile {
+    static_assert(EPT == 8 || EPT == 16, "EPT");
+    static constexpr int TPC = N / EPT;
+    using SC = Sched<N>;
+
+    // k index (element j0 + k*TPC) held in slot s at the start of pass p
+    __host__ __device__ static constexpr int kin(int p, int s)
+    {
+        return (s / SC::radix(p)) + (s % SC::radix(p)) * (EPT / SC::radix(p));
+    }
+    // position of slot s after the last pass: j0 + kout(s)*TPC
+    __host__ __device__ static constexpr int kout(int s) { return kin(SC::P - 1, s); }
+
+    // load the pass-p inputs from LDS
+    template <int p> __device__ static __forceinline__ void lds_read(const cpx<T> *lds, int c, int j0, cpx<T> *v)
+    {
+        constexpr int R = SC::radix(p);
+#pragma unroll
+        for (int s = 0; s < EPT; s++) {
+            const int m = s / R, r = s % R;
+            const int b = j0 + m * TPC;
+            v[s] = lds[lds_idx<T, N>(c, b + r * (N / R))];
+        }
+    }
+
+    // twiddle + butterflies of pass p
+    template <int p, int S> __device__ static __forceinline__ void compute(const cpx<T> *__restrict__ tw, int j0, cpx<T> *v)
+    {
+        constexpr int R = SC::radix(p);
+        constexpr int Ns = SC::ns(p);
+#pragma unroll
+        for (int m = 0; m < EPT / R; m++) {
+            if constexpr (Ns > 1) {
+                const int b = j0 + m * TPC;
+                const int jl = b & (Ns - 1);
+                constexpr int unit = N / (Ns * R);
+#pragma unroll
+                for (int r = 1; r < R; r++) {
+                    cpx<T> w = tw[r * jl * unit];
+                    if constexpr (S > 0) w.y = -w.y;
+                    v[m * R + r] = cmul(v[m * R + r], w);
+                }
+            }
+            dftR<R, S>(v + m * R);
+        }
+    }
+
+    // store the pass-p outputs to LDS
+    template <int p> __device__ static __forceinline__ void lds_write(cpx<T> *lds, int c, int j0, const cpx<T> *v)
+    {
+        constexpr int R = SC::radix(p);
+        constexpr int Ns = SC::ns(p);
+#pragma unroll
+        for (int m = 0; m < EPT / R; m++) {
+            const int b = j0 + m * TPC;
+            const int idxD = (b / Ns) * Ns * R + (b & (Ns - 1));
+#pragma unroll
+            for (int r = 0; r < R; r++) lds[lds_idx<T, N>(c, idxD + r * Ns)] = v[m * R + r];
+        }
+    }
+
+    // passes p0..P-1 starting from registers already holding pass-p0
+    // inputs; the final pass leaves its outputs in v (positions kout).
+    // Requires a __syncthreads() before the call if other threads wrote
+    // the LDS tile region this thread's column uses.
+    template <int S, int p = 0>
+    __device__ static __forceinline__ void run_regs(cpx<T> *lds, const cpx<T> *__restrict__ tw, int c, int j0, cpx<T> *v)
+    {
+        compute<p, S>(tw, j0, v);
+        if constexpr (p + 1 < SC::P) {
+            lds_write<p>(lds, c, j0, v);
+            __syncthreads();
+            lds_read<p + 1>(lds, c, j0, v);
+            __syncthreads();
+            run_regs<S, p + 1>(lds, tw, c, j0, v);
+        }
+    }
+
+    // LDS -> registers (pass-0 inputs) -> FFT -> registers (final outputs)
+    template <int S>
+    __device__ static __forceinline__ void run_from_lds(cpx<T> *lds, const cpx<T> *__restrict__ tw, int c, int j0, cpx<T> *v)
+    {
+        lds_read<0>(lds, c, j0, v);
+        __syncthreads();
+        run_regs<S, 0>(lds, tw, c, j0, v);
+    }
+
+    // write the final outputs (from run_*) to LDS in natural order
+    __device__ static __forceinline__ void store_natural(cpx<T> *lds, int c, int j0, const cpx<T> *v)
+    {
+#pragma unroll
+        for (int s = 0; s < EPT; s++) lds[lds_idx<T, N>(c, j0 + kout(s) * TPC)] = v[s];
+    }
+
+    // reorder final outputs (slots by kout) into pass-0 input slots (kin(0,.))
+    __device__ static __forceinline__ void out_to_in(const cpx<T> *vo, cpx<T> *vi)
+    {
+#pragma unroll
+        for (int s = 0; s < EPT; s++) {
+#pragma unroll
+            for (int t = 0; t < EPT; t++)
+                if (kout(t) == kin(0, s)) vi[s] = vo[t];
+        }
+    }
+};
+
+} // namespace samhip

@@ -1,0 +1,97 @@
+// internal.hpp -- shared definitions of libsamhip (not part of the C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "fft_tile.hpp"
+#include "../../include/samhip.h"
+
+namespace samhip {
+
+// Grid description shared by every kernel (init.f:47-85, fft_init.f:52-72).
+struct Dims {
+    int nx, ny, nz;
+    int kxm, kym, kzm;
+    int nxm, nym, nzm;
+    int lu, lr, lm;
+    long long ss;            // Nx_min*Ny_min*Nz_min, one spectral field
+    long long zfield;        // Nx_min*Ny_min*Nz, one field in the Z layout
+    long long yfield;        // Nz*Nx_min*Ny, one field in the Y layout
+    long long pfield;        // Nx*Ny*Nz, one physical field
+    double wave_x, wave_y, wave_z;
+};
+
+// Device-resident diagnostics and clock (vel_max.f outputs, dt, time).
+struct DevDiag {
+    double time, dt;
+    double u_avg[4], u_var[4], u_max[4];
+    double energy;
+    double cfl_x, cfl_y, cfl_z;
+    double c_delta_sq;
+    long long nt;
+    unsigned long long umax_bits[4];   // running max |u| as IEEE bits
+    unsigned long long divmax_bits;
+    double spare[6];
+};
+
+// Scalars the RK epilogue needs (init.f:97-148, rhs.f, sam.f:293-314).
+struct StepConsts {
+    double gamma[4], zeta[4];
+    double vis, pr_inv, buoy_fac_x, buoy_fac_z, lapse;
+    double cfl0, adv_cfl_limit, dt0;
+    int i_cfl, strat;
+};
+
+/* Device buffer layouts (complex elements, first index fastest):
+ *   S : (Nz_min, Ny_min, Nx_min, F)   spectral state / RHS, FP64
+ *   Z : (Nz, Ny_min, Nx_min, F)       z in physical space, x,y spectral
+ *   Y : (Ny, Nx_min, Nz, F)           y,z physical, x spectral (kx <= kx_max)
+ *   P : (Nx, Ny, Nz, F) real          physical space
+ */
+
+// launchers (kernels.hip); T = float or double; all asynchronous on `st`.
+template <typename T> int launch_zb(const Dims &d, const double *S, int ldz, int nvar, T *Z, const void *twz, hipStream_t st);
+template <typename T> int launch_yb(const Dims &d, const T *Z, int nvar, T *Y, const void *twy, hipStream_t st);
+template <typename T> int launch_yf(const Dims &d, const T *Y, int nvar, T *Z, const void *twy, hipStream_t st);
+template <typename T> int launch_xb(const Dims &d, const T *Y, int nvar, T *P, const void *twx, hipStream_t st);
+template <typename T> int launch_xf(const Dims &d, const T *P, int nvar, T *Y, const void *twx, hipStream_t st);
+template <typename T> int launch_zf(const Dims &d, const T *Z, int nvar, double *S, const void *twz, hipStream_t st);
+// fused x-row kernel: inverse x, products, forward x (+ vel_max partials)
+template <typename T> int launch_xrow(const Dims &d, T *Y, int want_stats, double *partials, DevDiag *diag,
+                                      const void *twx, hipStream_t st, int *nblocks_out);
+// fused z-forward + projection + RK/IF update (mode 0) or RHS only (mode 1)
+template <typename T> int launch_zrhs(const Dims &d, const StepConsts &k, const T *Z, const double *U, double *R,
+                                      double *Unew, const DevDiag *diag, int m, int mode, const void *twz,
+                                      hipStream_t st);
+int launch_stats_phys(const Dims &d, const void *P, int is_double, int nvar, double *partials, DevDiag *diag,
+                      hipStream_t st, int *nblocks_out);
+int launch_finalize(const Dims &d, const StepConsts &k, const double *partials, int nblocks, DevDiag *diag,
+                    int update_dt, hipStream_t st);
+int launch_advance_clock(DevDiag *diag, hipStream_t st);
+int launch_project(const Dims &d, const StepConsts &k, const double *F, const double *usav, double *r, hipStream_t st);
+int launch_products(const Dims &d, const void *P, int is_double, void *Q, hipStream_t st);
+int launch_spectra(const Dims &d, double vis, const double *U, const double *R, double *partials, double *out,
+                   DevDiag *diag, hipStream_t st);
+int launch_strided_copy(const double *src, double *dst, long long n0, long long n1, long long n2, long long n3,
+                        long long s0, long long s1, long long s2, long long s3, long long d0, long long d1,
+                        long long d2, long long d3, int cplx, hipStream_t st);
+template <typename T> int launch_convert(const T *src, double *dst, long long n, hipStream_t st);
+template <typename T> int launch_convert_to(const double *src, T *dst, long long n, hipStream_t st);
+
+// SGS (sgs.hip)
+template <typename T> int launch_strain_smag(const Dims &d, const T *P, T *smag, T *S6, double dxi, double dyi,
+                                             double dzi, hipStream_t st);
+template <typename T> int launch_filter(const Dims &d, const T *in, T *out, T *tmp, int nvar, hipStream_t st);
+template <typename T> int launch_dm_fields(const Dims &d, const T *P, const T *smag, const T *S6, T *MS, T *UU,
+                                           hipStream_t st);
+template <typename T> int launch_dm_reduce(const Dims &d, const T *uh, const T *sh, const T *mh, const T *lh,
+                                           double *partials, hipStream_t st, int *nblocks_out);
+int launch_dm_coef(const double *partials, int nblocks, double c_fixed, int dynamic, DevDiag *diag, hipStream_t st);
+template <typename T> int launch_sgs_flux(const Dims &d, const T *P, const T *smag, const DevDiag *diag, T *Q,
+                                          double dxi, double dyi, double dzi, double pr_t_inv, hipStream_t st);
+
+// max number of per-block partial rows the step's reductions may produce
+long long partials_capacity(const Dims &d);
+
+} // namespace samhip

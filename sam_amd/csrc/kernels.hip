@@ -1,0 +1,925 @@
+// kernels.hip -- transform, fused physical-space and fused spectral kernels
+// of the RK3 step for gfx950.
+//
+// Dataflow of one RK substep (sam.f:169-314, rhs.f):
+//   k_zb    S(u)  -> Z        z inverse FFT with z zero-pad   (z_trans_b.f:31-52)
+//   k_yb    Z     -> Y        y zero-pad + y inverse FFT      (x2z_decomp.f, xy_trans_b.f:14-37)
+//   k_xrow  Y(Lu) -> Y(Lr)    x C2R, vel_max partials, products, x R2C, x1/(NxNy)
+//                             (xy_trans_b.f:39-60, vel_max.f, rhs.f:32-61, xy_trans_f.f:26-37)
+//   k_yf    Y     -> Z        y forward FFT + y truncation     (xy_trans_f.f:39-54, z2x_decomp.f)
+//   k_zrhs  Z, u, r -> r, u'  z forward FFT, x1/Nz, truncation, divergence, buoyancy,
+//                             projection, integrating-factor RK update
+//                             (z_trans_f.f:21-31, rhs.f:108-142, sam.f:210-221,293-314)
+// The unfused k_xb / k_xf / k_zf serve the stage entry points and the LES path.
+#include "internal.hpp"
+
+namespace samhip {
+
+#define SAM_SWITCH_N(n, ...)                                     \
+    switch (n) {                                                 \
+    case 16: { constexpr int N = 16; __VA_ARGS__; } break;       \
+    case 32: { constexpr int N = 32; __VA_ARGS__; } break;       \
+    case 64: { constexpr int N = 64; __VA_ARGS__; } break;       \
+    case 128: { constexpr int N = 128; __VA_ARGS__; } break;     \
+    case 256: { constexpr int N = 256; __VA_ARGS__; } break;     \
+    case 512: { constexpr int N = 512; __VA_ARGS__; } break;     \
+    case 1024: { constexpr int N = 1024; __VA_ARGS__; } break;   \
+    default: return SAM_ERR_UNSUPPORTED;                         \
+    }
+
+static inline int launched()
+{
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? SAM_OK : SAM_ERR_HIP;
+}
+
+// columns whose strided side is read/written in 128-byte rows
+template <typename T> struct Row128 { static constexpr int C = 128 / (int)sizeof(cpx<T>); };
+
+template <typename T> struct Pair { cpx<T> a, b; };
+
+// ---------------------------------------------------------------------
+// k_zb: z inverse transform of contiguous spectral columns
+// ---------------------------------------------------------------------
+template <typename T, int N, int C>
+__global__ __launch_bounds__(C * N / 8) void k_zb(const double *__restrict__ S, int ldz, long long ncol,
+                                                   T *__restrict__ Zr, const cpx<T> *__restrict__ tw, int kzm, int nzm)
+{
+    using TL = Tile<T, N, 8>;
+    __shared__ cpx<T> lds[C * N];
+    const cpx<double> *Sc = reinterpret_cast<const cpx<double> *>(S);
+    cpx<T> *Z = reinterpret_cast<cpx<T> *>(Zr);
+    const int c = threadIdx.x / TL::TPC, j0 = threadIdx.x % TL::TPC;
+    const long long col = (long long)blockIdx.x * C + c;
+    const bool ok = col < ncol;
+    cpx<T> v[8];
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+        const int p = j0 + TL::kin(0, s) * TL::TPC;
+        const int src = (p <= kzm) ? p : ((p >= N - kzm) ? p - N + nzm : -1);
+        cpx<T> x = {(T)0, (T)0};
+        if (ok && src >= 0) {
+            const cpx<double> dd = Sc[col * ldz + src];
+            x = {(T)dd.x, (T)dd.y};
+        }
+        v[s] = x;
+    }
+    TL::template run_regs<+1>(lds, tw, c, j0, v);
+    if (ok) {
+#pragma unroll
+        for (int s = 0; s < 8; s++) Z[col * N + j0 + TL::kout(s) * TL::TPC] = v[s];
+    }
+}
+
+template <typename T>
+int launch_zb(const Dims &d, const double *S, int ldz, int nvar, T *Z, const void *twz, hipStream_t st)
+{
+    const long long ncol = (long long)nvar * d.nxm * d.nym;
+    constexpr int C = 4;
+    SAM_SWITCH_N(d.nz, {
+        const unsigned nb = (unsigned)((ncol + C - 1) / C);
+        hipLaunchKernelGGL((k_zb<T, N, C>), dim3(nb), dim3(C * N / 8), 0, st, S, ldz, ncol, Z,
+                           (const cpx<T> *)twz, d.kzm, d.nzm);
+    });
+    return launched();
+}
+
+// ---------------------------------------------------------------------
+// k_yb: y zero-pad + y inverse transform, Z -> Y.  Tile = C consecutive z
+// of one (field, kx) slab; loads 128-byte rows, stores contiguous columns.
+// ---------------------------------------------------------------------
+template <typename T, int N, int C, int EPT>
+__global__ __launch_bounds__(C * N / EPT) void k_yb(const T *__restrict__ Zr, T *__restrict__ Yr,
+                                                     const cpx<T> *__restrict__ tw, Dims d)
+{
+    using TL = Tile<T, N, EPT>;
+    constexpr int NT = C * N / EPT;
+    __shared__ cpx<T> lds[C * N];
+    const int nzt = d.nz / C;
+    int b = blockIdx.x;
+    const int zt = b % nzt;
+    b /= nzt;
+    const int ix = b % d.nxm, f = b / d.nxm;
+    const int z0 = zt * C;
+    const cpx<T> *Z = reinterpret_cast<const cpx<T> *>(Zr) + ((long long)f * d.nxm + ix) * d.nym * d.nz + z0;
+    for (int e = threadIdx.x; e < C * N; e += NT) {
+        const int p = e / C, c = e % C;
+        const int iy = (p <= d.kym) ? p : ((p >= N - d.kym) ? p - N + d.nym : -1);
+        cpx<T> x = {(T)0, (T)0};
+        if (iy >= 0) x = Z[(long long)iy * d.nz + c];
+        lds[lds_idx<T, N>(c, p)] = x;
+    }
+    __syncthreads();
+    const int c = threadIdx.x / TL::TPC, j0 = threadIdx.x % TL::TPC;
+    cpx<T> v[EPT];
+    TL::template run_from_lds<+1>(lds, tw, c, j0, v);
+    cpx<T> *Y = reinterpret_cast<cpx<T> *>(Yr) + (((long long)f * d.nz + z0 + c) * d.nxm + ix) * N;
+#pragma unroll
+    for (int s = 0; s < EPT; s++) Y[j0 + TL::kout(s) * TL::TPC] = v[s];
+}
+
+template <typename T>
+int launch_yb(const Dims &d, const T *Z, int nvar, T *Y, const void *twy, hipStream_t st)
+{
+    constexpr int C = Row128<T>::C;
+    SAM_SWITCH_N(d.ny, {
+        const unsigned nb = (unsigned)((long long)nvar * d.nxm * (d.nz / C));
+        if constexpr (C * N / 8 <= 512)
+            hipLaunchKernelGGL((k_yb<T, N, C, 8>), dim3(nb), dim3(C * N / 8), 0, st, Z, Y, (const cpx<T> *)twy, d);
+        else
+            hipLaunchKernelGGL((k_yb<T, N, C, 16>), dim3(nb), dim3(C * N / 16), 0, st, Z, Y, (const cpx<T> *)twy, d);
+    });
+    return launched();
+}
+
+// ---------------------------------------------------------------------
+// k_yf: y forward transform + truncation, Y -> Z (rows of C z values).
+// ---------------------------------------------------------------------
+template <typename T, int N, int C, int EPT>
+__global__ __launch_bounds__(C * N / EPT) void k_yf(const T *__restrict__ Yr, T *__restrict__ Zr,
+                                                     const cpx<T> *__restrict__ tw, Dims d)
+{
+    using TL = Tile<T, N, EPT>;
+    constexpr int NT = C * N / EPT;
+    __shared__ cpx<T> lds[C * N];
+    const int nzt = d.nz / C;
+    int b = blockIdx.x;
+    const int zt = b % nzt;
+    b /= nzt;
+    const int ix = b % d.nxm, q = b / d.nxm;
+    const int z0 = zt * C;
+    const int c = threadIdx.x / TL::TPC, j0 = threadIdx.x % TL::TPC;
+    const cpx<T> *Y = reinterpret_cast<const cpx<T> *>(Yr) + (((long long)q * d.nz + z0 + c) * d.nxm + ix) * N;
+    cpx<T> v[EPT];
+#pragma unroll
+    for (int s = 0; s < EPT; s++) v[s] = Y[j0 + TL::kin(0, s) * TL::TPC];
+    TL::template run_regs<-1>(lds, tw, c, j0, v);
+    TL::store_natural(lds, c, j0, v);
+    __syncthreads();
+    cpx<T> *Z = reinterpret_cast<cpx<T> *>(Zr) + ((long long)q * d.nxm + ix) * d.nym * d.nz + z0;
+    for (int e = threadIdx.x; e < C * d.nym; e += NT) {
+        const int iy = e / C, cc = e % C;
+        const int p = (iy <= d.kym) ? iy : iy + N - d.nym;
+        Z[(long long)iy * d.nz + cc] = lds[lds_idx<T, N>(cc, p)];
+    }
+}
+
+template <typename T>
+int launch_yf(const Dims &d, const T *Y, int nvar, T *Z, const void *twy, hipStream_t st)
+{
+    constexpr int C = Row128<T>::C;
+    SAM_SWITCH_N(d.ny, {
+        const unsigned nb = (unsigned)((long long)nvar * d.nxm * (d.nz / C));
+        if constexpr (C * N / 8 <= 512)
+            hipLaunchKernelGGL((k_yf<T, N, C, 8>), dim3(nb), dim3(C * N / 8), 0, st, Y, Z, (const cpx<T> *)twy, d);
+        else
+            hipLaunchKernelGGL((k_yf<T, N, C, 16>), dim3(nb), dim3(C * N / 16), 0, st, Y, Z, (const cpx<T> *)twy, d);
+    });
+    return launched();
+}
+
+// ---------------------------------------------------------------------
+// x rows.  Two real rows (y, y+1) travel as one complex FFT column:
+// inverse:  Z_k = X_k + i Y_k, Z_{N-k} = conj(X_k) + i conj(Y_k)  -> z = x + i y
+// forward:  X_k = (Z_k + conj Z_{N-k})/2, Y_k = -i (Z_k - conj Z_{N-k})/2
+// kx = 0 keeps only the real part, as xy_trans_b.f:28-31 does.
+// ---------------------------------------------------------------------
+template <typename T, int N, int CP, int NT>
+__device__ __forceinline__ void xrow_load_pairs(cpx<T> *lds, const cpx<T> *__restrict__ Yf, int ny, int y0, int kxm)
+{
+    for (int e = threadIdx.x; e < CP * (kxm + 1); e += NT) {
+        const int k = e / CP, c = e % CP;
+        Pair<T> pr = *reinterpret_cast<const Pair<T> *>(Yf + (long long)k * ny + y0 + 2 * c);
+        cpx<T> X = pr.a, Yv = pr.b;
+        if (k == 0) { X.y = (T)0; Yv.y = (T)0; }
+        lds[lds_idx<T, N>(c, k)] = {X.x - Yv.y, X.y + Yv.x};
+        if (k > 0) lds[lds_idx<T, N>(c, N - k)] = {X.x + Yv.y, Yv.x - X.y};
+    }
+    const int nzero = N - 2 * kxm - 1;
+    for (int e = threadIdx.x; e < CP * nzero; e += NT) {
+        const int p = kxm + 1 + e / CP, c = e % CP;
+        lds[lds_idx<T, N>(c, p)] = {(T)0, (T)0};
+    }
+}
+
+template <typename T, int N, int CP, int NT>
+__device__ __forceinline__ void xrow_store_pairs(const cpx<T> *lds, cpx<T> *__restrict__ Yq, int ny, int y0, int kxm,
+                                                 T h)
+{
+    for (int e = threadIdx.x; e < CP * (kxm + 1); e += NT) {
+        const int k = e / CP, c = e % CP;
+        const cpx<T> zk = lds[lds_idx<T, N>(c, k)];
+        const cpx<T> zm = lds[lds_idx<T, N>(c, (N - k) & (N - 1))];
+        Pair<T> pr;
+        pr.a = {(zk.x + zm.x) * h, (zk.y - zm.y) * h};
+        pr.b = {(zk.y + zm.y) * h, (zm.x - zk.x) * h};
+        *reinterpret_cast<Pair<T> *>(Yq + (long long)k * ny + y0 + 2 * c) = pr;
+    }
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// product q of rhs.f:46-60: (uu, uv, vv, uw, vw, ww, uT, vT, wT)
+__host__ __device__ constexpr int prod_i(int q) { return q == 0 ? 0 : q == 1 ? 0 : q == 2 ? 1 : q == 3 ? 0 : q == 4 ? 1 : q == 5 ? 2 : q == 6 ? 0 : q == 7 ? 1 : 2; }
+__host__ __device__ constexpr int prod_j(int q) { return q == 0 ? 0 : q == 1 ? 1 : q == 2 ? 1 : q == 3 ? 2 : q == 4 ? 2 : q == 5 ? 2 : 3; }
+
+template <typename T, int N, int CP, int LU, int LR>
+__global__ __launch_bounds__(CP * N / 8) void k_xrow(T *__restrict__ Yr, Dims d, const cpx<T> *__restrict__ tw,
+                                                      int want_stats, double *__restrict__ partials,
+                                                      DevDiag *__restrict__ diag)
+{
+    using TL = Tile<T, N, 8>;
+    constexpr int NT = CP * N / 8;
+    constexpr int NW = (NT + 63) / 64;
+    __shared__ cpx<T> lds[CP * N];
+    __shared__ double red[NW][2 * LU];
+    const int nyt = d.ny / (2 * CP);
+    const int yt = blockIdx.x % nyt, z = blockIdx.x / nyt;
+    const int y0 = yt * 2 * CP;
+    const int c = threadIdx.x / TL::TPC, j0 = threadIdx.x % TL::TPC;
+    cpx<T> *Y = reinterpret_cast<cpx<T> *>(Yr);
+    const long long fstride = d.yfield;
+    const long long zoff = (long long)z * d.nxm * d.ny;
+
+    cpx<T> uf[LU][8];
+#pragma unroll
+    for (int f = 0; f < LU; f++) {
+        xrow_load_pairs<T, N, CP, NT>(lds, Y + f * fstride + zoff, d.ny, y0, d.kxm);
+        __syncthreads();
+        TL::template run_from_lds<+1>(lds, tw, c, j0, uf[f]);
+    }
+
+    if (want_stats) {
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+        for (int f = 0; f < LU; f++) {
+            double s = 0, s2 = 0, mx = 0;
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const double a = (double)uf[f][k].x, b = (double)uf[f][k].y;
+                s += a + b;
+                s2 += a * a + b * b;
+                mx = fmax(mx, fmax(fabs(a), fabs(b)));
+            }
+            s = wave_sum(s);
+            s2 = wave_sum(s2);
+            mx = wave_max(mx);
+            if (lane == 0) {
+                red[w][f] = s;
+                red[w][LU + f] = s2;
+                atomicMax(&diag->umax_bits[f], (unsigned long long)__double_as_longlong(mx));
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x < 2 * LU) {
+            double t = 0;
+            for (int i = 0; i < NW; i++) t += red[i][threadIdx.x];
+            partials[(long long)blockIdx.x * 2 * LU + threadIdx.x] = t;
+        }
+    }
+
+    const T h = (T)(0.5 / ((double)d.nx * (double)d.ny));
+#pragma unroll
+    for (int q = 0; q < LR; q++) {
+        constexpr int dummy = 0;
+        (void)dummy;
+        cpx<T> po[8], pi[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const cpx<T> a = uf[prod_i(q)][k], b = uf[prod_j(q)][k];
+            po[k] = {a.x * b.x, a.y * b.y};
+        }
+        TL::out_to_in(po, pi);
+        __syncthreads();
+        TL::template run_regs<-1>(lds, tw, c, j0, pi);
+        TL::store_natural(lds, c, j0, pi);
+        __syncthreads();
+        xrow_store_pairs<T, N, CP, NT>(lds, Y + q * fstride + zoff, d.ny, y0, d.kxm, h);
+    }
+}
+
+template <typename T> struct XCP { static constexpr int CP = Row128<T>::C / 2; };
+
+template <typename T>
+int launch_xrow(const Dims &d, T *Y, int want_stats, double *partials, DevDiag *diag, const void *twx, hipStream_t st,
+                int *nblocks_out)
+{
+    constexpr int CP = XCP<T>::CP;
+    const unsigned nb = (unsigned)((long long)d.nz * (d.ny / (2 * CP)));
+    if (nblocks_out) *nblocks_out = (int)nb;
+    if (d.ny % (2 * CP)) return SAM_ERR_UNSUPPORTED;
+    SAM_SWITCH_N(d.nx, {
+        if constexpr (CP * N / 8 <= 1024) {
+            if (d.lu == 4)
+                hipLaunchKernelGGL((k_xrow<T, N, CP, 4, 9>), dim3(nb), dim3(CP * N / 8), 0, st, Y, d,
+                                   (const cpx<T> *)twx, want_stats, partials, diag);
+            else
+                hipLaunchKernelGGL((k_xrow<T, N, CP, 3, 6>), dim3(nb), dim3(CP * N / 8), 0, st, Y, d,
+                                   (const cpx<T> *)twx, want_stats, partials, diag);
+        } else {
+            return SAM_ERR_UNSUPPORTED;
+        }
+    });
+    return launched();
+}
+
+// k_xb: Y -> physical rows (Nx,Ny,Nz,F)
+template <typename T, int N, int CP>
+__global__ __launch_bounds__(CP * N / 8) void k_xb(const T *__restrict__ Yr, T *__restrict__ P, Dims d,
+                                                    const cpx<T> *__restrict__ tw)
+{
+    using TL = Tile<T, N, 8>;
+    constexpr int NT = CP * N / 8;
+    __shared__ cpx<T> lds[CP * N];
+    const int nyt = d.ny / (2 * CP);
+    int b = blockIdx.x;
+    const int yt = b % nyt;
+    b /= nyt;
+    const int z = b % d.nz, f = b / d.nz;
+    const int y0 = yt * 2 * CP;
+    const int c = threadIdx.x / TL::TPC, j0 = threadIdx.x % TL::TPC;
+    const cpx<T> *Y = reinterpret_cast<const cpx<T> *>(Yr) + f * d.yfield + (long long)z * d.nxm * d.ny;
+    xrow_load_pairs<T, N, CP, NT>(lds, Y, d.ny, y0, d.kxm);
+    __syncthreads();
+    cpx<T> v[8];
+    TL::template run_from_lds<+1>(lds, tw, c, j0, v);
+    T *row0 = P + f * d.pfield + ((long long)z * d.ny + y0 + 2 * c) * N;
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+        const int x = j0 + TL::kout(s) * TL::TPC;
+        row0[x] = v[s].x;
+        row0[N + x] = v[s].y;
+    }
+}
+
+template <typename T> int launch_xb(const Dims &d, const T *Y, int nvar, T *P, const void *twx, hipStream_t st)
+{
+    constexpr int CP = XCP<T>::CP;
+    const unsigned nb = (unsigned)((long long)nvar * d.nz * (d.ny / (2 * CP)));
+    SAM_SWITCH_N(d.nx, {
+        hipLaunchKernelGGL((k_xb<T, N, CP>), dim3(nb), dim3(CP * N / 8), 0, st, Y, P, d, (const cpx<T> *)twx);
+    });
+    return launched();
+}
+
+// k_xf: physical rows -> Y (x1/(NxNy), kx <= kx_max)
+template <typename T, int N, int CP>
+__global__ __launch_bounds__(CP * N / 8) void k_xf(const T *__restrict__ P, T *__restrict__ Yr, Dims d,
+                                                    const cpx<T> *__restrict__ tw)
+{
+    using TL = Tile<T, N, 8>;
+    constexpr int NT = CP * N / 8;
+    __shared__ cpx<T> lds[CP * N];
+    const int nyt = d.ny / (2 * CP);
+    int b = blockIdx.x;
+    const int yt = b % nyt;
+    b /= nyt;
+    const int z = b % d.nz, f = b / d.nz;
+    const int y0 = yt * 2 * CP;
+    const int c = threadIdx.x / TL::TPC, j0 = threadIdx.x % TL::TPC;
+    const T *row0 = P + f * d.pfield + ((long long)z * d.ny + y0 + 2 * c) * N;
+    cpx<T> v[8];
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+        const int x = j0 + TL::kin(0, s) * TL::TPC;
+        v[s] = {row0[x], row0[N + x]};
+    }
+    TL::template run_regs<-1>(lds, tw, c, j0, v);
+    TL::store_natural(lds, c, j0, v);
+    __syncthreads();
+    cpx<T> *Y = reinterpret_cast<cpx<T> *>(Yr) + f * d.yfield + (long long)z * d.nxm * d.ny;
+    const T h = (T)(0.5 / ((double)d.nx * (double)d.ny));
+    xrow_store_pairs<T, N, CP, NT>(lds, Y, d.ny, y0, d.kxm, h);
+}
+
+template <typename T> int launch_xf(const Dims &d, const T *P, int nvar, T *Y, const void *twx, hipStream_t st)
+{
+    constexpr int CP = XCP<T>::CP;
+    const unsigned nb = (unsigned)((long long)nvar * d.nz * (d.ny / (2 * CP)));
+    SAM_SWITCH_N(d.nx, {
+        hipLaunchKernelGGL((k_xf<T, N, CP>), dim3(nb), dim3(CP * N / 8), 0, st, P, Y, d, (const cpx<T> *)twx);
+    });
+    return launched();
+}
+
+// ---------------------------------------------------------------------
+// k_zf: z forward transform, x1/Nz, truncation -> S (FP64)
+// ---------------------------------------------------------------------
+template <typename T, int N, int C>
+__global__ __launch_bounds__(C * N / 8) void k_zf(const T *__restrict__ Zr, long long ncol, double *__restrict__ S,
+                                                   const cpx<T> *__restrict__ tw, int kzm, int nzm)
+{
+    using TL = Tile<T, N, 8>;
+    constexpr int NT = C * N / 8;
+    __shared__ cpx<T> lds[C * N];
+    const int c = threadIdx.x / TL::TPC, j0 = threadIdx.x % TL::TPC;
+    const long long col0 = (long long)blockIdx.x * C;
+    const long long col = col0 + c;
+    const cpx<T> *Z = reinterpret_cast<const cpx<T> *>(Zr);
+    cpx<T> v[8];
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+        const int p = j0 + TL::kin(0, s) * TL::TPC;
+        v[s] = (col < ncol) ? Z[col * N + p] : cpx<T>{(T)0, (T)0};
+    }
+    TL::template run_regs<-1>(lds, tw, c, j0, v);
+    TL::store_natural(lds, c, j0, v);
+    __syncthreads();
+    cpx<double> *So = reinterpret_cast<cpx<double> *>(S);
+    const double inv = 1.0 / (double)N;
+    for (int e = threadIdx.x; e < C * nzm; e += NT) {
+        const int cc = e / nzm, k = e % nzm;
+        if (col0 + cc >= ncol) continue;
+        const int p = (k <= kzm) ? k : k + N - nzm;
+        const cpx<T> t = lds[lds_idx<T, N>(cc, p)];
+        So[(col0 + cc) * nzm + k] = {(double)t.x * inv, (double)t.y * inv};
+    }
+}
+
+template <typename T> int launch_zf(const Dims &d, const T *Z, int nvar, double *S, const void *twz, hipStream_t st)
+{
+    const long long ncol = (long long)nvar * d.nxm * d.nym;
+    constexpr int C = 4;
+    SAM_SWITCH_N(d.nz, {
+        const unsigned nb = (unsigned)((ncol + C - 1) / C);
+        hipLaunchKernelGGL((k_zf<T, N, C>), dim3(nb), dim3(C * N / 8), 0, st, Z, ncol, S, (const cpx<T> *)twz, d.kzm,
+                           d.nzm);
+    });
+    return launched();
+}
+
+// ---------------------------------------------------------------------
+// Spectral RHS at one mode (rhs.f:108-142).  F: forward-transformed
+// fluxes (already x 1/(NxNy Nz)).  Returns r[0..LU-1] and the pressure.
+// ---------------------------------------------------------------------
+__device__ __forceinline__ cpx<double> mi(cpx<double> a) { return {a.y, -a.x}; }   // -i a
+__device__ __forceinline__ cpx<double> lin3(double a, cpx<double> x, double b, cpx<double> y, double c, cpx<double> z)
+{
+    return {a * x.x + b * y.x + c * z.x, a * x.y + b * y.y + c * z.y};
+}
+
+template <int LU>
+__device__ __forceinline__ void rhs_mode(const cpx<double> *F, double rkx, double rky, double rkz, double rk_sq,
+                                         cpx<double> wsav, cpx<double> tsav, const StepConsts &k, cpx<double> *r,
+                                         cpx<double> &p)
+{
+    const double rkc = fmax(rk_sq, 1.0e-10);
+    r[0] = mi(lin3(rkx, F[0], rky, F[1], rkz, F[3]));
+    r[1] = mi(lin3(rkx, F[1], rky, F[2], rkz, F[4]));
+    r[2] = mi(lin3(rkx, F[3], rky, F[4], rkz, F[5]));
+    if constexpr (LU == 4) {
+        r[0] = {r[0].x + k.buoy_fac_x * tsav.x, r[0].y + k.buoy_fac_x * tsav.y};
+        r[2] = {r[2].x + k.buoy_fac_z * tsav.x, r[2].y + k.buoy_fac_z * tsav.y};
+        const cpx<double> t = mi(lin3(rkx, F[6], rky, F[7], rkz, F[8]));
+        r[3] = {t.x - k.lapse * wsav.x, t.y - k.lapse * wsav.y};
+    }
+    const cpx<double> dv = mi(lin3(rkx, r[0], rky, r[1], rkz, r[2]));
+    p = {dv.x / rkc, dv.y / rkc};
+    // r_i -= i k_i p
+    r[0] = {r[0].x + rkx * p.y, r[0].y - rkx * p.x};
+    r[1] = {r[1].x + rky * p.y, r[1].y - rky * p.x};
+    r[2] = {r[2].x + rkz * p.y, r[2].y - rkz * p.x};
+}
+
+__device__ __forceinline__ int kwave(int i, int km, int nm) { return i <= km ? i : i - nm; }
+
+// k_zrhs: LR z-columns of one (kx,ky) -> RHS and RK/IF update of Lu fields
+template <typename T, int N, int LU, int LR, int EPT>
+__global__ __launch_bounds__(LR * N / EPT) void k_zrhs(const T *__restrict__ Zr, const double *__restrict__ Ur,
+                                                        double *__restrict__ Rr, double *__restrict__ Unr, Dims d,
+                                                        StepConsts k, const DevDiag *__restrict__ diag,
+                                                        const cpx<T> *__restrict__ tw, int m, int mode)
+{
+    using TL = Tile<T, N, EPT>;
+    constexpr int NT = LR * N / EPT;
+    __shared__ cpx<T> lds[LR * N];
+    const long long col = blockIdx.x;
+    const int ix = (int)(col / d.nym), iy = (int)(col % d.nym);
+    const int q = threadIdx.x / TL::TPC, j0 = threadIdx.x % TL::TPC;
+    const cpx<T> *Zc = reinterpret_cast<const cpx<T> *>(Zr) + (q * d.zfield + col * N);
+    cpx<T> v[EPT];
+#pragma unroll
+    for (int s = 0; s < EPT; s++) v[s] = Zc[j0 + TL::kin(0, s) * TL::TPC];
+    TL::template run_regs<-1>(lds, tw, q, j0, v);
+    TL::store_natural(lds, q, j0, v);
+    __syncthreads();
+
+    const cpx<double> *U = reinterpret_cast<const cpx<double> *>(Ur);
+    cpx<double> *R = reinterpret_cast<cpx<double> *>(Rr);
+    cpx<double> *Un = reinterpret_cast<cpx<double> *>(Unr);
+    const double rkx = d.wave_x * (double)ix;
+    const double rky = d.wave_y * (double)kwave(iy, d.kym, d.nym);
+    const double rk_sq2 = rkx * rkx + rky * rky;
+    const double inv = 1.0 / (double)N;
+    const long long ss = d.ss;
+    double dt = 0, dtf1 = 0, dtf2 = 0, vf = 0;
+    if (mode == 0) {
+        dt = diag->dt;
+        dtf2 = k.zeta[m] * dt;
+        dtf1 = k.gamma[m] * dt;
+        vf = k.vis * (dtf1 + dtf2);
+    }
+    for (int kz = threadIdx.x; kz < d.nzm; kz += NT) {
+        const int p = (kz <= d.kzm) ? kz : kz + N - d.nzm;
+        cpx<double> F[LR];
+#pragma unroll
+        for (int qq = 0; qq < LR; qq++) {
+            const cpx<T> t = lds[lds_idx<T, N>(qq, p)];
+            F[qq] = {(double)t.x * inv, (double)t.y * inv};
+        }
+        const double rkz = d.wave_z * (double)kwave(kz, d.kzm, d.nzm);
+        const double rk_sq = rk_sq2 + rkz * rkz;
+        const long long si = col * d.nzm + kz;
+        cpx<double> wsav = {0, 0}, tsav = {0, 0};
+        if constexpr (LU == 4) {
+            wsav = U[2 * ss + si];
+            tsav = U[3 * ss + si];
+        }
+        cpx<double> r[4], pr;
+        rhs_mode<LU>(F, rkx, rky, rkz, rk_sq, wsav, tsav, k, r, pr);
+        if (mode == 0) {
+#pragma unroll
+            for (int n = 0; n < LU; n++) {
+                const double vfn = (n == 3) ? vf * k.pr_inv : vf;
+                const double vis_fac = exp(-vfn * rk_sq);
+                cpx<double> u = U[n * ss + si];
+                if (m > 0) {
+                    const cpx<double> rp = R[n * ss + si];
+                    u = {u.x + dtf2 * rp.x, u.y + dtf2 * rp.y};
+                }
+                u = {u.x + dtf1 * r[n].x, u.y + dtf1 * r[n].y};
+                Un[n * ss + si] = {u.x * vis_fac, u.y * vis_fac};
+                R[n * ss + si] = r[n];
+            }
+        } else {
+#pragma unroll
+            for (int n = 0; n < LU; n++) R[n * ss + si] = r[n];
+            R[LU * ss + si] = pr;
+        }
+    }
+}
+
+template <typename T>
+int launch_zrhs(const Dims &d, const StepConsts &k, const T *Z, const double *U, double *R, double *Unew,
+                const DevDiag *diag, int m, int mode, const void *twz, hipStream_t st)
+{
+    const unsigned nb = (unsigned)((long long)d.nxm * d.nym);
+    SAM_SWITCH_N(d.nz, {
+        if (d.lu == 4) {
+            if constexpr (9 * N / 8 <= 1024 && 9 * N * (int)sizeof(cpx<T>) <= 160 * 1024)
+                hipLaunchKernelGGL((k_zrhs<T, N, 4, 9, 8>), dim3(nb), dim3(9 * N / 8), 0, st, Z, U, R, Unew, d, k, diag,
+                                   (const cpx<T> *)twz, m, mode);
+            else if constexpr (9 * N * (int)sizeof(cpx<T>) <= 160 * 1024)
+                hipLaunchKernelGGL((k_zrhs<T, N, 4, 9, 16>), dim3(nb), dim3(9 * N / 16), 0, st, Z, U, R, Unew, d, k,
+                                   diag, (const cpx<T> *)twz, m, mode);
+            else
+                return SAM_ERR_UNSUPPORTED;
+        } else {
+            if constexpr (6 * N / 8 <= 1024)
+                hipLaunchKernelGGL((k_zrhs<T, N, 3, 6, 8>), dim3(nb), dim3(6 * N / 8), 0, st, Z, U, R, Unew, d, k, diag,
+                                   (const cpx<T> *)twz, m, mode);
+            else
+                hipLaunchKernelGGL((k_zrhs<T, N, 3, 6, 16>), dim3(nb), dim3(6 * N / 16), 0, st, Z, U, R, Unew, d, k,
+                                   diag, (const cpx<T> *)twz, m, mode);
+        }
+    });
+    return launched();
+}
+
+// ---------------------------------------------------------------------
+// Reductions: vel_max.f partials, finalize, clock
+// ---------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_stats_phys(const T *__restrict__ P, long long np, int nvar, long long chunk,
+                                                     double *__restrict__ partials, DevDiag *__restrict__ diag)
+{
+    __shared__ double red[4][8];
+    const long long b0 = (long long)blockIdx.x * chunk;
+    const long long b1 = b0 + chunk < np ? b0 + chunk : np;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int f = 0; f < nvar; f++) {
+        double s = 0, s2 = 0, mx = 0;
+        for (long long i = b0 + threadIdx.x; i < b1; i += 256) {
+            const double a = (double)P[f * np + i];
+            s += a;
+            s2 += a * a;
+            mx = fmax(mx, fabs(a));
+        }
+        s = wave_sum(s);
+        s2 = wave_sum(s2);
+        mx = wave_max(mx);
+        if (lane == 0) {
+            red[w][f] = s;
+            red[w][4 + f] = s2;
+            atomicMax(&diag->umax_bits[f], (unsigned long long)__double_as_longlong(mx));
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 2 * nvar) {
+        const int f = threadIdx.x % nvar, which = threadIdx.x / nvar;
+        double t = 0;
+        for (int i = 0; i < 4; i++) t += red[i][which * 4 + f];
+        partials[(long long)blockIdx.x * 2 * nvar + threadIdx.x] = t;
+    }
+}
+
+int launch_stats_phys(const Dims &d, const void *P, int is_double, int nvar, double *partials, DevDiag *diag,
+                      hipStream_t st, int *nblocks_out)
+{
+    const long long np = d.pfield;
+    const long long chunk = 8192;
+    const unsigned nb = (unsigned)((np + chunk - 1) / chunk);
+    *nblocks_out = (int)nb;
+    if (is_double)
+        hipLaunchKernelGGL((k_stats_phys<double>), dim3(nb), dim3(256), 0, st, (const double *)P, np, nvar, chunk,
+                           partials, diag);
+    else
+        hipLaunchKernelGGL((k_stats_phys<float>), dim3(nb), dim3(256), 0, st, (const float *)P, np, nvar, chunk,
+                           partials, diag);
+    return launched();
+}
+
+// vel_max.f:28-49 from per-block partials (rows of 2*Lu doubles)
+__global__ __launch_bounds__(256) void k_finalize(const double *__restrict__ partials, int nblocks, int lu,
+                                                   DevDiag *__restrict__ diag, StepConsts k, Dims d, int update_dt)
+{
+    __shared__ double red[256];
+    __shared__ double tot[8];
+    for (int f = 0; f < 2 * lu; f++) {
+        double s = 0;
+        for (int b = threadIdx.x; b < nblocks; b += 256) s += partials[(long long)b * 2 * lu + f];
+        red[threadIdx.x] = s;
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) tot[f] = red[0];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double inv = 1.0 / ((double)d.nx * (double)d.ny * (double)d.nz);
+        for (int f = 0; f < 4; f++) {
+            if (f < lu) {
+                diag->u_avg[f] = tot[f] * inv;
+                diag->u_var[f] = tot[lu + f] * inv - diag->u_avg[f] * diag->u_avg[f];
+                diag->u_max[f] = __longlong_as_double((long long)diag->umax_bits[f]);
+            } else {
+                diag->u_avg[f] = diag->u_var[f] = diag->u_max[f] = 0.0;
+            }
+            diag->umax_bits[f] = 0ull;
+        }
+        diag->energy = 0.5 * (diag->u_var[0] + diag->u_var[1] + diag->u_var[2]);
+        const double cx = diag->u_max[0] * d.kxm * d.wave_x;
+        const double cy = diag->u_max[1] * d.kym * d.wave_y;
+        const double cz = diag->u_max[2] * d.kzm * d.wave_z;
+        if (update_dt && k.i_cfl == 1) {
+            const double cm = fmax(cx, fmax(cy, cz));
+            diag->dt = k.cfl0 * k.adv_cfl_limit / cm;
+        }
+        diag->cfl_x = cx * diag->dt;
+        diag->cfl_y = cy * diag->dt;
+        diag->cfl_z = cz * diag->dt;
+        diag->spare[0] = cx;
+        diag->spare[1] = cy;
+        diag->spare[2] = cz;
+    }
+}
+
+int launch_finalize(const Dims &d, const StepConsts &k, const double *partials, int nblocks, DevDiag *diag,
+                    int update_dt, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, partials, nblocks, d.lu, diag, k, d, update_dt);
+    return launched();
+}
+
+__global__ void k_advance_clock(DevDiag *diag)
+{
+    if (threadIdx.x == 0) {
+        diag->time += diag->dt;
+        diag->nt += 1;
+    }
+}
+
+int launch_advance_clock(DevDiag *diag, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_advance_clock, dim3(1), dim3(64), 0, st, diag);
+    return launched();
+}
+
+// ---------------------------------------------------------------------
+// Pointwise stage kernels: projection (rhs.f:74-142) and products
+// ---------------------------------------------------------------------
+template <int LU>
+__global__ __launch_bounds__(256) void k_project(const double *__restrict__ Fr, const double *__restrict__ Usr,
+                                                  double *__restrict__ Rr, Dims d, StepConsts k)
+{
+    const long long si = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (si >= d.ss) return;
+    const int kz = (int)(si % d.nzm);
+    const long long col = si / d.nzm;
+    const int iy = (int)(col % d.nym), ix = (int)(col / d.nym);
+    const cpx<double> *F = reinterpret_cast<const cpx<double> *>(Fr);
+    const cpx<double> *Us = reinterpret_cast<const cpx<double> *>(Usr);
+    cpx<double> *R = reinterpret_cast<cpx<double> *>(Rr);
+    constexpr int LR = LU == 4 ? 9 : 6;
+    cpx<double> f[LR];
+#pragma unroll
+    for (int q = 0; q < LR; q++) f[q] = F[q * d.ss + si];
+    const double rkx = d.wave_x * ix, rky = d.wave_y * kwave(iy, d.kym, d.nym), rkz = d.wave_z * kwave(kz, d.kzm, d.nzm);
+    const double rk_sq = rkx * rkx + rky * rky + rkz * rkz;
+    cpx<double> wsav = {0, 0}, tsav = {0, 0};
+    if constexpr (LU == 4) {
+        wsav = Us[2 * d.ss + si];
+        tsav = Us[3 * d.ss + si];
+    }
+    cpx<double> r[4], p;
+    rhs_mode<LU>(f, rkx, rky, rkz, rk_sq, wsav, tsav, k, r, p);
+#pragma unroll
+    for (int n = 0; n < LU; n++) R[n * d.ss + si] = r[n];
+    R[LU * d.ss + si] = p;
+}
+
+int launch_project(const Dims &d, const StepConsts &k, const double *F, const double *usav, double *r, hipStream_t st)
+{
+    const unsigned nb = (unsigned)((d.ss + 255) / 256);
+    if (d.lu == 4)
+        hipLaunchKernelGGL(k_project<4>, dim3(nb), dim3(256), 0, st, F, usav, r, d, k);
+    else
+        hipLaunchKernelGGL(k_project<3>, dim3(nb), dim3(256), 0, st, F, usav, r, d, k);
+    return launched();
+}
+
+template <typename T, int LU>
+__global__ __launch_bounds__(256) void k_products(const T *__restrict__ P, T *__restrict__ Q, long long np)
+{
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= np) return;
+    T u[LU];
+#pragma unroll
+    for (int f = 0; f < LU; f++) u[f] = P[f * np + i];
+    constexpr int LR = LU == 4 ? 9 : 6;
+#pragma unroll
+    for (int q = 0; q < LR; q++) Q[q * np + i] = u[prod_i(q)] * u[prod_j(q)];
+}
+
+int launch_products(const Dims &d, const void *P, int is_double, void *Q, hipStream_t st)
+{
+    const long long np = d.pfield;
+    const unsigned nb = (unsigned)((np + 255) / 256);
+    if (is_double) {
+        if (d.lu == 4)
+            hipLaunchKernelGGL((k_products<double, 4>), dim3(nb), dim3(256), 0, st, (const double *)P, (double *)Q, np);
+        else
+            hipLaunchKernelGGL((k_products<double, 3>), dim3(nb), dim3(256), 0, st, (const double *)P, (double *)Q, np);
+    } else {
+        if (d.lu == 4)
+            hipLaunchKernelGGL((k_products<float, 4>), dim3(nb), dim3(256), 0, st, (const float *)P, (float *)Q, np);
+        else
+            hipLaunchKernelGGL((k_products<float, 3>), dim3(nb), dim3(256), 0, st, (const float *)P, (float *)Q, np);
+    }
+    return launched();
+}
+
+// ---------------------------------------------------------------------
+// spectra.f:13-111 (i_les = 0): shell sums of |u|^2, Re(r.u*),
+// k^2 vis |u|^2, sample counts; divergence rms/max.
+// ---------------------------------------------------------------------
+constexpr int SPEC_YCH = 8;
+
+__global__ __launch_bounds__(256) void k_spectra(const double *__restrict__ Ur, const double *__restrict__ Rr, Dims d,
+                                                  double vis, double *__restrict__ partials, DevDiag *__restrict__ diag)
+{
+    extern __shared__ double acc[];   // 5*lm + 1
+    const int lm = d.lm;
+    for (int i = threadIdx.x; i < 5 * lm + 1; i += 256) acc[i] = 0.0;
+    __syncthreads();
+    const int ix = blockIdx.x / SPEC_YCH, yc = blockIdx.x % SPEC_YCH;
+    const int ych = (d.nym + SPEC_YCH - 1) / SPEC_YCH;
+    const int iy0 = yc * ych, iy1 = min(d.nym, iy0 + ych);
+    const cpx<double> *U = reinterpret_cast<const cpx<double> *>(Ur);
+    const cpx<double> *R = reinterpret_cast<const cpx<double> *>(Rr);
+    const double factor = (ix == 0 || 2 * ix == d.nx) ? 1.0 : 2.0;
+    const double rkx = d.wave_x * ix;
+    double dmax = 0.0;
+    const long long n = (long long)(iy1 > iy0 ? iy1 - iy0 : 0) * d.nzm;
+    for (long long e = threadIdx.x; e < n; e += 256) {
+        const int iy = iy0 + (int)(e / d.nzm), kz = (int)(e % d.nzm);
+        const int ky = kwave(iy, d.kym, d.nym), kzw = kwave(kz, d.kzm, d.nzm);
+        const int ksq = ix * ix + ky * ky + kzw * kzw;
+        const int is = (int)(sqrt((double)ksq) + 0.5);
+        const double rky = d.wave_y * ky, rkz = d.wave_z * kzw;
+        const double rk_sq = rkx * rkx + rky * rky + rkz * rkz;
+        const long long si = ((long long)ix * d.nym + iy) * d.nzm + kz;
+        const cpx<double> u1 = U[si], u2 = U[d.ss + si], u3 = U[2 * d.ss + si];
+        const cpx<double> r1 = R[si], r2 = R[d.ss + si], r3 = R[2 * d.ss + si];
+        const double usq = u1.x * u1.x + u1.y * u1.y + u2.x * u2.x + u2.y * u2.y + u3.x * u3.x + u3.y * u3.y;
+        const double udu = r1.x * u1.x + r1.y * u1.y + r2.x * u2.x + r2.y * u2.y + r3.x * u3.x + r3.y * u3.y;
+        atomicAdd(&acc[is], factor * usq);
+        atomicAdd(&acc[lm + is], factor * udu);
+        atomicAdd(&acc[2 * lm + is], factor * rk_sq * vis * usq);
+        atomicAdd(&acc[4 * lm + is], factor);
+        const cpx<double> dv = lin3(rkx, u1, rky, u2, rkz, u3);
+        const double dsq = dv.x * dv.x + dv.y * dv.y;
+        atomicAdd(&acc[5 * lm], dsq);
+        dmax = fmax(dmax, dsq);
+    }
+    dmax = wave_max(dmax);
+    if ((threadIdx.x & 63) == 0) atomicMax(&diag->divmax_bits, (unsigned long long)__double_as_longlong(dmax));
+    __syncthreads();
+    for (int i = threadIdx.x; i < 5 * lm + 1; i += 256) partials[(long long)blockIdx.x * (5 * lm + 1) + i] = acc[i];
+}
+
+__global__ __launch_bounds__(256) void k_spectra_reduce(const double *__restrict__ partials, int nblocks, int lm,
+                                                         double *__restrict__ out, DevDiag *__restrict__ diag)
+{
+    const int w = 5 * lm + 1;
+    for (int i = threadIdx.x; i < w; i += 256) {
+        double s = 0.0;
+        for (int b = 0; b < nblocks; b++) s += partials[(long long)b * w + i];
+        out[i < 5 * lm ? i : 5 * lm + 1] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double se = 0.0;
+        for (int i = 0; i < lm; i++) se += out[i];
+        out[5 * lm] = 0.5 * se;
+        out[5 * lm + 1] = sqrt(out[5 * lm + 1]);
+        out[5 * lm + 2] = sqrt(__longlong_as_double((long long)diag->divmax_bits));
+        diag->divmax_bits = 0ull;
+    }
+}
+
+int launch_spectra(const Dims &d, double vis, const double *U, const double *R, double *partials, double *out,
+                   DevDiag *diag, hipStream_t st)
+{
+    const int nb = d.nxm * SPEC_YCH;
+    const size_t sh = sizeof(double) * (5 * d.lm + 1);
+    hipLaunchKernelGGL(k_spectra, dim3(nb), dim3(256), sh, st, U, R, d, vis, partials, diag);
+    hipLaunchKernelGGL(k_spectra_reduce, dim3(1), dim3(256), 0, st, partials, nb, d.lm, out, diag);
+    return launched();
+}
+
+// ---------------------------------------------------------------------
+// precision conversion helpers
+// ---------------------------------------------------------------------
+template <typename T> __global__ void k_to_double(const T *__restrict__ s, double *__restrict__ d, long long n)
+{
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) d[i] = (double)s[i];
+}
+template <typename T> __global__ void k_from_double(const double *__restrict__ s, T *__restrict__ d, long long n)
+{
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) d[i] = (T)s[i];
+}
+template <typename T> int launch_convert(const T *src, double *dst, long long n, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_to_double<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, src, dst, n);
+    return launched();
+}
+template <typename T> int launch_convert_to(const double *src, T *dst, long long n, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_from_double<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, src, dst, n);
+    return launched();
+}
+
+long long partials_capacity(const Dims &d)
+{
+    long long a = (long long)d.nz * d.ny;                            // x-row blocks x 2Lu
+    long long b = (long long)d.nxm * SPEC_YCH * (5 * d.lm + 1);       // spectra
+    long long c = (d.pfield / 8192 + 1) * 8;                          // physical stats
+    long long e = (d.pfield / 1024 + 1) * 2;                          // dm reductions
+    long long m = a > b ? a : b;
+    m = m > c ? m : c;
+    m = m > e ? m : e;
+    return m + 1024;
+}
+
+#define SAM_INST(T)                                                                                                   \
+    template int launch_zb<T>(const Dims &, const double *, int, int, T *, const void *, hipStream_t);                \
+    template int launch_yb<T>(const Dims &, const T *, int, T *, const void *, hipStream_t);                          \
+    template int launch_yf<T>(const Dims &, const T *, int, T *, const void *, hipStream_t);                          \
+    template int launch_xb<T>(const Dims &, const T *, int, T *, const void *, hipStream_t);                          \
+    template int launch_xf<T>(const Dims &, const T *, int, T *, const void *, hipStream_t);                          \
+    template int launch_zf<T>(const Dims &, const T *, int, double *, const void *, hipStream_t);                     \
+    template int launch_xrow<T>(const Dims &, T *, int, double *, DevDiag *, const void *, hipStream_t, int *);       \
+    template int launch_zrhs<T>(const Dims &, const StepConsts &, const T *, const double *, double *, double *,     \
+                                const DevDiag *, int, int, const void *, hipStream_t);                               \
+    template int launch_convert<T>(const T *, double *, long long, hipStream_t);                                      \
+    template int launch_convert_to<T>(const double *, T *, long long, hipStream_t);
+SAM_INST(double)
+SAM_INST(float)
+
+} // namespace samhip

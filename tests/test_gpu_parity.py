@@ -1,0 +1,164 @@
+"""GPU parity: libsamhip (through its C ABI) against the CPU oracle on the
+same inputs.  FP64 bar: relative L2 <= 1e-12 (the north-star tolerance for
+the drop-in), per stage and after N full RK3 steps.  FP32-split bar: the
+error is reported and bounded by FP32 rounding (<= 2e-6 relative per stage).
+"""
+import numpy as np
+import pytest
+
+from sam_amd import ic
+from sam_amd import samhip as S
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-12
+
+
+def rel_l2(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
+
+
+def pair(oracle_mod, n, ny=None, nz=None, prec=S.PREC_FP64, **kw):
+    O = oracle_mod
+    cfg = O.make_cfg(n, ny, nz, **kw)
+    o = O.Oracle(cfg)
+    p = S.make_params(n, ny, nz, xl=cfg.xl, yl=cfg.yl, zl=cfg.zl, n_dealias=cfg.n_dealias,
+                      i_strat=cfg.i_strat, nrk_max=cfg.nrk_max, i_cfl=cfg.i_cfl, precision=prec,
+                      vis=cfg.vis, pr=cfg.pr, grav=cfg.grav, to=cfg.to, lapse=cfg.lapse,
+                      cfl0=cfg.cfl0, dt0=cfg.dt0)
+    return o, S.Sam(p)
+
+
+def rand_spec(o, nvar, seed=0, ldz=None):
+    rng = np.random.default_rng(seed)
+    sh = (ldz or o.nzm, o.nym, o.nxm, nvar)
+    u = rng.standard_normal(sh) + 1j * rng.standard_normal(sh)
+    # kx = 0 plane: keep it Hermitian-consistent enough for transforms (not required)
+    return np.asfortranarray(u)
+
+
+GRIDS = [
+    dict(n=16),
+    dict(n=32, i_strat=1),
+    dict(n=16, ny=32, nz=64, xl=3.0, yl=5.0, zl=7.0),
+    dict(n=32, n_dealias=0),
+    dict(n=32, n_dealias=2, i_strat=1),
+    dict(n=64),
+]
+
+
+@pytest.mark.parametrize("g", GRIDS, ids=lambda g: "-".join(f"{k}{v}" for k, v in g.items()))
+def test_stage_z_trans_b(oracle_mod, g):
+    g = dict(g)
+    o, s = pair(oracle_mod, g.pop("n"), **g)
+    u = rand_spec(o, o.lu, seed=1)
+    ref = o.z_trans_b(u)                              # (Ny_min, Nx_min, Nz, n)
+    got = s.z_trans_b(u)                              # (Nz, Ny_min, Nx_min, n)
+    assert rel_l2(np.transpose(got, (1, 2, 0, 3)), ref) < TOL64
+
+
+@pytest.mark.parametrize("g", GRIDS, ids=lambda g: "-".join(f"{k}{v}" for k, v in g.items()))
+def test_stage_xy_trans_b(oracle_mod, g):
+    g = dict(g)
+    o, s = pair(oracle_mod, g.pop("n"), **g)
+    zb = rand_spec(o, o.lu, seed=2, ldz=o.nz)         # (Nz, Ny_min, Nx_min, n), z leading
+    ref = o.xy_trans_b(o.x2z(np.transpose(zb, (1, 2, 0, 3)).copy(order="F")))
+    got = s.xy_trans_b(zb)
+    assert rel_l2(got, ref) < TOL64
+
+
+@pytest.mark.parametrize("g", GRIDS, ids=lambda g: "-".join(f"{k}{v}" for k, v in g.items()))
+def test_stage_xy_trans_f(oracle_mod, g):
+    g = dict(g)
+    o, s = pair(oracle_mod, g.pop("n"), **g)
+    rng = np.random.default_rng(3)
+    p = np.asfortranarray(rng.standard_normal((o.nx, o.ny, o.nz, o.lr)))
+    ref = o.xy_trans_f(p)
+    got = s.xy_trans_f(p)
+    assert rel_l2(got, ref) < TOL64
+
+
+@pytest.mark.parametrize("g", GRIDS, ids=lambda g: "-".join(f"{k}{v}" for k, v in g.items()))
+def test_stage_z_trans_f(oracle_mod, g):
+    g = dict(g)
+    o, s = pair(oracle_mod, g.pop("n"), **g)
+    a = rand_spec(o, o.lr, seed=4, ldz=o.nz)
+    ref = o.z_trans_f(a)
+    got = s.z_trans_f(a)
+    assert rel_l2(got, ref) < TOL64
+
+
+def test_stage_products_velmax_project(oracle_mod):
+    o, s = pair(oracle_mod, 32, i_strat=1, grav=16.0, lapse0=1.0, i_cfl=1, cfl0=0.5)
+    rng = np.random.default_rng(5)
+    phys = np.asfortranarray(rng.standard_normal((o.nx, o.ny, o.nz, o.lu)))
+    assert rel_l2(s.products(phys), o.products(phys)) == 0.0
+    st_ref = o.vel_max(phys, 0.01)
+    st = s.vel_max(phys, 0.01)
+    np.testing.assert_allclose(st, st_ref, rtol=1e-12, atol=1e-15)
+    F = rand_spec(o, o.lr, seed=6)
+    us = rand_spec(o, o.lu, seed=7)
+    assert rel_l2(s.project(F, us), o.project(F, us)) < TOL64
+
+
+@pytest.mark.parametrize("strat", [0, 1])
+def test_rhs_full(oracle_mod, strat):
+    o, s = pair(oracle_mod, 32, i_strat=strat, grav=16.0, lapse0=1.0, vis=5e-3)
+    u = ic.random_solenoidal(32, k0=4, seed=11, i_strat=strat)
+    if strat:
+        u[:, :, :, 3] = 0.1 * ic.random_solenoidal(32, k0=3, seed=12)[:, :, :, 0]
+    s.set_state(u)
+    r_ref, st_ref = o.rhs(u)
+    r, d = s.rhs(want_spectra=True)
+    assert rel_l2(r[..., : o.lu], r_ref[..., : o.lu]) < TOL64
+    assert rel_l2(r[..., o.lu], r_ref[..., o.lu]) < TOL64
+    np.testing.assert_allclose(d["u_max"][: o.lu], st_ref[8:8 + o.lu], rtol=1e-12)
+    np.testing.assert_allclose(d["energy"], st_ref[12], rtol=1e-12)
+    sp = s.spectra()
+    sp_ref = o.spectra(u, r_ref)
+    for key in ("ek", "tk", "dk", "sample"):
+        np.testing.assert_allclose(sp[key], sp_ref[key], rtol=1e-11, atol=1e-14)
+    np.testing.assert_allclose(sp["energy_w"], sp_ref["energy_w"], rtol=1e-12)
+
+
+def test_taylor_green_64_steps(oracle_mod):
+    """config 1 (64^3 TG, nu = 1/400, dt = 0.01): N steps, FP64 parity 1e-12"""
+    nsteps = 20
+    o, s = pair(oracle_mod, 64, vis=0.0025, dt0=0.01)
+    u0 = ic.taylor_green(64)
+    s.set_state(u0)
+    d = s.step(nsteps)
+    got = s.get_state()
+    ref, t, dt, st = o.step(u0, nsteps)
+    assert rel_l2(got, ref) < TOL64
+    assert abs(d["time"] - t) < 1e-12
+    np.testing.assert_allclose(d["energy"], st[12], rtol=1e-12)
+
+
+def test_stratified_cfl_steps(oracle_mod):
+    """config-2 physics at 32^3: N = 4 (N^2 = grav*lapse/To = 16), CFL dt"""
+    nsteps = 10
+    o, s = pair(oracle_mod, 32, i_strat=1, grav=16.0, lapse0=1.0, vis=5e-3, i_cfl=1, cfl0=0.5)
+    u0 = ic.random_solenoidal(32, k0=4, seed=20261203, i_strat=1)
+    s.set_state(u0)
+    d = s.step(nsteps)
+    got = s.get_state()
+    ref, t, dt, st = o.step(u0, nsteps)
+    assert rel_l2(got, ref) < TOL64
+    assert abs(d["dt"] - dt) < 1e-12 * dt
+    assert abs(d["time"] - t) < 1e-12
+
+
+def test_split_precision_bounded(oracle_mod):
+    """FP32-split: transforms/products in FP32, state/RK in FP64"""
+    o, s = pair(oracle_mod, 32, i_strat=1, grav=16.0, lapse0=1.0, vis=5e-3, i_cfl=1, cfl0=0.5,
+                prec=S.PREC_SPLIT)
+    u0 = ic.random_solenoidal(32, k0=4, seed=3, i_strat=1)
+    s.set_state(u0)
+    s.step(5)
+    got = s.get_state()
+    ref, t, dt, st = o.step(u0, 5)
+    err = rel_l2(got, ref)
+    assert err < 1e-5, err
