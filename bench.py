@@ -1,0 +1,219 @@
+#!/usr/bin/env python3
+"""Throughput of SAM's RK3 pseudo-spectral step on one MI355X.
+
+Workload (default): the stratified Boussinesq DNS of the 256^3 config run at
+512^3 -- seeded random-phase solenoidal velocity, E(k) ~ k^4 exp(-2(k/4)^2),
+u_rms = 1, T' = 0, N = 4 (grav*lapse/To = 16), nu = kappa = 5e-4, Pr = 1,
+RK3 with CFL 0.5, 2/3 dealiasing.  One "step" is one full RK3 step (three
+substeps) of the whole 512^3 grid with the state resident in HBM.
+
+  python bench.py [--steps K] [--warmup W] [--n 512] [--no-cpu] [--no-split]
+
+Prints ONE JSON line (metric/value/unit ... roofline, cpu_baseline).
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
+METRIC = "RK3 Mpoint-steps/s at 512^3 + %HBM peak; learned vs dm.f SGS step time"
+
+
+def grid_counts(n, i_strat=1):
+    kmax = int((n - 0.1) / 3.0)
+    nxm, nym, nzm = kmax + 1, 2 * kmax + 1, 2 * kmax + 1
+    S = nxm * nym * nzm
+    Z = nxm * nym * n
+    Y = nxm * n * n
+    lu, lr = (4, 9) if i_strat else (3, 6)
+    return dict(S=S, Z=Z, Y=Y, P=n ** 3, lu=lu, lr=lr)
+
+
+def kernel_bytes(n, bc, i_strat=1):
+    """Algorithmic HBM bytes per launch of each step kernel (fused 3-pass
+    model): state/RHS in FP64 (16 B), transform elements bc = 16 or 8 B."""
+    g = grid_counts(n, i_strat)
+    S, Z, Y, lu, lr = g["S"], g["Z"], g["Y"], g["lu"], g["lr"]
+    return {
+        "zb": lu * S * 16 + lu * Z * bc,
+        "yb": lu * Z * bc + lu * Y * bc,
+        "xrow": lu * Y * bc + lr * Y * bc,
+        "yf": lr * Y * bc + lr * Z * bc,
+        "zrhs": lr * Z * bc + 4 * lu * S * 16,
+    }
+
+
+def step_bytes(n, bc, i_strat=1, nrk=3):
+    g = grid_counts(n, i_strat)
+    per_sub = 5 * g["lu"] * g["S"] * 16 + 2 * (g["lu"] + g["lr"]) * (g["Z"] + g["Y"]) * bc
+    return nrk * per_sub
+
+
+def make_params(S, n, precision, sgs=0):
+    return S.make_params(n, i_strat=1, grav=16.0, to=1.0, lapse=1.0, vis=5e-4, pr=1.0,
+                         i_cfl=1, cfl0=0.5, dt0=0.01, precision=precision, sgs=sgs)
+
+
+def run_gpu(n, precision, steps, warmup, u0, S):
+    sam = S.Sam(make_params(S, n, precision))
+    sam.set_state(u0)
+    sam.step(warmup)
+    sam.sync()
+    sam.prof_enable(True)
+    sam.sync()
+    t0 = time.perf_counter()
+    diag = sam.step(steps, want_spectra=True)
+    sam.sync()
+    t1 = time.perf_counter()
+    prof = sam.prof_read()
+    sam.prof_enable(False)
+    spec = sam.spectra()
+    sam.close()
+    return dict(wall=t1 - t0, diag=diag, prof=prof, spec=spec)
+
+
+def dominant_roofline(n, bc, prof, steps):
+    kb = kernel_bytes(n, bc)
+    best = None
+    for name, nbytes in kb.items():
+        ms, cnt = prof.get(name, (0.0, 0))
+        if cnt == 0:
+            continue
+        if best is None or ms > best[1]:
+            best = (name, ms, cnt, nbytes)
+    name, ms, cnt, nbytes = best
+    avg_s = ms / cnt / 1e3
+    achieved = nbytes / avg_s / 1e9
+    per_kernel = {}
+    for k, nb in kb.items():
+        kms, kc = prof.get(k, (0.0, 0))
+        if kc:
+            per_kernel[k] = dict(avg_us=round(kms / kc * 1e3, 2), GBps=round(nb / (kms / kc / 1e3) / 1e9, 1),
+                                 share=round(kms / sum(v[0] for v in prof.values()), 3))
+    return dict(kernel=name, bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK / 1e9, unit="GB/s",
+                frac=round(achieved / (HBM_PEAK / 1e9), 4), traffic=None, bytes_per_launch=nbytes,
+                avg_launch_us=round(avg_s * 1e6, 2), launches=cnt), per_kernel
+
+
+def cpu_baseline(n_cpu, steps_cpu, u0_small):
+    from oracle import oracle as O
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    o = O.Oracle(O.make_cfg(n_cpu, i_strat=1, grav=16.0, to=1.0, lapse0=1.0, vis=5e-4, pr=1.0,
+                            i_cfl=1, cfl0=0.5, dt0=0.01), threads=threads)
+    t0 = time.perf_counter()
+    o.step(u0_small, steps_cpu)
+    dt = time.perf_counter() - t0
+    val = n_cpu ** 3 * steps_cpu / dt / 1e6
+    return dict(value=round(val, 3), unit="Mpoint-steps/s", cores=threads, kind="port",
+                sample=f"{steps_cpu} RK3 step(s) of the same stratified DNS at {n_cpu}^3 by the C FP64 "
+                       f"oracle (OpenMP, {threads} threads), {dt:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-split", action="store_true")
+    ap.add_argument("--cpu-n", type=int, default=0, help="CPU baseline grid (default: same n if RAM allows)")
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (the step's multi-GPU slab exchange is design-only); "
+              f"got --gpus {args.gpus}", file=sys.stderr)
+        sys.exit(2)
+
+    from sam_amd import ic
+    from sam_amd import samhip as S
+    S.load()
+
+    n = args.n
+    log = lambda *a: print(*a, file=sys.stderr, flush=True)
+    log(f"[bench] building IC {n}^3")
+    u0 = ic.random_solenoidal(n, k0=4.0, seed=20261203, i_strat=1)
+
+    log("[bench] FP64 run")
+    r64 = run_gpu(n, S.PREC_FP64, args.steps, args.warmup, u0, S)
+    ms_step = r64["wall"] / args.steps * 1e3
+    value = n ** 3 * args.steps / r64["wall"] / 1e6
+    roof, per_kernel = dominant_roofline(n, 16, r64["prof"], args.steps)
+    sb = step_bytes(n, 16)
+    step_frac = sb / (r64["wall"] / args.steps) / HBM_PEAK
+    log(f"[bench] FP64 {ms_step:.2f} ms/step, {value:.1f} Mpt-steps/s, step HBM frac {step_frac:.3f}")
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Mpoint-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: seeded random-phase solenoidal IC, E(k)~k^4 exp(-2(k/4)^2), u_rms=1, T'=0",
+        "config": {"workload": f"stratified Boussinesq DNS {n}^3, N=4, nu=kappa=5e-4, Pr=1, RK3 CFL 0.5, "
+                               f"2/3 dealias, FP64 plain", "n": n, "precision": "fp64"},
+        "roofline": roof,
+        "step_hbm": {"bytes_per_step": sb, "frac": round(step_frac, 4),
+                     "achieved_GBps": round(sb / (r64["wall"] / args.steps) / 1e9, 1)},
+        "kernels": per_kernel,
+    }
+
+    if not args.no_split:
+        log("[bench] FP32-split run")
+        rsp = run_gpu(n, S.PREC_SPLIT, args.steps, args.warmup, u0, S)
+        ms_sp = rsp["wall"] / args.steps * 1e3
+        sb32 = step_bytes(n, 8)
+        roof32, pk32 = dominant_roofline(n, 8, rsp["prof"], args.steps)
+        e64, e32 = r64["spec"]["energy_w"], rsp["spec"]["energy_w"]
+        d64, d32 = np.sum(r64["spec"]["dk"]), np.sum(rsp["spec"]["dk"])
+        ek64, ek32 = r64["spec"]["ek"], rsp["spec"]["ek"]
+        out["fp32_split"] = {
+            "value": round(n ** 3 * args.steps / rsp["wall"] / 1e6, 2),
+            "ms_per_step": round(ms_sp, 3),
+            "step_hbm_frac": round(sb32 / (rsp["wall"] / args.steps) / HBM_PEAK, 4),
+            "roofline": roof32,
+            "kernels": pk32,
+            "ke_rel_err_vs_fp64": float(abs(e32 - e64) / abs(e64)),
+            "diss_rel_err_vs_fp64": float(abs(d32 - d64) / abs(d64)),
+            "ek_rel_l2_err_vs_fp64": float(np.linalg.norm(ek32 - ek64) / np.linalg.norm(ek64)),
+            "after_steps": args.warmup + args.steps,
+        }
+        log(f"[bench] split {ms_sp:.2f} ms/step, KE err {out['fp32_split']['ke_rel_err_vs_fp64']:.2e}")
+
+    if not args.no_cpu:
+        n_cpu = args.cpu_n or n
+        try:
+            import psutil
+            avail = psutil.virtual_memory().available
+        except Exception:
+            avail = 64 << 30
+        need = 230 * n_cpu ** 3   # oracle working set, bytes (about 28 GB at 512^3)
+        while need > 0.6 * avail and n_cpu > 64:
+            n_cpu //= 2
+            need = 230 * n_cpu ** 3
+        log(f"[bench] CPU baseline {n_cpu}^3 x {args.cpu_steps}")
+        u_small = u0 if n_cpu == n else ic.random_solenoidal(n_cpu, k0=4.0, seed=20261203, i_strat=1)
+        out["cpu_baseline"] = cpu_baseline(n_cpu, args.cpu_steps, u_small)
+    else:
+        out["cpu_baseline"] = None
+
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -146,6 +146,31 @@ int  sam_eddy_visc(sam_handle *h, const double *phys, double *vt, double *c_delt
 /* (1,4,1)/6 periodic test filter of nvar physical fields (filter.f:9-42) */
 int  sam_test_filter(sam_handle *h, const double *in, double *out, int nvar);
 
+/* ---- measurement (no counterpart in the reference, which has no timers) */
+/* kernel classes timed with HIP events on the handle's stream */
+#define SAM_K_ZB     0   /* z inverse transform            */
+#define SAM_K_YB     1   /* y pad + y inverse transform    */
+#define SAM_K_XROW   2   /* fused x C2R/products/R2C       */
+#define SAM_K_FINAL  3   /* vel_max finalize / dt          */
+#define SAM_K_YF     4   /* y forward transform + truncate */
+#define SAM_K_ZRHS   5   /* z forward + projection + RK/IF */
+#define SAM_K_SPEC   6   /* spectra                        */
+#define SAM_K_XB     7   /* x C2R to physical (LES path)   */
+#define SAM_K_XF     8   /* x R2C from physical (LES path) */
+#define SAM_K_SGS    9   /* SGS model kernels (LES path)   */
+#define SAM_K_MISC  10   /* clock and small kernels        */
+#define SAM_K_COUNT 16
+/* on != 0: record a HIP event pair around every kernel the step launches */
+int  sam_prof_enable(sam_handle *h, int on);
+/* accumulated milliseconds and launch counts per kernel class since the
+ * last call (arrays of SAM_K_COUNT); resets the accumulators */
+int  sam_prof_read(sam_handle *h, double *ms, long long *count);
+/* device-wide synchronisation (bench timing brackets) */
+int  sam_sync(sam_handle *h);
+/* plain device-to-device copy of `bytes` bytes, `reps` times, on the
+ * handle's stream: the calibration kernel for HBM counters */
+int  sam_device_copy(sam_handle *h, long long bytes, int reps);
+
 #ifdef __cplusplus
 }
 #endif

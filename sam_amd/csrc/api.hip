@@ -71,7 +71,70 @@ struct sam_handle {
     void *tw[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
     int have_spectra = 0;
     int sgs_active = 0;
+    // HIP-event timing of the step's kernels (sam_prof_*)
+    bool prof = false;
+    std::vector<hipEvent_t> ev_free;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
+    double prof_ms[SAM_K_COUNT] = {0};
+    long long prof_cnt[SAM_K_COUNT] = {0};
 };
+
+static hipEvent_t ev_get(sam_handle *h)
+{
+    hipEvent_t e = nullptr;
+    if (!h->ev_free.empty()) {
+        e = h->ev_free.back();
+        h->ev_free.pop_back();
+    } else if (hipEventCreate(&e) != hipSuccess) {
+        (void)hipGetLastError();
+        e = nullptr;
+    }
+    return e;
+}
+
+// brackets one kernel launch with an event pair when timing is on
+struct ProfScope {
+    sam_handle *h;
+    int id;
+    hipEvent_t a = nullptr, b = nullptr;
+    ProfScope(sam_handle *h_, int id_) : h(h_), id(id_)
+    {
+        if (h->prof) {
+            a = ev_get(h);
+            b = ev_get(h);
+            if (a && b) hipEventRecord(a, h->st);
+        }
+    }
+    ~ProfScope()
+    {
+        if (h->prof && a && b) {
+            hipEventRecord(b, h->st);
+            h->ev_pending.push_back({id, {a, b}});
+        } else {
+            if (a) h->ev_free.push_back(a);
+            if (b) h->ev_free.push_back(b);
+        }
+    }
+};
+
+// fold finished event pairs into the accumulators (after a stream sync)
+static void prof_collect(sam_handle *h)
+{
+    for (auto &pe : h->ev_pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, pe.second.first, pe.second.second) == hipSuccess) {
+            h->prof_ms[pe.first] += ms;
+            h->prof_cnt[pe.first] += 1;
+        } else {
+            (void)hipGetLastError();
+        }
+        h->ev_free.push_back(pe.second.first);
+        h->ev_free.push_back(pe.second.second);
+    }
+    h->ev_pending.clear();
+}
+
+#define PROF(id) ProfScope prof_scope_##__LINE__(h, id)
 
 static bool pow2_in(int n, int lo, int hi) { return n >= lo && n <= hi && (n & (n - 1)) == 0; }
 
@@ -138,6 +201,8 @@ extern "C" int sam_destroy(sam_handle *h)
     hipFree(h->diag);
     for (int i = 0; i < 3; i++)
         for (int j = 0; j < 2; j++) hipFree(h->tw[i][j]);
+    prof_collect(h);
+    for (auto e : h->ev_free) hipEventDestroy(e);
     if (h->st) hipStreamDestroy(h->st);
     delete h;
     return SAM_OK;
@@ -333,6 +398,7 @@ extern "C" int sam_set_clock(sam_handle *h, double time, double dt, long long nt
 static int fetch_diag(sam_handle *h, sam_diag *o)
 {
     HIPCHK(hipStreamSynchronize(h->st));
+    prof_collect(h);
     if (!o) return SAM_OK;
     DevDiag dg;
     HIPCHK(hipMemcpy(&dg, h->diag, sizeof dg, hipMemcpyDeviceToHost));
@@ -361,16 +427,31 @@ template <typename T> static int fluxes_to_z(sam_handle *h, int stats)
     const Dims &d = h->d;
     T *Z = (T *)h->Zb, *Y = (T *)h->Yb;
     const int pi = h->prec;
-    RCHK(launch_zb<T>(d, h->U[h->cur], d.nzm, d.lu, Z, h->tw[2][pi], h->st));
-    RCHK(launch_yb<T>(d, Z, d.lu, Y, h->tw[1][pi], h->st));
+    {
+        PROF(SAM_K_ZB);
+        RCHK(launch_zb<T>(d, h->U[h->cur], d.nzm, d.lu, Z, h->tw[2][pi], h->st));
+    }
+    {
+        PROF(SAM_K_YB);
+        RCHK(launch_yb<T>(d, Z, d.lu, Y, h->tw[1][pi], h->st));
+    }
     if (h->p.sgs == SAM_SGS_NONE) {
         int nb = 0;
-        RCHK(launch_xrow<T>(d, Y, stats, h->partials, h->diag, h->tw[0][pi], h->st, &nb));
-        if (stats) RCHK(launch_finalize(d, h->k, h->partials, nb, h->diag, 1, h->st));
+        {
+            PROF(SAM_K_XROW);
+            RCHK(launch_xrow<T>(d, Y, stats, h->partials, h->diag, h->tw[0][pi], h->st, &nb));
+        }
+        if (stats) {
+            PROF(SAM_K_FINAL);
+            RCHK(launch_finalize(d, h->k, h->partials, nb, h->diag, 1, h->st));
+        }
     } else {
         return set_err(SAM_ERR_UNSUPPORTED, "SGS path not built yet");
     }
-    RCHK(launch_yf<T>(d, Y, d.lr, Z, h->tw[1][pi], h->st));
+    {
+        PROF(SAM_K_YF);
+        RCHK(launch_yf<T>(d, Y, d.lr, Z, h->tw[1][pi], h->st));
+    }
     return SAM_OK;
 }
 
@@ -379,9 +460,13 @@ template <typename T> static int substep(sam_handle *h, int m, int spectra)
     const Dims &d = h->d;
     int rc = fluxes_to_z<T>(h, m == 0);
     if (rc) return rc;
-    RCHK(launch_zrhs<T>(d, h->k, (const T *)h->Zb, h->U[h->cur], h->R, h->U[1 - h->cur], h->diag, m, 0,
-                        h->tw[2][h->prec], h->st));
+    {
+        PROF(SAM_K_ZRHS);
+        RCHK(launch_zrhs<T>(d, h->k, (const T *)h->Zb, h->U[h->cur], h->R, h->U[1 - h->cur], h->diag, m, 0,
+                            h->tw[2][h->prec], h->st));
+    }
     if (spectra) {
+        PROF(SAM_K_SPEC);
         RCHK(launch_spectra(d, h->p.vis, h->U[h->cur], h->R, h->partials, h->specout, h->diag, h->st));
         h->have_spectra = 1;
     }
@@ -399,6 +484,7 @@ extern "C" int sam_step(sam_handle *h, int nsteps, int want_spectra, sam_diag *d
             int rc = h->prec ? substep<float>(h, m, spec) : substep<double>(h, m, spec);
             if (rc) return rc;
         }
+        PROF(SAM_K_MISC);
         RCHK(launch_advance_clock(h->diag, h->st));
     }
     return fetch_diag(h, diag);
@@ -627,4 +713,51 @@ extern "C" int sam_test_filter(sam_handle *h, const double *in, double *out, int
     (void)in; (void)out; (void)nvar;
     if (!h) return set_err(SAM_ERR_VALUE, "null handle");
     return set_err(SAM_ERR_UNSUPPORTED, "SGS path not built yet");
+}
+
+// ---------------------------------------------------------------------
+// measurement helpers
+// ---------------------------------------------------------------------
+extern "C" int sam_prof_enable(sam_handle *h, int on)
+{
+    if (!h) return set_err(SAM_ERR_VALUE, "null handle");
+    HIPCHK(hipStreamSynchronize(h->st));
+    prof_collect(h);
+    h->prof = on != 0;
+    return SAM_OK;
+}
+
+extern "C" int sam_prof_read(sam_handle *h, double *ms, long long *count)
+{
+    if (!h || !ms || !count) return set_err(SAM_ERR_VALUE, "null argument");
+    HIPCHK(hipStreamSynchronize(h->st));
+    prof_collect(h);
+    for (int i = 0; i < SAM_K_COUNT; i++) {
+        ms[i] = h->prof_ms[i];
+        count[i] = h->prof_cnt[i];
+        h->prof_ms[i] = 0.0;
+        h->prof_cnt[i] = 0;
+    }
+    return SAM_OK;
+}
+
+extern "C" int sam_sync(sam_handle *h)
+{
+    if (!h) return set_err(SAM_ERR_VALUE, "null handle");
+    HIPCHK(hipDeviceSynchronize());
+    prof_collect(h);
+    return SAM_OK;
+}
+
+extern "C" int sam_device_copy(sam_handle *h, long long bytes, int reps)
+{
+    if (!h || bytes <= 0 || reps < 1) return set_err(SAM_ERR_VALUE, "bad argument");
+    const long long n = (bytes + 15) / 16;
+    int rc = ensure(h->stage0, (size_t)n * 16);
+    if (rc) return rc;
+    rc = ensure(h->stage1, (size_t)n * 16);
+    if (rc) return rc;
+    for (int r = 0; r < reps; r++) RCHK(launch_copy16(h->stage0.p, h->stage1.p, n, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    return SAM_OK;
 }

@@ -91,6 +91,9 @@ int launch_dm_coef(const double *partials, int nblocks, double c_fixed, int dyna
 template <typename T> int launch_sgs_flux(const Dims &d, const T *P, const T *smag, const DevDiag *diag, T *Q,
                                           double dxi, double dyi, double dzi, double pr_t_inv, hipStream_t st);
 
+// plain 16-byte-per-thread copy (counter calibration)
+int launch_copy16(const void *src, void *dst, long long n16, hipStream_t st);
+
 // max number of per-block partial rows the step's reductions may produce
 long long partials_capacity(const Dims &d);
 

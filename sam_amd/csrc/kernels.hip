@@ -895,6 +895,19 @@ template <typename T> int launch_convert_to(const double *src, T *dst, long long
     return launched();
 }
 
+__global__ __launch_bounds__(256) void k_copy16(const uint4 *__restrict__ s, uint4 *__restrict__ d, long long n)
+{
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) d[i] = s[i];
+}
+
+int launch_copy16(const void *src, void *dst, long long n16, hipStream_t st)
+{
+    long long nb = (n16 + 255) / 256;
+    if (nb > 65536) nb = 65536;
+    hipLaunchKernelGGL(k_copy16, dim3((unsigned)nb), dim3(256), 0, st, (const uint4 *)src, (uint4 *)dst, n16);
+    return launched();
+}
+
 long long partials_capacity(const Dims &d)
 {
     long long a = (long long)d.nz * d.ny;                            // x-row blocks x 2Lu

@@ -36,7 +36,12 @@ EXPORTS = (
     "sam_z_trans_b", "sam_xy_trans_b", "sam_xy_trans_f", "sam_z_trans_f",
     "sam_products", "sam_vel_max", "sam_project",
     "sam_eddy_visc", "sam_test_filter",
+    "sam_prof_enable", "sam_prof_read", "sam_sync", "sam_device_copy",
 )
+
+# kernel classes of sam_prof_read (SAM_K_* in include/samhip.h)
+KERNEL_CLASSES = ("zb", "yb", "xrow", "final", "yf", "zrhs", "spec", "xb", "xf", "sgs", "misc")
+K_COUNT = 16
 
 
 class SamError(RuntimeError):
@@ -115,6 +120,10 @@ def load():
     L.sam_project.argtypes = [_hp, _dp, _dp, _dp]
     L.sam_eddy_visc.argtypes = [_hp, _dp, _dp, _dp]
     L.sam_test_filter.argtypes = [_hp, _dp, _dp, ctypes.c_int]
+    L.sam_prof_enable.argtypes = [_hp, ctypes.c_int]
+    L.sam_prof_read.argtypes = [_hp, _dp, ctypes.POINTER(ctypes.c_longlong)]
+    L.sam_sync.argtypes = [_hp]
+    L.sam_device_copy.argtypes = [_hp, ctypes.c_longlong, ctypes.c_int]
     for name in EXPORTS:
         fn = getattr(L, name)
         if fn.restype is None or fn.restype is ctypes.c_int:
@@ -265,6 +274,22 @@ class Sam:
         cd = np.zeros(1)
         _chk(self.L.sam_eddy_visc(self.h, _ptr(phys), _ptr(vt), cd.ctypes.data_as(_dp)))
         return vt, cd[0]
+
+    # ---- measurement ---------------------------------------------------
+    def prof_enable(self, on=True):
+        _chk(self.L.sam_prof_enable(self.h, int(on)))
+
+    def prof_read(self):
+        ms = np.zeros(K_COUNT)
+        cnt = (ctypes.c_longlong * K_COUNT)()
+        _chk(self.L.sam_prof_read(self.h, ms.ctypes.data_as(_dp), cnt))
+        return {name: (float(ms[i]), int(cnt[i])) for i, name in enumerate(KERNEL_CLASSES)}
+
+    def sync(self):
+        _chk(self.L.sam_sync(self.h))
+
+    def device_copy(self, nbytes, reps=1):
+        _chk(self.L.sam_device_copy(self.h, int(nbytes), int(reps)))
 
     def test_filter(self, f):
         f = np.asfortranarray(f, dtype=np.float64)

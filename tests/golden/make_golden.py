@@ -1,0 +1,68 @@
+"""Write the golden fixtures of tests/golden/ from the reference's own code.
+
+Runs in the development container only (it needs oracle/_ref/libsamref.so,
+which `make -C oracle ref` compiles from /root/reference's fft.f, init.f,
+fft_init.f, set_range.f, z_trans_b.f, z_trans_f.f, xy_trans_b.f and
+xy_trans_f.f).  The fixtures are inputs and the reference routines' outputs
+on them; tests compare the oracle and the GPU against them.
+
+    python tests/golden/make_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+from oracle import oracle as O  # noqa: E402
+
+
+def fftpack_fixture():
+    rng = np.random.default_rng(1985)
+    out = {}
+    for n in (8, 12, 16, 30, 64, 512):
+        c = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+        r = rng.standard_normal(n)
+        out[f"c_in_{n}"] = c
+        out[f"cfftf_{n}"] = O.Reference.cfft(c, -1)
+        out[f"cfftb_{n}"] = O.Reference.cfft(c, +1)
+        out[f"r_in_{n}"] = r
+        out[f"rfftf_{n}"] = O.Reference.rfft(r, -1)
+        out[f"rfftb_{n}"] = O.Reference.rfft(r, +1)
+    np.savez_compressed(os.path.join(HERE, "fftpack.npz"), **out)
+
+
+def transforms_fixture(name, nx, ny, nz, xl, yl, zl, n_dealias, i_strat, seed):
+    R = O.Reference(nx, ny, nz, xl, yl, zl, n_dealias, i_strat)
+    rng = np.random.default_rng(seed)
+    nv = R.lu
+    sh = (R.nze, R.nym, R.nxm, nv)
+    u = np.asfortranarray(rng.standard_normal(sh) + 1j * rng.standard_normal(sh))
+    u[R.nzm:] = 0.0
+    zb = R.z_trans_b(u, nv)                                   # (Ny_min, Nx_min, Nz, n)
+    # x2z on one rank: pad y, x modes above Nx_min are unused by xy_trans_b
+    nxh = nx // 2
+    y = np.zeros((ny, nxh, nz, nv), np.complex128, order="F")
+    ky = R.kym
+    y[: ky + 1, : R.nxm] = zb[: ky + 1]
+    y[ny - ky:, : R.nxm] = zb[R.nym - ky:]
+    phys = R.xy_trans_b(y, nv)                                # (Nx, Ny, Nz, n)
+    p_in = np.asfortranarray(rng.standard_normal((nx, ny, nz, R.lr)))
+    xyf = R.xy_trans_f(p_in, R.lr)                           # (Nz, Ny_min, Nx_min, Lr)
+    zf = R.z_trans_f(xyf, R.lr)                               # (Nz_min, Ny_min, Nx_min, Lr)
+    dims = np.array([nx, ny, nz, n_dealias, i_strat, R.kxm, R.kym, R.kzm, R.nxm, R.nym, R.nzm])
+    np.savez_compressed(os.path.join(HERE, f"transforms_{name}.npz"), dims=dims,
+                        box=np.array([xl, yl, zl]), u=u, z_trans_b=zb, x2z=y, xy_trans_b=phys,
+                        p_in=p_in, xy_trans_f=xyf, z_trans_f=zf)
+
+
+if __name__ == "__main__":
+    if not O.ref_available():
+        sys.exit("oracle/_ref/libsamref.so missing: run `make -C oracle ref` in the container with /root/reference")
+    fftpack_fixture()
+    transforms_fixture("16s", 16, 16, 16, 2 * np.pi, 2 * np.pi, 2 * np.pi, 1, 1, 7)
+    transforms_fixture("16x8x32", 16, 8, 32, 3.0, 5.0, 7.0, 1, 0, 8)
+    transforms_fixture("16d0", 16, 16, 16, 2 * np.pi, 2 * np.pi, 2 * np.pi, 0, 0, 9)
+    print("golden fixtures written to", HERE)
