@@ -420,6 +420,99 @@ static int fetch_diag(sam_handle *h, sam_diag *o)
     return SAM_OK;
 }
 
+// physical-space LES fluxes: Y (Lu) -> P -> SGS model -> Q (Lr) -> Y (Lr)
+struct LesBufs {
+    void *smag, *S6, *MS, *UU, *UH, *SH, *MH, *LH, *tmp;
+};
+
+template <typename T> static int les_bufs(sam_handle *h, LesBufs &b, bool dynamic)
+{
+    const Dims &d = h->d;
+    const size_t f = sizeof(T) * d.pfield;
+    int rc = ensure(h->Pb, f * (d.lu + d.lr));
+    if (rc) return rc;
+    rc = ensure(h->sgsw, f * (dynamic ? 41 : 1));
+    if (rc) return rc;
+    char *w = (char *)h->sgsw.p;
+    b.smag = w;
+    if (!dynamic) {
+        b.S6 = b.MS = b.UU = b.UH = b.SH = b.MH = b.LH = b.tmp = nullptr;
+        return SAM_OK;
+    }
+    b.S6 = w + f * 1;
+    b.MS = w + f * 7;
+    b.UU = w + f * 13;
+    b.UH = w + f * 19;
+    b.SH = w + f * 22;
+    b.MH = w + f * 28;
+    b.LH = w + f * 34;
+    b.tmp = w + f * 40;
+    return SAM_OK;
+}
+
+static double delta_sq(const sam_handle *h)
+{
+    const double dx = h->p.xl / h->p.nx, dy = h->p.yl / h->p.ny, dz = h->p.zl / h->p.nz;
+    return pow(dx * dy * dz, 2.0 / 3.0);   // init.f:28
+}
+
+// eddy-viscosity coefficient C Delta^2 into diag->c_delta_sq (and |S| into b.smag)
+template <typename T> static int sgs_coefficient(sam_handle *h, const T *P, LesBufs &b, DevDiag *dg)
+{
+    const Dims &d = h->d;
+    const double dxi = d.nx / h->p.xl, dyi = d.ny / h->p.yl, dzi = d.nz / h->p.zl;   // 1/dx (init.f:22-27)
+    const bool dyn = h->p.sgs == SAM_SGS_DYN;
+    RCHK(launch_strain_smag<T>(d, P, (T *)b.smag, dyn ? (T *)b.S6 : nullptr, dxi, dyi, dzi, h->st));
+    if (!dyn) {
+        RCHK(launch_dm_coef(h->partials, 0, h->p.c_smag * delta_sq(h), 0, dg, h->st));
+        return SAM_OK;
+    }
+    RCHK(launch_dm_fields<T>(d, P, (const T *)b.smag, (const T *)b.S6, (T *)b.MS, (T *)b.UU, h->st));
+    RCHK(launch_filter<T>(d, P, (T *)b.UH, (T *)b.tmp, 3, h->st));
+    RCHK(launch_filter<T>(d, (const T *)b.S6, (T *)b.SH, (T *)b.tmp, 6, h->st));
+    RCHK(launch_filter<T>(d, (const T *)b.MS, (T *)b.MH, (T *)b.tmp, 6, h->st));
+    RCHK(launch_filter<T>(d, (const T *)b.UU, (T *)b.LH, (T *)b.tmp, 6, h->st));
+    int nb = 0;
+    RCHK(launch_dm_reduce<T>(d, (const T *)b.UH, (const T *)b.SH, (const T *)b.MH, (const T *)b.LH, h->partials,
+                             h->st, &nb));
+    RCHK(launch_dm_coef(h->partials, nb, 0.0, 1, dg, h->st));
+    return SAM_OK;
+}
+
+template <typename T> static int les_fluxes(sam_handle *h, int stats)
+{
+    const Dims &d = h->d;
+    const int pi = h->prec;
+    if (h->p.sgs == SAM_SGS_MLP) return set_err(SAM_ERR_UNSUPPORTED, "learned closure not enabled on this handle");
+    LesBufs b;
+    int rc = les_bufs<T>(h, b, h->p.sgs == SAM_SGS_DYN);
+    if (rc) return rc;
+    T *P = (T *)h->Pb.p, *Q = P + d.pfield * d.lu;
+    T *Y = (T *)h->Yb;
+    {
+        PROF(SAM_K_XB);
+        RCHK(launch_xb<T>(d, Y, d.lu, P, h->tw[0][pi], h->st));
+    }
+    if (stats) {
+        PROF(SAM_K_FINAL);
+        int nb = 0;
+        RCHK(launch_stats_phys(d, P, sizeof(T) == sizeof(double), d.lu, h->partials, h->diag, h->st, &nb));
+        RCHK(launch_finalize(d, h->k, h->partials, nb, h->diag, 1, h->st));
+    }
+    {
+        PROF(SAM_K_SGS);
+        rc = sgs_coefficient<T>(h, P, b, h->diag);
+        if (rc) return rc;
+        RCHK(launch_sgs_flux<T>(d, P, (const T *)b.smag, h->diag, Q, d.nx / h->p.xl, d.ny / h->p.yl,
+                                d.nz / h->p.zl, 1.0 / h->p.pr_t, h->st));
+    }
+    {
+        PROF(SAM_K_XF);
+        RCHK(launch_xf<T>(d, Q, d.lr, Y, h->tw[0][pi], h->st));
+    }
+    return SAM_OK;
+}
+
 // Everything up to (and including) the forward y transform: leaves the
 // x,y-transformed fluxes of the state U[cur] in Zb (Lr fields, Z layout).
 template <typename T> static int fluxes_to_z(sam_handle *h, int stats)
@@ -446,7 +539,8 @@ template <typename T> static int fluxes_to_z(sam_handle *h, int stats)
             RCHK(launch_finalize(d, h->k, h->partials, nb, h->diag, 1, h->st));
         }
     } else {
-        return set_err(SAM_ERR_UNSUPPORTED, "SGS path not built yet");
+        int rc = les_fluxes<T>(h, stats);
+        if (rc) return rc;
     }
     {
         PROF(SAM_K_YF);
@@ -701,18 +795,54 @@ extern "C" int sam_project(sam_handle *h, const double *F, const double *usav, d
     return SAM_OK;
 }
 
+template <typename T> static int eddy_stage(sam_handle *h, const double *phys, double *vt, double *cdsq)
+{
+    const Dims &d = h->d;
+    LesBufs b;
+    int rc = les_bufs<T>(h, b, h->p.sgs == SAM_SGS_DYN);
+    if (rc) return rc;
+    T *P = (T *)h->Pb.p;
+    rc = upload_T<T>(h, P, phys, d.pfield * d.lu);
+    if (rc) return rc;
+    DevDiag *sd = h->diag + 1;
+    rc = sgs_coefficient<T>(h, P, b, sd);
+    if (rc) return rc;
+    rc = ensure(h->stage0, sizeof(double) * d.pfield);
+    if (rc) return rc;
+    RCHK(launch_nut<T>(d, (const T *)b.smag, sd, (double *)h->stage0.p, h->st));
+    HIPCHK(hipMemcpyAsync(vt, h->stage0.p, sizeof(double) * d.pfield, hipMemcpyDeviceToHost, h->st));
+    DevDiag dg;
+    HIPCHK(hipMemcpyAsync(&dg, sd, sizeof dg, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    if (cdsq) *cdsq = dg.c_delta_sq;
+    return SAM_OK;
+}
+
 extern "C" int sam_eddy_visc(sam_handle *h, const double *phys, double *vt, double *c_delta_sq)
 {
-    (void)phys; (void)vt; (void)c_delta_sq;
-    if (!h) return set_err(SAM_ERR_VALUE, "null handle");
-    return set_err(SAM_ERR_UNSUPPORTED, "SGS path not built yet");
+    if (!h || !phys || !vt) return set_err(SAM_ERR_VALUE, "null argument");
+    if (h->p.sgs != SAM_SGS_SMAG && h->p.sgs != SAM_SGS_DYN)
+        return set_err(SAM_ERR_STATE, "handle has no eddy-viscosity model (sgs must be SMAG or DYN)");
+    return h->prec ? eddy_stage<float>(h, phys, vt, c_delta_sq) : eddy_stage<double>(h, phys, vt, c_delta_sq);
+}
+
+template <typename T> static int filter_stage(sam_handle *h, const double *in, double *out, int nvar)
+{
+    const Dims &d = h->d;
+    int rc = ensure(h->Pb, sizeof(T) * d.pfield * (2 * nvar + 1));
+    if (rc) return rc;
+    T *A = (T *)h->Pb.p, *B = A + d.pfield * nvar, *tmp = B + d.pfield * nvar;
+    rc = upload_T<T>(h, A, in, d.pfield * nvar);
+    if (rc) return rc;
+    RCHK(launch_filter<T>(d, A, B, tmp, nvar, h->st));
+    return download_T<T>(h, out, B, d.pfield * nvar);
 }
 
 extern "C" int sam_test_filter(sam_handle *h, const double *in, double *out, int nvar)
 {
-    (void)in; (void)out; (void)nvar;
-    if (!h) return set_err(SAM_ERR_VALUE, "null handle");
-    return set_err(SAM_ERR_UNSUPPORTED, "SGS path not built yet");
+    if (!h || !in || !out) return set_err(SAM_ERR_VALUE, "null argument");
+    if (nvar < 1 || nvar > 64) return set_err(SAM_ERR_VALUE, "nvar must be 1..64");
+    return h->prec ? filter_stage<float>(h, in, out, nvar) : filter_stage<double>(h, in, out, nvar);
 }
 
 // ---------------------------------------------------------------------

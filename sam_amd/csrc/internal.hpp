@@ -90,6 +90,7 @@ template <typename T> int launch_dm_reduce(const Dims &d, const T *uh, const T *
 int launch_dm_coef(const double *partials, int nblocks, double c_fixed, int dynamic, DevDiag *diag, hipStream_t st);
 template <typename T> int launch_sgs_flux(const Dims &d, const T *P, const T *smag, const DevDiag *diag, T *Q,
                                           double dxi, double dyi, double dzi, double pr_t_inv, hipStream_t st);
+template <typename T> int launch_nut(const Dims &d, const T *smag, const DevDiag *diag, double *vt, hipStream_t st);
 
 // plain 16-byte-per-thread copy (counter calibration)
 int launch_copy16(const void *src, void *dst, long long n16, hipStream_t st);

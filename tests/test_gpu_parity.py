@@ -162,3 +162,47 @@ def test_split_precision_bounded(oracle_mod):
     ref, t, dt, st = o.step(u0, 5)
     err = rel_l2(got, ref)
     assert err < 1e-5, err
+
+
+# ---------------------------------------------------------------------------
+# LES: Smagorinsky (sm.f) and dynamic (dm.f) closures on the collocated grid
+# ---------------------------------------------------------------------------
+def les_pair(oracle_mod, n, sgs, prec=S.PREC_FP64, **kw):
+    O = oracle_mod
+    cfg = O.make_cfg(n, sgs=sgs, **kw)
+    o = O.Oracle(cfg)
+    p = S.make_params(n, xl=cfg.xl, n_dealias=cfg.n_dealias, i_strat=cfg.i_strat, i_cfl=cfg.i_cfl,
+                      precision=prec, sgs=sgs, vis=cfg.vis, pr=cfg.pr, grav=cfg.grav, to=cfg.to,
+                      lapse=cfg.lapse, cfl0=cfg.cfl0, dt0=cfg.dt0, c_smag=cfg.c_smag, pr_t=cfg.pr_t)
+    return o, S.Sam(p)
+
+
+@pytest.mark.parametrize("sgs", [S.SGS_SMAG, S.SGS_DYN])
+def test_eddy_viscosity(oracle_mod, sgs):
+    o, s = les_pair(oracle_mod, 32, sgs, i_strat=1, c_smag=0.03)
+    u = ic.random_solenoidal(32, k0=6, seed=31, i_strat=1)
+    phys = o.to_physical(u)
+    vt_ref, cd_ref = o.eddy_visc(phys)
+    vt, cd = s.eddy_visc(phys)
+    assert abs(cd - cd_ref) <= 1e-11 * abs(cd_ref)
+    assert rel_l2(vt, vt_ref) < 1e-11
+
+
+def test_test_filter(oracle_mod):
+    o, s = les_pair(oracle_mod, 32, S.SGS_DYN)
+    rng = np.random.default_rng(2)
+    f = np.asfortranarray(rng.standard_normal((32, 32, 32)))
+    assert rel_l2(s.test_filter(f), o.test_filter(f)) < 1e-14
+
+
+@pytest.mark.parametrize("sgs", [S.SGS_SMAG, S.SGS_DYN])
+def test_les_steps(oracle_mod, sgs):
+    o, s = les_pair(oracle_mod, 32, sgs, i_strat=1, grav=64.0, lapse0=1.0, vis=1e-5, i_cfl=1,
+                    cfl0=0.5, c_smag=0.03)
+    u0 = ic.random_solenoidal(32, k0=6, seed=41, i_strat=1)
+    s.set_state(u0)
+    d = s.step(5)
+    ref, t, dt, st = o.step(u0, 5)
+    assert rel_l2(s.get_state(), ref) < 1e-11
+    if sgs == S.SGS_DYN:
+        assert abs(d["c_delta_sq"] - st[17]) <= 1e-10 * abs(st[17])
