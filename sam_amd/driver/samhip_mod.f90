@@ -1,0 +1,114 @@
+! samhip_mod.f90 -- iso_c_binding interface to libsamhip (include/samhip.h).
+!
+! The derived types mirror the C structs field for field; every entry point
+! returns the library's int status (0 = ok).
+module samhip_mod
+  use iso_c_binding
+  implicit none
+
+  integer, parameter :: SAM_OK = 0, SAM_ERR_REQUIRED = 6, SAM_ERR_VALUE = 9, &
+       SAM_ERR_SIZE = 10, SAM_ERR_UNSUPPORTED = 20
+  integer, parameter :: SAM_PREC_FP64 = 0, SAM_PREC_SPLIT = 1
+  integer, parameter :: SAM_SGS_NONE = 0, SAM_SGS_SMAG = 1, SAM_SGS_DYN = 2, SAM_SGS_MLP = 3
+
+  type, bind(c) :: sam_params
+     integer(c_int) :: nx, ny, nz, n_dealias, i_strat, nrk_max, i_cfl, iubc_z, precision, sgs
+     real(c_double) :: xl, yl, zl, vis, pr, grav, to, lapse, cfl0, dt0, c_smag, pr_t
+  end type sam_params
+
+  type, bind(c) :: sam_dims
+     integer(c_int) :: kx_max, ky_max, kz_max, nx_min, ny_min, nz_min, nze, lu, lr, lm
+  end type sam_dims
+
+  type, bind(c) :: sam_diag
+     integer(c_long_long) :: nt
+     real(c_double) :: time, dt
+     real(c_double) :: u_avg(4), u_var(4), u_max(4)
+     real(c_double) :: energy, cfl_x, cfl_y, cfl_z, c_delta_sq
+     integer(c_int) :: sgs_active, pad_
+  end type sam_diag
+
+  interface
+     integer(c_int) function sam_create(p, h) bind(c, name='sam_create')
+       import :: c_int, c_ptr, sam_params
+       type(sam_params), intent(in) :: p
+       type(c_ptr), intent(out) :: h
+     end function sam_create
+
+     integer(c_int) function sam_destroy(h) bind(c, name='sam_destroy')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: h
+     end function sam_destroy
+
+     integer(c_int) function sam_get_dims(h, d) bind(c, name='sam_get_dims')
+       import :: c_int, c_ptr, sam_dims
+       type(c_ptr), value :: h
+       type(sam_dims), intent(out) :: d
+     end function sam_get_dims
+
+     type(c_ptr) function sam_last_error() bind(c, name='sam_last_error')
+       import :: c_ptr
+     end function sam_last_error
+
+     integer(c_int) function sam_set_state(h, u, ldz) bind(c, name='sam_set_state')
+       import :: c_int, c_ptr, c_double_complex
+       type(c_ptr), value :: h
+       complex(c_double_complex), intent(in) :: u(*)
+       integer(c_int), value :: ldz
+     end function sam_set_state
+
+     integer(c_int) function sam_get_state(h, u, ldz) bind(c, name='sam_get_state')
+       import :: c_int, c_ptr, c_double_complex
+       type(c_ptr), value :: h
+       complex(c_double_complex), intent(out) :: u(*)
+       integer(c_int), value :: ldz
+     end function sam_get_state
+
+     integer(c_int) function sam_set_clock(h, time, dt, nt) bind(c, name='sam_set_clock')
+       import :: c_int, c_ptr, c_double, c_long_long
+       type(c_ptr), value :: h
+       real(c_double), value :: time, dt
+       integer(c_long_long), value :: nt
+     end function sam_set_clock
+
+     integer(c_int) function sam_step(h, nsteps, want_spectra, diag) bind(c, name='sam_step')
+       import :: c_int, c_ptr, sam_diag
+       type(c_ptr), value :: h
+       integer(c_int), value :: nsteps, want_spectra
+       type(sam_diag), intent(out) :: diag
+     end function sam_step
+
+     integer(c_int) function sam_rhs(h, r, ldz, want_spectra, diag) bind(c, name='sam_rhs')
+       import :: c_int, c_ptr, sam_diag
+       type(c_ptr), value :: h
+       type(c_ptr), value :: r
+       integer(c_int), value :: ldz, want_spectra
+       type(sam_diag), intent(out) :: diag
+     end function sam_rhs
+
+     integer(c_int) function sam_get_spectra(h, out) bind(c, name='sam_get_spectra')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(out) :: out(*)
+     end function sam_get_spectra
+  end interface
+
+contains
+
+  ! message of the library's last error as a Fortran string
+  function sam_error_message() result(msg)
+    character(len=512) :: msg
+    type(c_ptr) :: p
+    character(kind=c_char), pointer :: s(:)
+    integer :: i
+    msg = ' '
+    p = sam_last_error()
+    if (.not. c_associated(p)) return
+    call c_f_pointer(p, s, [512])
+    do i = 1, 512
+       if (s(i) == c_null_char) exit
+       msg(i:i) = s(i)
+    end do
+  end function sam_error_message
+
+end module samhip_mod
