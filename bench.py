@@ -103,6 +103,18 @@ def dominant_roofline(n, bc, prof, steps):
                 avg_launch_us=round(avg_s * 1e6, 2), launches=cnt), per_kernel
 
 
+def measured_traffic(n, kernel, prec):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (profiles/step_<n>_hbm_traffic.json), if one exists"""
+    path = os.path.join(ROOT, "profiles", f"step_{n}_hbm_traffic.json")
+    try:
+        with open(path) as fh:
+            t = json.load(fh)
+        return t["kernels"][f"{kernel}_{prec}"]["hbm_bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def cpu_baseline(n_cpu, steps_cpu, u0_small):
     from oracle import oracle as O
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
@@ -127,6 +139,8 @@ def main():
     ap.add_argument("--no-split", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=0, help="CPU baseline grid (default: same n if RAM allows)")
     ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run the plain copy kernel (1 GiB x 3) for HBM counter calibration")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -148,6 +162,7 @@ def main():
     ms_step = r64["wall"] / args.steps * 1e3
     value = n ** 3 * args.steps / r64["wall"] / 1e6
     roof, per_kernel = dominant_roofline(n, 16, r64["prof"], args.steps)
+    roof["traffic"] = measured_traffic(n, roof["kernel"], "f64")
     sb = step_bytes(n, 16)
     step_frac = sb / (r64["wall"] / args.steps) / HBM_PEAK
     log(f"[bench] FP64 {ms_step:.2f} ms/step, {value:.1f} Mpt-steps/s, step HBM frac {step_frac:.3f}")
@@ -173,12 +188,19 @@ def main():
         "kernels": per_kernel,
     }
 
+    if args.calibrate_copy:
+        cal = S.Sam(make_params(S, 64, S.PREC_FP64))
+        cal.device_copy(1 << 30, 3)
+        cal.close()
+        out["copy_calibration_bytes"] = {"read": 1 << 30, "write": 1 << 30}
+
     if not args.no_split:
         log("[bench] FP32-split run")
         rsp = run_gpu(n, S.PREC_SPLIT, args.steps, args.warmup, u0, S)
         ms_sp = rsp["wall"] / args.steps * 1e3
         sb32 = step_bytes(n, 8)
         roof32, pk32 = dominant_roofline(n, 8, rsp["prof"], args.steps)
+        roof32["traffic"] = measured_traffic(n, roof32["kernel"], "f32")
         e64, e32 = r64["spec"]["energy_w"], rsp["spec"]["energy_w"]
         d64, d32 = np.sum(r64["spec"]["dk"]), np.sum(rsp["spec"]["dk"])
         ek64, ek32 = r64["spec"]["ek"], rsp["spec"]["ek"]

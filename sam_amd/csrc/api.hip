@@ -415,7 +415,7 @@ static int fetch_diag(sam_handle *h, sam_diag *o)
     o->cfl_x = dg.cfl_x;
     o->cfl_y = dg.cfl_y;
     o->cfl_z = dg.cfl_z;
-    o->c_delta_sq = dg.c_delta_sq;
+    o->c_delta_sq = dg.spare[3];
     o->sgs_active = h->sgs_active;
     return SAM_OK;
 }
@@ -457,14 +457,14 @@ static double delta_sq(const sam_handle *h)
 }
 
 // eddy-viscosity coefficient C Delta^2 into diag->c_delta_sq (and |S| into b.smag)
-template <typename T> static int sgs_coefficient(sam_handle *h, const T *P, LesBufs &b, DevDiag *dg)
+template <typename T> static int sgs_coefficient(sam_handle *h, const T *P, LesBufs &b, DevDiag *dg, int report)
 {
     const Dims &d = h->d;
     const double dxi = d.nx / h->p.xl, dyi = d.ny / h->p.yl, dzi = d.nz / h->p.zl;   // 1/dx (init.f:22-27)
     const bool dyn = h->p.sgs == SAM_SGS_DYN;
     RCHK(launch_strain_smag<T>(d, P, (T *)b.smag, dyn ? (T *)b.S6 : nullptr, dxi, dyi, dzi, h->st));
     if (!dyn) {
-        RCHK(launch_dm_coef(h->partials, 0, h->p.c_smag * delta_sq(h), 0, dg, h->st));
+        RCHK(launch_dm_coef(h->partials, 0, h->p.c_smag * delta_sq(h), 0, report, dg, h->st));
         return SAM_OK;
     }
     RCHK(launch_dm_fields<T>(d, P, (const T *)b.smag, (const T *)b.S6, (T *)b.MS, (T *)b.UU, h->st));
@@ -475,7 +475,7 @@ template <typename T> static int sgs_coefficient(sam_handle *h, const T *P, LesB
     int nb = 0;
     RCHK(launch_dm_reduce<T>(d, (const T *)b.UH, (const T *)b.SH, (const T *)b.MH, (const T *)b.LH, h->partials,
                              h->st, &nb));
-    RCHK(launch_dm_coef(h->partials, nb, 0.0, 1, dg, h->st));
+    RCHK(launch_dm_coef(h->partials, nb, 0.0, 1, report, dg, h->st));
     return SAM_OK;
 }
 
@@ -501,7 +501,7 @@ template <typename T> static int les_fluxes(sam_handle *h, int stats)
     }
     {
         PROF(SAM_K_SGS);
-        rc = sgs_coefficient<T>(h, P, b, h->diag);
+        rc = sgs_coefficient<T>(h, P, b, h->diag, stats);
         if (rc) return rc;
         RCHK(launch_sgs_flux<T>(d, P, (const T *)b.smag, h->diag, Q, d.nx / h->p.xl, d.ny / h->p.yl,
                                 d.nz / h->p.zl, 1.0 / h->p.pr_t, h->st));
@@ -805,7 +805,7 @@ template <typename T> static int eddy_stage(sam_handle *h, const double *phys, d
     rc = upload_T<T>(h, P, phys, d.pfield * d.lu);
     if (rc) return rc;
     DevDiag *sd = h->diag + 1;
-    rc = sgs_coefficient<T>(h, P, b, sd);
+    rc = sgs_coefficient<T>(h, P, b, sd, 1);
     if (rc) return rc;
     rc = ensure(h->stage0, sizeof(double) * d.pfield);
     if (rc) return rc;
@@ -814,7 +814,7 @@ template <typename T> static int eddy_stage(sam_handle *h, const double *phys, d
     DevDiag dg;
     HIPCHK(hipMemcpyAsync(&dg, sd, sizeof dg, hipMemcpyDeviceToHost, h->st));
     HIPCHK(hipStreamSynchronize(h->st));
-    if (cdsq) *cdsq = dg.c_delta_sq;
+    if (cdsq) *cdsq = dg.spare[3];
     return SAM_OK;
 }
 

@@ -87,7 +87,8 @@ template <typename T> int launch_dm_fields(const Dims &d, const T *P, const T *s
                                            hipStream_t st);
 template <typename T> int launch_dm_reduce(const Dims &d, const T *uh, const T *sh, const T *mh, const T *lh,
                                            double *partials, hipStream_t st, int *nblocks_out);
-int launch_dm_coef(const double *partials, int nblocks, double c_fixed, int dynamic, DevDiag *diag, hipStream_t st);
+int launch_dm_coef(const double *partials, int nblocks, double c_fixed, int dynamic, int report, DevDiag *diag,
+                   hipStream_t st);
 template <typename T> int launch_sgs_flux(const Dims &d, const T *P, const T *smag, const DevDiag *diag, T *Q,
                                           double dxi, double dyi, double dzi, double pr_t_inv, hipStream_t st);
 template <typename T> int launch_nut(const Dims &d, const T *smag, const DevDiag *diag, double *vt, hipStream_t st);

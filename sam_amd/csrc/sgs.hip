@@ -209,7 +209,7 @@ int launch_dm_reduce(const Dims &d, const T *uh, const T *sh, const T *mh, const
 }
 
 __global__ __launch_bounds__(256) void k_dm_coef(const double *__restrict__ partials, int nblocks, double c_fixed,
-                                                  int dynamic, DevDiag *__restrict__ diag)
+                                                  int dynamic, int report, DevDiag *__restrict__ diag)
 {
     __shared__ double a[256], b[256];
     double s0 = 0.0, s1 = 0.0;
@@ -228,12 +228,17 @@ __global__ __launch_bounds__(256) void k_dm_coef(const double *__restrict__ part
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) diag->c_delta_sq = dynamic ? -0.5 * a[0] / b[0] : c_fixed;
+    if (threadIdx.x == 0) {
+        const double c = dynamic ? -0.5 * a[0] / b[0] : c_fixed;
+        diag->c_delta_sq = c;                 // used by the flux kernel of this substep
+        if (report) diag->spare[3] = c;       // reported value: first substep of the step
+    }
 }
 
-int launch_dm_coef(const double *partials, int nblocks, double c_fixed, int dynamic, DevDiag *diag, hipStream_t st)
+int launch_dm_coef(const double *partials, int nblocks, double c_fixed, int dynamic, int report, DevDiag *diag,
+                   hipStream_t st)
 {
-    hipLaunchKernelGGL(k_dm_coef, dim3(1), dim3(256), 0, st, partials, nblocks, c_fixed, dynamic, diag);
+    hipLaunchKernelGGL(k_dm_coef, dim3(1), dim3(256), 0, st, partials, nblocks, c_fixed, dynamic, report, diag);
     return launched_sgs();
 }
 
