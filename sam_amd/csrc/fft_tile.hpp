@@ -108,6 +108,42 @@ template <int S, typename T> __device__ __forceinline__ void dft8(cpx<T> *v)
     v[7] = e3 - w3o3;
 }
 
+// LDS ordering between passes.  When every column of the tile belongs to a
+// single wavefront (threads per column divide 64), the exchange only has to
+// order this wave's own LDS accesses: wait for them and stop the compiler
+// from moving LDS accesses across; otherwise a workgroup barrier.
+template <bool WAVE> __device__ __forceinline__ void tile_sync()
+{
+    if constexpr (WAVE) {
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    } else {
+        __syncthreads();
+    }
+}
+
+// twiddles w^r, r = 0..R-1, of w = exp(S 2 pi i e0 / N) from the table
+// tw[k] = exp(-2 pi i k/N): w^1, w^2, w^4 are read, the rest multiplied
+template <int R, int S, typename T>
+__device__ __forceinline__ void twiddles(const cpx<T> *__restrict__ tw, int e0, cpx<T> *w)
+{
+    auto ld = [&](int e) {
+        cpx<T> t = tw[e];
+        if constexpr (S > 0) t.y = -t.y;
+        return t;
+    };
+    w[1] = ld(e0);
+    if constexpr (R >= 4) {
+        w[2] = ld(2 * e0);
+        w[3] = cmul(w[1], w[2]);
+    }
+    if constexpr (R == 8) {
+        w[4] = ld(4 * e0);
+        w[5] = cmul(w[1], w[4]);
+        w[6] = cmul(w[2], w[4]);
+        w[7] = cmul(w[3], w[4]);
+    }
+}
+
 // Butterfly of radix R on v[0..R-1] (inputs in stride order r)
 template <int R, int S, typename T> __device__ __forceinline__ void dftR(cpx<T> *v)
 {
@@ -123,6 +159,7 @@ template <int R, int S, typename T> __device__ __forceinline__ void dftR(cpx<T> 
 template <typename T, int N, int EPT> struct Tile {
     static_assert(EPT == 8 || EPT == 16, "EPT");
     static constexpr int TPC = N / EPT;
+    static constexpr bool WAVE = TPC <= 64 && (64 % TPC) == 0;   // column inside one wavefront
     using SC = Sched<N>;
 
     // k index (element j0 + k*TPC) held in slot s at the start of pass p
@@ -156,12 +193,10 @@ template <typename T, int N, int EPT> struct Tile {
                 const int b = j0 + m * TPC;
                 const int jl = b & (Ns - 1);
                 constexpr int unit = N / (Ns * R);
+                cpx<T> w[R];
+                twiddles<R, S>(tw, jl * unit, w);
 #pragma unroll
-                for (int r = 1; r < R; r++) {
-                    cpx<T> w = tw[r * jl * unit];
-                    if constexpr (S > 0) w.y = -w.y;
-                    v[m * R + r] = cmul(v[m * R + r], w);
-                }
+                for (int r = 1; r < R; r++) v[m * R + r] = cmul(v[m * R + r], w[r]);
             }
             dftR<R, S>(v + m * R);
         }
@@ -183,27 +218,30 @@ template <typename T, int N, int EPT> struct Tile {
 
     // passes p0..P-1 starting from registers already holding pass-p0
     // inputs; the final pass leaves its outputs in v (positions kout).
-    // Requires a __syncthreads() before the call if other threads wrote
-    // the LDS tile region this thread's column uses.
+    // Requires a __syncthreads() before the call if other threads may still
+    // access the LDS tile region this thread's column uses.
     template <int S, int p = 0>
     __device__ static __forceinline__ void run_regs(cpx<T> *lds, const cpx<T> *__restrict__ tw, int c, int j0, cpx<T> *v)
     {
         compute<p, S>(tw, j0, v);
         if constexpr (p + 1 < SC::P) {
             lds_write<p>(lds, c, j0, v);
-            __syncthreads();
+            tile_sync<WAVE>();
             lds_read<p + 1>(lds, c, j0, v);
-            __syncthreads();
+            tile_sync<WAVE>();
             run_regs<S, p + 1>(lds, tw, c, j0, v);
         }
     }
 
-    // LDS -> registers (pass-0 inputs) -> FFT -> registers (final outputs)
+    // LDS -> registers (pass-0 inputs) -> FFT -> registers (final outputs).
+    // Touches only column c of the tile; when WAVE, other waves may still be
+    // working on their columns at return, so a caller that next writes other
+    // columns must __syncthreads() first.
     template <int S>
     __device__ static __forceinline__ void run_from_lds(cpx<T> *lds, const cpx<T> *__restrict__ tw, int c, int j0, cpx<T> *v)
     {
         lds_read<0>(lds, c, j0, v);
-        __syncthreads();
+        tile_sync<WAVE>();
         run_regs<S, 0>(lds, tw, c, j0, v);
     }
 

@@ -235,7 +235,7 @@ __host__ __device__ constexpr int prod_i(int q) { return q == 0 ? 0 : q == 1 ? 0
 __host__ __device__ constexpr int prod_j(int q) { return q == 0 ? 0 : q == 1 ? 1 : q == 2 ? 1 : q == 3 ? 2 : q == 4 ? 2 : q == 5 ? 2 : 3; }
 
 template <typename T, int N, int CP, int LU, int LR>
-__global__ __launch_bounds__(CP * N / 8) void k_xrow(T *__restrict__ Yr, Dims d, const cpx<T> *__restrict__ tw,
+__global__ __launch_bounds__(CP * N / 8, 2) void k_xrow(T *__restrict__ Yr, Dims d, const cpx<T> *__restrict__ tw,
                                                       int want_stats, double *__restrict__ partials,
                                                       DevDiag *__restrict__ diag)
 {
@@ -255,6 +255,7 @@ __global__ __launch_bounds__(CP * N / 8) void k_xrow(T *__restrict__ Yr, Dims d,
     cpx<T> uf[LU][8];
 #pragma unroll
     for (int f = 0; f < LU; f++) {
+        if (f > 0) __syncthreads();   // other waves may still be reading their columns
         xrow_load_pairs<T, N, CP, NT>(lds, Y + f * fstride + zoff, d.ny, y0, d.kxm);
         __syncthreads();
         TL::template run_from_lds<+1>(lds, tw, c, j0, uf[f]);
