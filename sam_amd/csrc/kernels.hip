@@ -234,15 +234,58 @@ __device__ __forceinline__ double wave_max(double v)
 __host__ __device__ constexpr int prod_i(int q) { return q == 0 ? 0 : q == 1 ? 0 : q == 2 ? 1 : q == 3 ? 0 : q == 4 ? 1 : q == 5 ? 2 : q == 6 ? 0 : q == 7 ? 1 : 2; }
 __host__ __device__ constexpr int prod_j(int q) { return q == 0 ? 0 : q == 1 ? 1 : q == 2 ? 1 : q == 3 ? 2 : q == 4 ? 2 : q == 5 ? 2 : 3; }
 
+// Raw x-row tile of one field (k-rows 0..kxm of 2*CP consecutive y, 128 bytes
+// per k-row) streamed from HBM straight into LDS with global_load_lds_dwordx4:
+// one instruction moves 8 k-rows (64 lanes x 16 B) and holds no VGPRs, so the
+// next field's tile arrives while the current one is transformed.
+using lds_void_t = __attribute__((address_space(3))) void;
+
+template <typename T, int CP, int NT>
+__device__ __forceinline__ void xrow_issue_stage(cpx<T> *stg, const cpx<T> *__restrict__ Yf, int ny, int y0, int kxm)
+{
+    static_assert(2 * CP * (int)sizeof(cpx<T>) == 128, "one k-row is 128 bytes");
+    static_assert(NT >= 64 && NT % 64 == 0, "whole wavefronts issue the staged loads");
+    constexpr int NW = NT / 64;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int ninstr = (kxm + 8) / 8;
+    for (int i = w; i < ninstr; i += NW) {
+        int k = i * 8 + (lane >> 3);
+        if (k > kxm) k = kxm;      // rows past kx_max: re-read a valid row, ignored
+        const char *src = reinterpret_cast<const char *>(Yf + (long long)k * ny + y0) + (lane & 7) * 16;
+        __builtin_amdgcn_global_load_lds(src, (lds_void_t *)(stg + i * 16 * CP), 16, 0, 0);
+    }
+}
+
+__device__ __forceinline__ void wait_vmem() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
+// staged raw tile -> packed two-row spectra in the FFT tile
+template <typename T, int N, int CP, int NT>
+__device__ __forceinline__ void xrow_unpack_stage(cpx<T> *lds, const cpx<T> *stg, int kxm)
+{
+    for (int e = threadIdx.x; e < CP * (kxm + 1); e += NT) {
+        const int k = e / CP, c = e % CP;
+        cpx<T> X = stg[k * 2 * CP + 2 * c], Yv = stg[k * 2 * CP + 2 * c + 1];
+        if (k == 0) { X.y = (T)0; Yv.y = (T)0; }
+        lds[lds_idx<T, N>(c, k)] = {X.x - Yv.y, X.y + Yv.x};
+        if (k > 0) lds[lds_idx<T, N>(c, N - k)] = {X.x + Yv.y, Yv.x - X.y};
+    }
+    const int nzero = N - 2 * kxm - 1;
+    for (int e = threadIdx.x; e < CP * nzero; e += NT) {
+        const int p = kxm + 1 + e / CP, c = e % CP;
+        lds[lds_idx<T, N>(c, p)] = {(T)0, (T)0};
+    }
+}
+
 template <typename T, int N, int CP, int LU, int LR>
 __global__ __launch_bounds__(CP * N / 8, 2) void k_xrow(T *__restrict__ Yr, Dims d, const cpx<T> *__restrict__ tw,
-                                                      int want_stats, double *__restrict__ partials,
-                                                      DevDiag *__restrict__ diag)
+                                                         int want_stats, double *__restrict__ partials,
+                                                         DevDiag *__restrict__ diag)
 {
     using TL = Tile<T, N, 8>;
     constexpr int NT = CP * N / 8;
     constexpr int NW = (NT + 63) / 64;
     __shared__ cpx<T> lds[CP * N];
+    __shared__ cpx<T> stg[NT >= 64 ? (N / 2 + 8) * 2 * CP : 1];
     __shared__ double red[NW][2 * LU];
     const int nyt = d.ny / (2 * CP);
     const int yt = blockIdx.x % nyt, z = blockIdx.x / nyt;
@@ -253,12 +296,27 @@ __global__ __launch_bounds__(CP * N / 8, 2) void k_xrow(T *__restrict__ Yr, Dims
     const long long zoff = (long long)z * d.nxm * d.ny;
 
     cpx<T> uf[LU][8];
+    if constexpr (NT >= 64) {
+        // whole wavefronts: stream the next field into LDS during this field's FFT
+        xrow_issue_stage<T, CP, NT>(stg, Y + zoff, d.ny, y0, d.kxm);
 #pragma unroll
-    for (int f = 0; f < LU; f++) {
-        if (f > 0) __syncthreads();   // other waves may still be reading their columns
-        xrow_load_pairs<T, N, CP, NT>(lds, Y + f * fstride + zoff, d.ny, y0, d.kxm);
-        __syncthreads();
-        TL::template run_from_lds<+1>(lds, tw, c, j0, uf[f]);
+        for (int f = 0; f < LU; f++) {
+            wait_vmem();
+            __syncthreads();   // staged tile complete; every wave is done with the FFT tile
+            xrow_unpack_stage<T, N, CP, NT>(lds, stg, d.kxm);
+            __syncthreads();   // FFT tile ready; staging buffer free
+            if (f + 1 < LU) xrow_issue_stage<T, CP, NT>(stg, Y + (f + 1) * fstride + zoff, d.ny, y0, d.kxm);
+            TL::template run_from_lds<+1>(lds, tw, c, j0, uf[f]);
+        }
+    } else {
+        // sub-wavefront groups (short rows): direct cooperative loads
+#pragma unroll
+        for (int f = 0; f < LU; f++) {
+            if (f > 0) __syncthreads();
+            xrow_load_pairs<T, N, CP, NT>(lds, Y + f * fstride + zoff, d.ny, y0, d.kxm);
+            __syncthreads();
+            TL::template run_from_lds<+1>(lds, tw, c, j0, uf[f]);
+        }
     }
 
     if (want_stats) {
@@ -293,8 +351,6 @@ __global__ __launch_bounds__(CP * N / 8, 2) void k_xrow(T *__restrict__ Yr, Dims
     const T h = (T)(0.5 / ((double)d.nx * (double)d.ny));
 #pragma unroll
     for (int q = 0; q < LR; q++) {
-        constexpr int dummy = 0;
-        (void)dummy;
         cpx<T> po[8], pi[8];
 #pragma unroll
         for (int k = 0; k < 8; k++) {

@@ -22,7 +22,7 @@ CONFIG1 = dict(Nx=64, Ny=64, Nz=64, xL=2 * np.pi, yL=2 * np.pi, zL=2 * np.pi, vi
 
 def run(tmp_path, **params):
     sio.write_input(str(tmp_path / "input.dat"), **params)
-    return subprocess.run([DRIVER], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    return subprocess.run([DRIVER], cwd=tmp_path, capture_output=True, text=True, timeout=90)
 
 
 @needs_driver
