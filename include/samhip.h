@@ -146,6 +146,43 @@ int  sam_eddy_visc(sam_handle *h, const double *phys, double *vt, double *c_delt
 /* (1,4,1)/6 periodic test filter of nvar physical fields (filter.f:9-42) */
 int  sam_test_filter(sam_handle *h, const double *in, double *out, int nvar);
 
+/* ---- learned SGS closure (no counterpart in the reference) ------------ */
+/* Per-point MLP 12 -> 64 -> 64 -> 64 -> 9 (GELU), BF16 MFMA / FP32 accumulate.
+ * Inputs: du_i/dx_j (9, row-major) and dT'/dx_j (3) by centred differences;
+ * outputs: deviatoric tau_11,22,33,12,23,13 and q_1..3; the step then uses
+ * F_ij = u_i u_j + tau_ij and F_Tj = u_j T' + q_j. */
+typedef struct sam_mlp sam_mlp;
+#define SAM_MLP_NPARAM 10448     /* FP32 parameters; + 32 normalisation scales */
+int  sam_mlp_create(sam_mlp **m, unsigned seed, long long pool_capacity);
+int  sam_mlp_destroy(sam_mlp *m);
+/* params: SAM_MLP_NPARAM weights/biases followed by 32 scales */
+int  sam_mlp_get_params(sam_mlp *m, float *params);
+int  sam_mlp_set_params(sam_mlp *m, const float *params);
+/* training samples from the current state of a DNS handle: box filter by
+ * `ratio` in each direction (labels tau, q) and gradient features of the
+ * filtered field on the coarse grid */
+int  sam_mlp_add_samples(sam_mlp *m, sam_handle *dns, int ratio);
+/* host samples: feats [n][12], labels [n][9] */
+int  sam_mlp_add_samples_host(sam_mlp *m, const float *feats, const float *labels, long long n);
+long long sam_mlp_pool_size(const sam_mlp *m);
+/* copy pool samples [offset, offset+n) to host: feats [n][12], labels [n][9] */
+int  sam_mlp_get_samples(sam_mlp *m, long long offset, long long n, float *feats, float *labels);
+int  sam_mlp_clear_pool(sam_mlp *m);
+/* input/output scales = 1/rms and rms of the pool's columns */
+int  sam_mlp_fit_norms(sam_mlp *m);
+/* nsteps Adam steps (beta 0.9/0.999) on pseudo-random minibatches of the
+ * pool; batch must be a multiple of 4096; losses[nsteps] (may be NULL) */
+int  sam_mlp_train(sam_mlp *m, int nsteps, int batch, float lr, unsigned seed, float *losses);
+/* loss (mean square, normalised outputs) and its gradient w.r.t. the
+ * SAM_MLP_NPARAM parameters for host samples (n multiple of 4096) */
+int  sam_mlp_grad(sam_mlp *m, const float *feats, const float *labels, int n, float *grad, double *loss);
+/* inference: feats [n][12] -> out [n][9] (physical units, deviatoric tau) */
+int  sam_mlp_infer(sam_mlp *m, const float *feats, float *out, long long n);
+/* LES handle h uses network m when its sgs model is SAM_SGS_MLP */
+int  sam_attach_mlp(sam_handle *h, sam_mlp *m);
+/* switch the handle's SGS model between steps (e.g. fall back MLP -> DYN) */
+int  sam_set_sgs(sam_handle *h, int sgs);
+
 /* ---- measurement (no counterpart in the reference, which has no timers) */
 /* kernel classes timed with HIP events on the handle's stream */
 #define SAM_K_ZB     0   /* z inverse transform            */

@@ -1,0 +1,111 @@
+"""CPU oracle of the learned closure's network (numpy).
+
+TEST INFRASTRUCTURE ONLY (tests/, never the product path).  The closure has
+no reference counterpart; this restates the GPU network exactly:
+
+  12 -> 64 -> 64 -> 64 -> 9, GELU (tanh form), inputs padded to 16 and
+  outputs to 16; every GEMM operand (weights, activations, output and hidden
+  gradients) rounded to BF16 (round to nearest even), FP32 accumulation, FP32
+  bias, GELU and Adam.
+
+`bf16=False` gives the unrounded float64 network, used for finite-difference
+gradient checks.  Parameter layout (FP32 block of 10448 + 32 scales):
+W1[64][16] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[16][64] b4[16],
+then input scales (16) and output scales (16).
+"""
+import numpy as np
+
+NPARAM = 10448
+OFF = dict(W1=0, b1=1024, W2=1088, b2=5184, W3=5248, b3=9344, W4=9408, b4=10432)
+SHAPES = dict(W1=(64, 16), b1=(64,), W2=(64, 64), b2=(64,), W3=(64, 64), b3=(64,), W4=(16, 64), b4=(16,))
+NIN, NOUT = 12, 9
+
+
+def bf16(x):
+    """round float32 values to bfloat16 (nearest even), returned as float32"""
+    a = np.ascontiguousarray(np.asarray(x, dtype=np.float32))
+    u = a.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).reshape(a.shape)
+
+
+def unpack(params):
+    p = np.asarray(params, dtype=np.float64)
+    w = {k: p[OFF[k]:OFF[k] + int(np.prod(SHAPES[k]))].reshape(SHAPES[k]) for k in OFF}
+    fs = p[NPARAM:NPARAM + 16]
+    os_ = p[NPARAM + 16:NPARAM + 32]
+    return w, fs, os_
+
+
+def pack_grads(g):
+    out = np.zeros(NPARAM)
+    for k in OFF:
+        out[OFF[k]:OFF[k] + g[k].size] = g[k].ravel()
+    return out
+
+
+def gelu(x):
+    c, a = 0.7978845608028654, 0.044715
+    t = np.tanh(c * (x + a * x ** 3))
+    dg = 0.5 * (1 + t) + 0.5 * x * (1 - t * t) * c * (1 + 3 * a * x * x)
+    return 0.5 * x * (1 + t), dg
+
+
+class Net:
+    def __init__(self, params, bf=True):
+        self.w, self.fs, self.os = unpack(params)
+        self.bf = bf
+        self.dt = np.float32 if bf else np.float64
+
+    def r(self, x):
+        return bf16(x) if self.bf else np.asarray(x, dtype=np.float64)
+
+    def forward(self, feats):
+        """feats [n][12] raw -> dict of activations; out [n][16] normalised"""
+        dt = self.dt
+        n = feats.shape[0]
+        x = np.zeros((n, 16), dt)
+        x[:, :NIN] = feats * self.fs[:NIN]
+        act = {"a0": self.r(x)}
+        a = act["a0"]
+        for L, (W, b) in enumerate((("W1", "b1"), ("W2", "b2"), ("W3", "b3")), start=1):
+            z = (a.astype(dt) @ self.r(self.w[W].astype(dt)).T + self.w[b].astype(dt)).astype(dt)
+            g, dg = gelu(z.astype(np.float64))
+            act[f"z{L}"] = z
+            act[f"dg{L}"] = dg.astype(dt)
+            a = self.r(g.astype(dt))
+            act[f"a{L}"] = a
+        act["out"] = (a.astype(dt) @ self.r(self.w["W4"].astype(dt)).T + self.w["b4"].astype(dt)).astype(dt)
+        return act
+
+    def infer(self, feats):
+        """physical outputs [n][9], tau made deviatoric"""
+        o = self.forward(feats)["out"][:, :NOUT].astype(np.float64) * self.os[:NOUT]
+        tr = (o[:, 0] + o[:, 1] + o[:, 2]) / 3.0
+        o[:, :3] -= tr[:, None]
+        return o
+
+    def loss_grad(self, feats, labels):
+        """mean-square loss over normalised outputs and its parameter gradient"""
+        dt = self.dt
+        n = feats.shape[0]
+        act = self.forward(feats)
+        y = np.zeros((n, 16), dt)
+        y[:, :NOUT] = labels / self.os[:NOUT]
+        d = np.zeros((n, 16), dt)
+        d[:, :NOUT] = act["out"][:, :NOUT] - y[:, :NOUT]
+        loss = float(np.sum(d.astype(np.float64) ** 2) / (n * NOUT))
+        dout = self.r((2.0 * d / (n * NOUT)).astype(dt))
+        g = {"W4": dout.T @ act["a3"], "b4": dout.sum(0)}
+        da = dout @ self.r(self.w["W4"].astype(dt))
+        for L, (W, b) in ((3, ("W3", "b3")), (2, ("W2", "b2")), (1, ("W1", "b1"))):
+            dz = self.r((da * act[f"dg{L}"]).astype(dt))
+            g[W] = dz.T @ act[f"a{L - 1}"]
+            g[b] = dz.sum(0)
+            if L > 1:
+                da = dz @ self.r(self.w[W].astype(dt))
+        # padded inputs / outputs carry no gradient
+        g["W1"][:, NIN:] = 0
+        g["W4"][NOUT:, :] = 0
+        g["b4"][NOUT:] = 0
+        return loss, pack_grads(g)

@@ -53,6 +53,7 @@ struct DevBuf {
     size_t bytes = 0;
 };
 
+struct sam_mlp;
 struct sam_handle {
     sam_params p;
     Dims d;
@@ -71,6 +72,7 @@ struct sam_handle {
     void *tw[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
     int have_spectra = 0;
     int sgs_active = 0;
+    sam_mlp *mlp = nullptr;      // network used when p.sgs == SAM_SGS_MLP
     // HIP-event timing of the step's kernels (sam_prof_*)
     bool prof = false;
     std::vector<hipEvent_t> ev_free;
@@ -479,11 +481,13 @@ template <typename T> static int sgs_coefficient(sam_handle *h, const T *P, LesB
     return SAM_OK;
 }
 
+template <typename T> static int mlp_flux(sam_handle *h, const T *P, T *Q);
+
 template <typename T> static int les_fluxes(sam_handle *h, int stats)
 {
     const Dims &d = h->d;
     const int pi = h->prec;
-    if (h->p.sgs == SAM_SGS_MLP) return set_err(SAM_ERR_UNSUPPORTED, "learned closure not enabled on this handle");
+    if (h->p.sgs == SAM_SGS_MLP && !h->mlp) return set_err(SAM_ERR_STATE, "sgs = MLP but no network attached");
     LesBufs b;
     int rc = les_bufs<T>(h, b, h->p.sgs == SAM_SGS_DYN);
     if (rc) return rc;
@@ -501,10 +505,15 @@ template <typename T> static int les_fluxes(sam_handle *h, int stats)
     }
     {
         PROF(SAM_K_SGS);
-        rc = sgs_coefficient<T>(h, P, b, h->diag, stats);
-        if (rc) return rc;
-        RCHK(launch_sgs_flux<T>(d, P, (const T *)b.smag, h->diag, Q, d.nx / h->p.xl, d.ny / h->p.yl,
-                                d.nz / h->p.zl, 1.0 / h->p.pr_t, h->st));
+        if (h->p.sgs == SAM_SGS_MLP) {
+            rc = mlp_flux<T>(h, P, Q);
+            if (rc) return rc;
+        } else {
+            rc = sgs_coefficient<T>(h, P, b, h->diag, stats);
+            if (rc) return rc;
+            RCHK(launch_sgs_flux<T>(d, P, (const T *)b.smag, h->diag, Q, d.nx / h->p.xl, d.ny / h->p.yl,
+                                    d.nz / h->p.zl, 1.0 / h->p.pr_t, h->st));
+        }
     }
     {
         PROF(SAM_K_XF);
@@ -889,5 +898,340 @@ extern "C" int sam_device_copy(sam_handle *h, long long bytes, int reps)
     if (rc) return rc;
     for (int r = 0; r < reps; r++) RCHK(launch_copy16(h->stage0.p, h->stage1.p, n, h->st));
     HIPCHK(hipStreamSynchronize(h->st));
+    return SAM_OK;
+}
+
+// ---------------------------------------------------------------------
+// learned closure
+// ---------------------------------------------------------------------
+struct sam_mlp {
+    hipStream_t st = nullptr;
+    float *P = nullptr, *norms = nullptr, *m1 = nullptr, *m2 = nullptr;
+    int step = 0;
+    float *feats = nullptr, *labels = nullptr;
+    long long cap = 0, n = 0;
+    DevBuf act, gpart, lpart, npart, gout, tmp, lossv;
+};
+
+static int mlp_alloc_train(sam_mlp *m, int batch, int nch)
+{
+    const size_t per = (size_t)(16 + 3 * 64 + 3 * 64 + 16) * sizeof(short);
+    int rc = ensure(m->act, per * batch);
+    if (rc) return rc;
+    rc = ensure(m->gpart, sizeof(float) * (size_t)MLP_NPARAM * nch);
+    if (rc) return rc;
+    rc = ensure(m->lpart, sizeof(double) * 1024);
+    if (rc) return rc;
+    return ensure(m->gout, sizeof(float) * MLP_NPARAM);
+}
+
+static MlpTrainBufs mlp_bufs(sam_mlp *m, int batch)
+{
+    short *b = (short *)m->act.p;
+    MlpTrainBufs t;
+    t.a0 = b;
+    t.a1 = t.a0 + (size_t)16 * batch;
+    t.a2 = t.a1 + (size_t)64 * batch;
+    t.a3 = t.a2 + (size_t)64 * batch;
+    t.dz1 = t.a3 + (size_t)64 * batch;
+    t.dz2 = t.dz1 + (size_t)64 * batch;
+    t.dz3 = t.dz2 + (size_t)64 * batch;
+    t.dout = t.dz3 + (size_t)64 * batch;
+    return t;
+}
+
+static int wgrad_chunks(int batch)
+{
+    int nch = 256;
+    while (nch > 1 && (batch / 16) % nch) nch >>= 1;
+    return nch;
+}
+
+extern "C" int sam_mlp_create(sam_mlp **out, unsigned seed, long long cap)
+{
+    if (!out || cap < 0) return set_err(SAM_ERR_VALUE, "bad argument");
+    sam_mlp *m = new sam_mlp();
+    auto fail = [&](int code) {
+        sam_mlp_destroy(m);
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&m->st, hipStreamNonBlocking) != hipSuccess)
+        return fail(set_err(SAM_ERR_HIP, "hipStreamCreate failed"));
+    // PyTorch-style Linear init: U(-1/sqrt(fan_in), 1/sqrt(fan_in)) for weights and biases
+    std::vector<float> P(MLP_NPARAM + 32, 0.f);
+    unsigned long long x = 0x9E3779B97F4A7C15ull ^ (unsigned long long)seed;
+    auto rnd = [&]() {
+        x ^= x << 13; x ^= x >> 7; x ^= x << 17;
+        return (double)(x >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0;
+    };
+    struct L { int off, boff, out, in, ld, real_in, real_out; } layers[4] = {
+        {MLP_OFF_W1, MLP_OFF_B1, 64, 16, 16, MLP_IN, 64}, {MLP_OFF_W2, MLP_OFF_B2, 64, 64, 64, 64, 64},
+        {MLP_OFF_W3, MLP_OFF_B3, 64, 64, 64, 64, 64}, {MLP_OFF_W4, MLP_OFF_B4, 16, 64, 64, 64, MLP_OUT}};
+    for (auto &l : layers) {
+        const double bound = 1.0 / sqrt((double)l.real_in);
+        for (int o = 0; o < l.out; o++) {
+            for (int i = 0; i < l.in; i++)
+                P[l.off + o * l.ld + i] = (o < l.real_out && i < l.real_in) ? (float)(bound * rnd()) : 0.f;
+            P[l.boff + o] = o < l.real_out ? (float)(bound * rnd()) : 0.f;
+        }
+    }
+    for (int k = 0; k < 32; k++) P[MLP_NPARAM + k] = (k < MLP_IN || (k >= 16 && k < 16 + MLP_OUT)) ? 1.f : 0.f;
+    struct { float **pp; size_t n; } allocs[] = {{&m->P, MLP_NPARAM}, {&m->norms, 32}, {&m->m1, MLP_NPARAM},
+                                                 {&m->m2, MLP_NPARAM}};
+    for (auto &a : allocs)
+        if (hipMalloc(a.pp, sizeof(float) * a.n) != hipSuccess) return fail(set_err(SAM_ERR_NOMEM, "hipMalloc failed"));
+    if (cap > 0) {
+        if (hipMalloc(&m->feats, sizeof(float) * 16 * cap) != hipSuccess ||
+            hipMalloc(&m->labels, sizeof(float) * 16 * cap) != hipSuccess)
+            return fail(set_err(SAM_ERR_NOMEM, "sample pool of %lld points does not fit", cap));
+    }
+    m->cap = cap;
+    if (hipMemcpy(m->P, P.data(), sizeof(float) * MLP_NPARAM, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(m->norms, P.data() + MLP_NPARAM, sizeof(float) * 32, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(m->m1, 0, sizeof(float) * MLP_NPARAM) != hipSuccess ||
+        hipMemset(m->m2, 0, sizeof(float) * MLP_NPARAM) != hipSuccess)
+        return fail(set_err(SAM_ERR_HIP, "MLP upload failed"));
+    *out = m;
+    return SAM_OK;
+}
+
+extern "C" int sam_mlp_destroy(sam_mlp *m)
+{
+    if (!m) return SAM_OK;
+    if (m->st) hipStreamSynchronize(m->st);
+    hipFree(m->P); hipFree(m->norms); hipFree(m->m1); hipFree(m->m2);
+    hipFree(m->feats); hipFree(m->labels);
+    for (DevBuf *b : {&m->act, &m->gpart, &m->lpart, &m->npart, &m->gout, &m->tmp, &m->lossv}) hipFree(b->p);
+    if (m->st) hipStreamDestroy(m->st);
+    delete m;
+    return SAM_OK;
+}
+
+extern "C" int sam_mlp_get_params(sam_mlp *m, float *params)
+{
+    if (!m || !params) return set_err(SAM_ERR_VALUE, "null argument");
+    HIPCHK(hipStreamSynchronize(m->st));
+    HIPCHK(hipMemcpy(params, m->P, sizeof(float) * MLP_NPARAM, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(params + MLP_NPARAM, m->norms, sizeof(float) * 32, hipMemcpyDeviceToHost));
+    return SAM_OK;
+}
+
+extern "C" int sam_mlp_set_params(sam_mlp *m, const float *params)
+{
+    if (!m || !params) return set_err(SAM_ERR_VALUE, "null argument");
+    HIPCHK(hipStreamSynchronize(m->st));
+    HIPCHK(hipMemcpy(m->P, params, sizeof(float) * MLP_NPARAM, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(m->norms, params + MLP_NPARAM, sizeof(float) * 32, hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(m->m1, 0, sizeof(float) * MLP_NPARAM));
+    HIPCHK(hipMemset(m->m2, 0, sizeof(float) * MLP_NPARAM));
+    m->step = 0;
+    return SAM_OK;
+}
+
+extern "C" long long sam_mlp_pool_size(const sam_mlp *m) { return m ? m->n : 0; }
+
+extern "C" int sam_mlp_get_samples(sam_mlp *m, long long off, long long n, float *feats, float *labels)
+{
+    if (!m || !feats || !labels || off < 0 || n < 0 || off + n > m->n) return set_err(SAM_ERR_VALUE, "bad range");
+    std::vector<float> f((size_t)n * 16), l((size_t)n * 16);
+    HIPCHK(hipStreamSynchronize(m->st));
+    HIPCHK(hipMemcpy(f.data(), m->feats + 16 * off, sizeof(float) * 16 * n, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(l.data(), m->labels + 16 * off, sizeof(float) * 16 * n, hipMemcpyDeviceToHost));
+    for (long long p = 0; p < n; p++) {
+        for (int j = 0; j < MLP_IN; j++) feats[p * MLP_IN + j] = f[p * 16 + j];
+        for (int j = 0; j < MLP_OUT; j++) labels[p * MLP_OUT + j] = l[p * 16 + j];
+    }
+    return SAM_OK;
+}
+
+extern "C" int sam_mlp_clear_pool(sam_mlp *m)
+{
+    if (!m) return set_err(SAM_ERR_VALUE, "null argument");
+    m->n = 0;
+    return SAM_OK;
+}
+
+template <typename T> static int physical_velocity(sam_handle *h)
+{
+    const Dims &d = h->d;
+    int rc = ensure(h->Pb, sizeof(T) * d.pfield * (d.lu + d.lr));
+    if (rc) return rc;
+    const int pi = h->prec;
+    RCHK(launch_zb<T>(d, h->U[h->cur], d.nzm, d.lu, (T *)h->Zb, h->tw[2][pi], h->st));
+    RCHK(launch_yb<T>(d, (const T *)h->Zb, d.lu, (T *)h->Yb, h->tw[1][pi], h->st));
+    RCHK(launch_xb<T>(d, (const T *)h->Yb, d.lu, (T *)h->Pb.p, h->tw[0][pi], h->st));
+    return SAM_OK;
+}
+
+extern "C" int sam_mlp_add_samples(sam_mlp *m, sam_handle *dns, int r)
+{
+    if (!m || !dns) return set_err(SAM_ERR_VALUE, "null argument");
+    const Dims &d = dns->d;
+    if (r < 1 || d.nx % r || d.ny % r || d.nz % r || (d.nx / r) < 4 || (d.ny / r) < 4 || (d.nz / r) < 4)
+        return set_err(SAM_ERR_VALUE, "filter ratio %d does not divide the grid into >= 4 cells", r);
+    const int cx = d.nx / r, cy = d.ny / r, cz = d.nz / r;
+    const long long nc = (long long)cx * cy * cz;
+    if (m->n + nc > m->cap) return set_err(SAM_ERR_SIZE, "sample pool full (%lld + %lld > %lld)", m->n, nc, m->cap);
+    int rc = dns->prec ? physical_velocity<float>(dns) : physical_velocity<double>(dns);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(dns->st));
+    rc = ensure(m->tmp, sizeof(float) * nc * 4);
+    if (rc) return rc;
+    float *coarse = (float *)m->tmp.p;
+    if (dns->prec)
+        RCHK(launch_box_filter<float>(d, (const float *)dns->Pb.p, r, coarse, m->labels + 16 * m->n, m->st));
+    else
+        RCHK(launch_box_filter<double>(d, (const double *)dns->Pb.p, r, coarse, m->labels + 16 * m->n, m->st));
+    RCHK(launch_coarse_features(coarse, cx, cy, cz, d.lu, (float)(cx / dns->p.xl), (float)(cy / dns->p.yl),
+                                (float)(cz / dns->p.zl), m->feats + 16 * m->n, m->st));
+    HIPCHK(hipStreamSynchronize(m->st));
+    m->n += nc;
+    return SAM_OK;
+}
+
+// host [n][k] -> device [n][16] (zero padded)
+static int upload_padded(sam_mlp *m, float *dst, const float *src, long long n, int k)
+{
+    std::vector<float> tmp((size_t)n * 16, 0.f);
+    for (long long p = 0; p < n; p++)
+        for (int j = 0; j < k; j++) tmp[p * 16 + j] = src[p * k + j];
+    HIPCHK(hipMemcpyAsync(dst, tmp.data(), sizeof(float) * 16 * n, hipMemcpyHostToDevice, m->st));
+    HIPCHK(hipStreamSynchronize(m->st));
+    return SAM_OK;
+}
+
+extern "C" int sam_mlp_add_samples_host(sam_mlp *m, const float *feats, const float *labels, long long n)
+{
+    if (!m || !feats || !labels || n < 0) return set_err(SAM_ERR_VALUE, "bad argument");
+    if (m->n + n > m->cap) return set_err(SAM_ERR_SIZE, "sample pool full");
+    int rc = upload_padded(m, m->feats + 16 * m->n, feats, n, MLP_IN);
+    if (rc) return rc;
+    rc = upload_padded(m, m->labels + 16 * m->n, labels, n, MLP_OUT);
+    if (rc) return rc;
+    m->n += n;
+    return SAM_OK;
+}
+
+extern "C" int sam_mlp_fit_norms(sam_mlp *m)
+{
+    if (!m) return set_err(SAM_ERR_VALUE, "null argument");
+    if (m->n < 1) return set_err(SAM_ERR_STATE, "empty sample pool");
+    int rc = ensure(m->npart, sizeof(double) * 256 * 32);
+    if (rc) return rc;
+    RCHK(launch_pool_norms(m->feats, m->labels, m->n, (double *)m->npart.p, m->norms, m->st));
+    HIPCHK(hipStreamSynchronize(m->st));
+    return SAM_OK;
+}
+
+// one forward/backward pass over `batch` samples of (feats, labels, npool)
+static int mlp_pass(sam_mlp *m, const float *feats, const float *labels, long long npool, long long base,
+                    long long pa, long long pb, int batch, double *loss_dst, int apply, float lr)
+{
+    const int nch = wgrad_chunks(batch);
+    int rc = mlp_alloc_train(m, batch, nch);
+    if (rc) return rc;
+    MlpTrainBufs tb = mlp_bufs(m, batch);
+    int nblk = batch / 64;
+    if (nblk > 1024) nblk = 1024;
+    if (nblk < 1) nblk = 1;
+    RCHK(launch_mlp_fb(m->P, m->norms, feats, labels, npool, base, pa, pb, batch, tb, (double *)m->lpart.p, nblk,
+                       m->st));
+    RCHK(launch_sum_loss((const double *)m->lpart.p, nblk, loss_dst, m->st));
+    RCHK(launch_mlp_wgrad(tb, batch, (float *)m->gpart.p, nch, m->st));
+    if (apply) m->step++;
+    RCHK(launch_mlp_adam((const float *)m->gpart.p, nch, m->P, m->m1, m->m2, (float *)m->gout.p, lr, m->step > 0 ? m->step : 1,
+                         apply, m->st));
+    return SAM_OK;
+}
+
+extern "C" int sam_mlp_train(sam_mlp *m, int nsteps, int batch, float lr, unsigned seed, float *losses)
+{
+    if (!m || nsteps < 0) return set_err(SAM_ERR_VALUE, "bad argument");
+    if (batch < 4096 || batch % 4096) return set_err(SAM_ERR_VALUE, "batch must be a positive multiple of 4096");
+    if (m->n < 16) return set_err(SAM_ERR_STATE, "sample pool has %lld points", m->n);
+    int rc = ensure(m->lossv, sizeof(double) * (nsteps > 0 ? nsteps : 1));
+    if (rc) return rc;
+    // multiplier coprime with the pool size: a pseudo-random walk through all samples
+    static const long long primes[] = {1000003LL, 998244353LL, 1000000007LL, 2147483647LL};
+    long long pa = 1;
+    for (long long q : primes)
+        if (m->n % q) { pa = q; break; }
+    if (m->n >= (1LL << 31)) return set_err(SAM_ERR_SIZE, "sample pool above 2^31 points");
+    pa %= m->n;
+    const long long pb = ((long long)(seed % (unsigned)m->n) * 2654435761LL) % m->n;
+    for (int s = 0; s < nsteps; s++) {
+        const long long base = ((long long)s * batch + (long long)(seed % 65536u) * 7919LL) % m->n;
+        rc = mlp_pass(m, m->feats, m->labels, m->n, base, pa, pb, batch, (double *)m->lossv.p + s, 1, lr);
+        if (rc) return rc;
+    }
+    HIPCHK(hipStreamSynchronize(m->st));
+    if (losses && nsteps > 0) {
+        std::vector<double> l(nsteps);
+        HIPCHK(hipMemcpy(l.data(), m->lossv.p, sizeof(double) * nsteps, hipMemcpyDeviceToHost));
+        for (int s = 0; s < nsteps; s++) losses[s] = (float)(l[s] / ((double)batch * MLP_OUT));
+    }
+    return SAM_OK;
+}
+
+extern "C" int sam_mlp_grad(sam_mlp *m, const float *feats, const float *labels, int n, float *grad, double *loss)
+{
+    if (!m || !feats || !labels || !grad) return set_err(SAM_ERR_VALUE, "null argument");
+    if (n < 4096 || n % 4096) return set_err(SAM_ERR_VALUE, "n must be a positive multiple of 4096");
+    int rc = ensure(m->tmp, sizeof(float) * 32 * (size_t)n);
+    if (rc) return rc;
+    float *f = (float *)m->tmp.p, *l = f + 16 * (size_t)n;
+    if ((rc = upload_padded(m, f, feats, n, MLP_IN)) || (rc = upload_padded(m, l, labels, n, MLP_OUT))) return rc;
+    rc = ensure(m->lossv, sizeof(double));
+    if (rc) return rc;
+    rc = mlp_pass(m, f, l, n, 0, 1, 0, n, (double *)m->lossv.p, 0, 0.f);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(m->st));
+    HIPCHK(hipMemcpy(grad, m->gout.p, sizeof(float) * MLP_NPARAM, hipMemcpyDeviceToHost));
+    double ls = 0;
+    HIPCHK(hipMemcpy(&ls, m->lossv.p, sizeof(double), hipMemcpyDeviceToHost));
+    if (loss) *loss = ls / ((double)n * MLP_OUT);
+    return SAM_OK;
+}
+
+extern "C" int sam_mlp_infer(sam_mlp *m, const float *feats, float *out, long long n)
+{
+    if (!m || !feats || !out || n < 0) return set_err(SAM_ERR_VALUE, "bad argument");
+    int rc = ensure(m->tmp, sizeof(float) * 32 * (size_t)(n + 16));
+    if (rc) return rc;
+    float *f = (float *)m->tmp.p, *o = f + 16 * (size_t)(n + 16);
+    if ((rc = upload_padded(m, f, feats, n, MLP_IN))) return rc;
+    RCHK(launch_mlp_infer(m->P, m->norms, f, o, n, m->st));
+    std::vector<float> tmp((size_t)n * 16);
+    HIPCHK(hipMemcpyAsync(tmp.data(), o, sizeof(float) * 16 * n, hipMemcpyDeviceToHost, m->st));
+    HIPCHK(hipStreamSynchronize(m->st));
+    for (long long p = 0; p < n; p++)
+        for (int j = 0; j < MLP_OUT; j++) out[p * MLP_OUT + j] = tmp[p * 16 + j];
+    return SAM_OK;
+}
+
+extern "C" int sam_attach_mlp(sam_handle *h, sam_mlp *m)
+{
+    if (!h) return set_err(SAM_ERR_VALUE, "null handle");
+    h->mlp = m;
+    return SAM_OK;
+}
+
+extern "C" int sam_set_sgs(sam_handle *h, int sgs)
+{
+    if (!h) return set_err(SAM_ERR_VALUE, "null handle");
+    if (sgs < SAM_SGS_NONE || sgs > SAM_SGS_MLP) return set_err(SAM_ERR_VALUE, "unknown sgs model %d", sgs);
+    if (sgs == SAM_SGS_MLP && !h->mlp) return set_err(SAM_ERR_STATE, "attach a network first");
+    if (sgs != SAM_SGS_NONE && !(h->p.pr_t > 0)) return set_err(SAM_ERR_VALUE, "pr_t must be > 0");
+    h->p.sgs = sgs;
+    h->sgs_active = sgs;
+    return SAM_OK;
+}
+
+template <typename T> static int mlp_flux(sam_handle *h, const T *P, T *Q)
+{
+    const Dims &d = h->d;
+    HIPCHK(hipStreamSynchronize(h->mlp->st));   // parameters written on the network's stream
+    RCHK(launch_les_mlp_flux<T>(d, h->mlp->P, h->mlp->norms, P, Q, (float)(d.nx / h->p.xl), (float)(d.ny / h->p.yl),
+                                (float)(d.nz / h->p.zl), h->st));
     return SAM_OK;
 }

@@ -93,6 +93,38 @@ template <typename T> int launch_sgs_flux(const Dims &d, const T *P, const T *sm
                                           double dxi, double dyi, double dzi, double pr_t_inv, hipStream_t st);
 template <typename T> int launch_nut(const Dims &d, const T *smag, const DevDiag *diag, double *vt, hipStream_t st);
 
+// learned closure (mlp.hip).  FP32 parameter block: W1[64][16] b1[64]
+// W2[64][64] b2[64] W3[64][64] b3[64] W4[16][64] b4[16]; inputs padded 12->16,
+// outputs 9->16.  norms[32]: input scales (16) then output scales (16).
+constexpr int MLP_IN = 12, MLP_HID = 64, MLP_OUT = 9;
+constexpr int MLP_OFF_W1 = 0;
+constexpr int MLP_OFF_B1 = MLP_OFF_W1 + 64 * 16;
+constexpr int MLP_OFF_W2 = MLP_OFF_B1 + 64;
+constexpr int MLP_OFF_B2 = MLP_OFF_W2 + 64 * 64;
+constexpr int MLP_OFF_W3 = MLP_OFF_B2 + 64;
+constexpr int MLP_OFF_B3 = MLP_OFF_W3 + 64 * 64;
+constexpr int MLP_OFF_W4 = MLP_OFF_B3 + 64;
+constexpr int MLP_OFF_B4 = MLP_OFF_W4 + 16 * 64;
+constexpr int MLP_NPARAM = MLP_OFF_B4 + 16;
+struct MlpTrainBufs {
+    short *a0, *a1, *a2, *a3, *dz1, *dz2, *dz3, *dout;
+};
+int launch_mlp_fb(const float *P, const float *norms, const float *feats, const float *labels, long long npool,
+                  long long base, long long pa, long long pb, int batch, const MlpTrainBufs &b, double *loss_part,
+                  int nblk, hipStream_t st);
+int launch_mlp_wgrad(const MlpTrainBufs &b, int batch, float *part, int nchunks, hipStream_t st);
+int launch_mlp_adam(const float *part, int nch, float *P, float *m1, float *m2, float *gout, float lr, int step,
+                    int apply, hipStream_t st);
+int launch_sum_loss(const double *part, int n, double *out, hipStream_t st);
+int launch_mlp_infer(const float *P, const float *norms, const float *feats, float *out, long long n, hipStream_t st);
+template <typename T>
+int launch_les_mlp_flux(const Dims &d, const float *Pm, const float *norms, const T *U, T *Q, float dxi, float dyi,
+                        float dzi, hipStream_t st);
+template <typename T> int launch_box_filter(const Dims &d, const T *U, int r, float *coarse, float *labels, hipStream_t st);
+int launch_coarse_features(const float *C, int nx, int ny, int nz, int lu, float dxi, float dyi, float dzi,
+                           float *feats, hipStream_t st);
+int launch_pool_norms(const float *feats, const float *labels, long long n, double *part, float *norms, hipStream_t st);
+
 // plain 16-byte-per-thread copy (counter calibration)
 int launch_copy16(const void *src, void *dst, long long n16, hipStream_t st);
 

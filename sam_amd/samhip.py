@@ -37,7 +37,14 @@ EXPORTS = (
     "sam_products", "sam_vel_max", "sam_project",
     "sam_eddy_visc", "sam_test_filter",
     "sam_prof_enable", "sam_prof_read", "sam_sync", "sam_device_copy",
+    "sam_mlp_create", "sam_mlp_destroy", "sam_mlp_get_params", "sam_mlp_set_params",
+    "sam_mlp_add_samples", "sam_mlp_add_samples_host", "sam_mlp_pool_size", "sam_mlp_get_samples",
+    "sam_mlp_clear_pool",
+    "sam_mlp_fit_norms", "sam_mlp_train", "sam_mlp_grad", "sam_mlp_infer",
+    "sam_attach_mlp", "sam_set_sgs",
 )
+
+MLP_NPARAM = 10448
 
 # kernel classes of sam_prof_read (SAM_K_* in include/samhip.h)
 KERNEL_CLASSES = ("zb", "yb", "xrow", "final", "yf", "zrhs", "spec", "xb", "xf", "sgs", "misc")
@@ -124,11 +131,27 @@ def load():
     L.sam_prof_read.argtypes = [_hp, _dp, ctypes.POINTER(ctypes.c_longlong)]
     L.sam_sync.argtypes = [_hp]
     L.sam_device_copy.argtypes = [_hp, ctypes.c_longlong, ctypes.c_int]
+    _fp = ctypes.POINTER(ctypes.c_float)
+    L.sam_mlp_create.argtypes = [ctypes.POINTER(_hp), ctypes.c_uint, ctypes.c_longlong]
+    L.sam_mlp_destroy.argtypes = [_hp]
+    L.sam_mlp_get_params.argtypes = [_hp, _fp]
+    L.sam_mlp_set_params.argtypes = [_hp, _fp]
+    L.sam_mlp_add_samples.argtypes = [_hp, _hp, ctypes.c_int]
+    L.sam_mlp_add_samples_host.argtypes = [_hp, _fp, _fp, ctypes.c_longlong]
+    L.sam_mlp_pool_size.argtypes = [_hp]
+    L.sam_mlp_pool_size.restype = ctypes.c_longlong
+    L.sam_mlp_clear_pool.argtypes = [_hp]
+    L.sam_mlp_get_samples.argtypes = [_hp, ctypes.c_longlong, ctypes.c_longlong, _fp, _fp]
+    L.sam_mlp_fit_norms.argtypes = [_hp]
+    L.sam_mlp_train.argtypes = [_hp, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_uint, _fp]
+    L.sam_mlp_grad.argtypes = [_hp, _fp, _fp, ctypes.c_int, _fp, ctypes.POINTER(ctypes.c_double)]
+    L.sam_mlp_infer.argtypes = [_hp, _fp, _fp, ctypes.c_longlong]
+    L.sam_attach_mlp.argtypes = [_hp, _hp]
+    L.sam_set_sgs.argtypes = [_hp, ctypes.c_int]
     for name in EXPORTS:
-        fn = getattr(L, name)
-        if fn.restype is None or fn.restype is ctypes.c_int:
-            if name != "sam_last_error":
-                fn.restype = ctypes.c_int
+        if name in ("sam_last_error", "sam_mlp_pool_size"):
+            continue
+        getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L
 
@@ -297,3 +320,94 @@ class Sam:
         out = np.zeros_like(f, order="F")
         _chk(self.L.sam_test_filter(self.h, _ptr(f), _ptr(out), nvar))
         return out
+
+
+def _f32(a):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+class Mlp:
+    """The learned SGS closure (a libsamhip sam_mlp handle)."""
+
+    def __init__(self, seed=0, pool_capacity=0):
+        self.L = load()
+        h = _hp()
+        _chk(self.L.sam_mlp_create(ctypes.byref(h), seed, int(pool_capacity)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.sam_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_params(self):
+        p = np.zeros(MLP_NPARAM + 32, np.float32)
+        _chk(self.L.sam_mlp_get_params(self.h, p.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return p
+
+    def set_params(self, p):
+        p, ptr = _f32(p)
+        assert p.size == MLP_NPARAM + 32
+        _chk(self.L.sam_mlp_set_params(self.h, ptr))
+
+    def add_samples(self, sam, ratio):
+        _chk(self.L.sam_mlp_add_samples(self.h, sam.h, int(ratio)))
+
+    def add_samples_host(self, feats, labels):
+        f, fp = _f32(feats)
+        lab, lp = _f32(labels)
+        assert f.shape[1] == 12 and lab.shape[1] == 9 and f.shape[0] == lab.shape[0]
+        _chk(self.L.sam_mlp_add_samples_host(self.h, fp, lp, f.shape[0]))
+
+    def pool_size(self):
+        return int(self.L.sam_mlp_pool_size(self.h))
+
+    def get_samples(self, offset, n):
+        f = np.zeros((n, 12), np.float32)
+        lab = np.zeros((n, 9), np.float32)
+        _chk(self.L.sam_mlp_get_samples(self.h, offset, n, f.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                        lab.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return f, lab
+
+    def clear_pool(self):
+        _chk(self.L.sam_mlp_clear_pool(self.h))
+
+    def fit_norms(self):
+        _chk(self.L.sam_mlp_fit_norms(self.h))
+
+    def train(self, nsteps, batch=1 << 20, lr=1e-3, seed=0):
+        losses = np.zeros(max(nsteps, 1), np.float32)
+        _chk(self.L.sam_mlp_train(self.h, nsteps, batch, lr, seed,
+                                  losses.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return losses[:nsteps]
+
+    def grad(self, feats, labels):
+        f, fp = _f32(feats)
+        lab, lp = _f32(labels)
+        g = np.zeros(MLP_NPARAM, np.float32)
+        loss = ctypes.c_double()
+        _chk(self.L.sam_mlp_grad(self.h, fp, lp, f.shape[0], g.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                 ctypes.byref(loss)))
+        return loss.value, g
+
+    def infer(self, feats):
+        f, fp = _f32(feats)
+        out = np.zeros((f.shape[0], 9), np.float32)
+        _chk(self.L.sam_mlp_infer(self.h, fp, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), f.shape[0]))
+        return out
+
+
+def attach_mlp(sam, mlp):
+    _chk(load().sam_attach_mlp(sam.h, mlp.h))
+    sam._mlp = mlp          # keep the network alive as long as the grid uses it
+
+
+def set_sgs(sam, model):
+    _chk(load().sam_set_sgs(sam.h, int(model)))

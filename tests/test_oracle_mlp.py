@@ -1,0 +1,58 @@
+"""Pin the learned closure's CPU oracle (oracle/mlp_oracle.py) -- CPU only.
+The closure has no reference counterpart; its oracle is pinned by finite
+differences of the unrounded network, the BF16 rounding rule and GELU's
+derivative."""
+import numpy as np
+
+from oracle import mlp_oracle as M
+
+
+def random_params(seed, scale=0.3):
+    rng = np.random.default_rng(seed)
+    p = np.zeros(M.NPARAM + 32)
+    p[:M.NPARAM] = rng.standard_normal(M.NPARAM) * scale
+    p[M.OFF["W1"]:M.OFF["b1"]].reshape(64, 16)[:, 12:] = 0
+    w4 = p[M.OFF["W4"]:M.OFF["b4"]].reshape(16, 64)
+    w4[9:] = 0
+    p[M.OFF["b4"] + 9:M.NPARAM] = 0
+    p[M.NPARAM:M.NPARAM + 12] = rng.uniform(0.5, 2.0, 12)
+    p[M.NPARAM + 16:M.NPARAM + 25] = rng.uniform(0.5, 2.0, 9)
+    return p
+
+
+def test_bf16_rounding_rule():
+    x = np.array([1.0, 1.00390625, 1.0 + 2 ** -8 + 2 ** -9, -3.14159265, 65504.0], np.float32)
+    b = M.bf16(x)
+    assert b[0] == 1.0
+    assert b[1] == 1.0            # exactly halfway: ties to even (1.0)
+    assert b[2] == np.float32(1.0078125)
+    assert abs(b[3] + 3.140625) < 1e-7
+    assert b[4] == 65536.0
+
+
+def test_gelu_derivative():
+    x = np.linspace(-4, 4, 101)
+    g, dg = M.gelu(x)
+    h = 1e-6
+    fd = (M.gelu(x + h)[0] - M.gelu(x - h)[0]) / (2 * h)
+    assert np.abs(dg - fd).max() < 1e-8
+    assert abs(M.gelu(np.array([0.0]))[0][0]) == 0.0
+
+
+def test_gradient_finite_differences():
+    p = random_params(1)
+    rng = np.random.default_rng(2)
+    f = rng.standard_normal((64, 12))
+    y = rng.standard_normal((64, 9))
+    net = M.Net(p, bf=False)
+    loss, g = net.loss_grad(f, y)
+    idx = rng.choice(np.flatnonzero(g), 40, replace=False)
+    for j in idx:
+        for s in (+1, -1):
+            pass
+        h = 1e-6
+        pp, pm = p.copy(), p.copy()
+        pp[j] += h
+        pm[j] -= h
+        fd = (M.Net(pp, bf=False).loss_grad(f, y)[0] - M.Net(pm, bf=False).loss_grad(f, y)[0]) / (2 * h)
+        assert abs(fd - g[j]) < 1e-6 * max(1.0, abs(g[j])) + 1e-9, (j, fd, g[j])
