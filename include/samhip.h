@@ -182,6 +182,13 @@ int  sam_mlp_infer(sam_mlp *m, const float *feats, float *out, long long n);
 int  sam_attach_mlp(sam_handle *h, sam_mlp *m);
 /* switch the handle's SGS model between steps (e.g. fall back MLP -> DYN) */
 int  sam_set_sgs(sam_handle *h, int sgs);
+/* a-posteriori check of the learned closure on the current state, with the
+ * dynamic model as judge: the shell spectra of the SGS energy transfer
+ * T_m(k) = tk_m(k) - tk_none(k) (spectra.f's tk of the RHS with model m
+ * minus without) are compared; err = ||T_mlp - T_dyn|| / ||T_dyn|| over
+ * shells.  When err > tol the handle falls back to SAM_SGS_DYN
+ * (*switched = 1).  Costs three RHS evaluations; the state is unchanged. */
+int  sam_sgs_check(sam_handle *h, double tol, double *err, int *switched);
 
 /* ---- measurement (no counterpart in the reference, which has no timers) */
 /* kernel classes timed with HIP events on the handle's stream */

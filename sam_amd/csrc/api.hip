@@ -1235,3 +1235,39 @@ template <typename T> static int mlp_flux(sam_handle *h, const T *P, T *Q)
                                 (float)(d.nz / h->p.zl), h->st));
     return SAM_OK;
 }
+
+extern "C" int sam_sgs_check(sam_handle *h, double tol, double *err, int *switched)
+{
+    if (!h) return set_err(SAM_ERR_VALUE, "null handle");
+    if (switched) *switched = 0;
+    if (err) *err = 0.0;
+    if (h->p.sgs != SAM_SGS_MLP) return SAM_OK;
+    const int lm = h->d.lm;
+    std::vector<double> spec(5 * lm + 3), tk[3];
+    const int models[3] = {SAM_SGS_NONE, SAM_SGS_DYN, SAM_SGS_MLP};
+    for (int i = 0; i < 3; i++) {
+        h->p.sgs = models[i];
+        int rc = sam_rhs(h, nullptr, h->d.nzm, 1, nullptr);
+        if (!rc) rc = sam_get_spectra(h, spec.data());
+        if (rc) {
+            h->p.sgs = SAM_SGS_MLP;
+            return rc;
+        }
+        tk[i].assign(spec.begin() + lm, spec.begin() + 2 * lm);
+    }
+    h->p.sgs = SAM_SGS_MLP;
+    double num = 0.0, den = 0.0;
+    for (int k = 0; k < lm; k++) {
+        const double td = tk[1][k] - tk[0][k], tm = tk[2][k] - tk[0][k];
+        num += (tm - td) * (tm - td);
+        den += td * td;
+    }
+    const double e = den > 0 ? sqrt(num / den) : 0.0;
+    if (err) *err = e;
+    if (e > tol) {
+        h->p.sgs = SAM_SGS_DYN;
+        h->sgs_active = SAM_SGS_DYN;
+        if (switched) *switched = 1;
+    }
+    return SAM_OK;
+}

@@ -41,7 +41,7 @@ EXPORTS = (
     "sam_mlp_add_samples", "sam_mlp_add_samples_host", "sam_mlp_pool_size", "sam_mlp_get_samples",
     "sam_mlp_clear_pool",
     "sam_mlp_fit_norms", "sam_mlp_train", "sam_mlp_grad", "sam_mlp_infer",
-    "sam_attach_mlp", "sam_set_sgs",
+    "sam_attach_mlp", "sam_set_sgs", "sam_sgs_check",
 )
 
 MLP_NPARAM = 10448
@@ -148,6 +148,7 @@ def load():
     L.sam_mlp_infer.argtypes = [_hp, _fp, _fp, ctypes.c_longlong]
     L.sam_attach_mlp.argtypes = [_hp, _hp]
     L.sam_set_sgs.argtypes = [_hp, ctypes.c_int]
+    L.sam_sgs_check.argtypes = [_hp, ctypes.c_double, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]
     for name in EXPORTS:
         if name in ("sam_last_error", "sam_mlp_pool_size"):
             continue
@@ -407,6 +408,15 @@ class Mlp:
 def attach_mlp(sam, mlp):
     _chk(load().sam_attach_mlp(sam.h, mlp.h))
     sam._mlp = mlp          # keep the network alive as long as the grid uses it
+
+
+def sgs_check(sam, tol):
+    """a-posteriori check of the learned closure (dynamic model as judge);
+    returns (err, switched_to_dynamic)"""
+    e = ctypes.c_double()
+    sw = ctypes.c_int()
+    _chk(load().sam_sgs_check(sam.h, tol, ctypes.byref(e), ctypes.byref(sw)))
+    return e.value, bool(sw.value)
 
 
 def set_sgs(sam, model):

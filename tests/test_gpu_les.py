@@ -1,0 +1,80 @@
+"""GPU: the learned closure end to end -- DNS -> box-filtered samples ->
+training on the GPU -> LES in the loop -- and what it gains over the LES
+without a model; the dynamic model as judge with fallback."""
+import numpy as np
+import pytest
+
+from sam_amd import ic
+from sam_amd import les as L
+from sam_amd import samhip as S
+
+pytestmark = pytest.mark.gpu
+
+KW = dict(i_strat=1, grav=64.0, lapse=1.0, to=1.0, vis=4e-3, pr=1.0, i_cfl=0, dt0=2e-3,
+          precision=S.PREC_SPLIT)
+
+
+def shells_err(e, ref, k0, k1):
+    return float(np.linalg.norm(e[k0:k1] - ref[k0:k1]) / np.linalg.norm(ref[k0:k1]))
+
+
+@pytest.fixture(scope="module")
+def trained():
+    n_dns, r = 128, 4
+    dns = S.Sam(S.make_params(n_dns, **KW))
+    dns.set_state(ic.random_solenoidal(n_dns, k0=6, seed=20261203, i_strat=1))
+    dns.step(100)
+    mlp = S.Mlp(seed=1, pool_capacity=6 * (n_dns // r) ** 3)
+    L.collect_samples(dns, mlp, r, nsnap=6, steps_between=20)
+    losses = L.train(mlp, 400, batch=16384, lr=2e-3, seed=3)
+    return dns, mlp, losses
+
+
+def test_training_on_filtered_dns(trained):
+    dns, mlp, losses = trained
+    assert np.all(np.isfinite(losses))
+    assert losses[-20:].mean() < 0.7 * losses[:5].mean(), (losses[:5], losses[-5:])
+    # a-priori: the network's tau_12 correlates with the filtered DNS stress
+    f, y = mlp.get_samples(0, 8192)
+    pred = mlp.infer(f)
+    corr = [np.corrcoef(pred[:, k], y[:, k])[0, 1] for k in (3, 4, 5)]
+    assert min(corr) > 0.5, corr
+
+
+def test_learned_closure_gains_over_no_model(trained):
+    dns, mlp, _ = trained
+    n_les, nsteps = 32, 300
+    u_start = dns.get_state()
+    ref = dns.step(nsteps, want_spectra=True)
+    e_dns = L.shell_energy(dns.spectra())
+    errs = {}
+    for model in (S.SGS_NONE, S.SGS_MLP, S.SGS_DYN):
+        les = S.Sam(S.make_params(n_les, sgs=model, **KW))
+        if model == S.SGS_MLP:
+            S.attach_mlp(les, mlp)
+        les.set_state(L.spectral_truncate(u_start, n_les))
+        les.step(nsteps, want_spectra=True)
+        e = L.shell_energy(les.spectra())
+        assert np.all(np.isfinite(e))
+        errs[model] = shells_err(e, e_dns[: len(e)], 2, les.kxm)
+    print("E(k) relative error vs DNS: none %.3f  mlp %.3f  dynamic %.3f" %
+          (errs[S.SGS_NONE], errs[S.SGS_MLP], errs[S.SGS_DYN]))
+    assert errs[S.SGS_MLP] < errs[S.SGS_NONE]
+
+
+def test_dynamic_model_judges_and_falls_back(trained):
+    _, good, _ = trained
+    les = S.Sam(S.make_params(32, sgs=S.SGS_MLP, **KW))
+    les.set_state(ic.random_solenoidal(32, k0=6, seed=5, i_strat=1))
+    S.attach_mlp(les, good)
+    err_good, sw = S.sgs_check(les, 1e9)
+    assert not sw and np.isfinite(err_good)
+    bad = S.Mlp(seed=9)
+    p = bad.get_params()
+    p[:S.MLP_NPARAM] *= 20.0                     # a badly scaled, untrained network
+    bad.set_params(p)
+    S.attach_mlp(les, bad)
+    err_bad, sw = S.sgs_check(les, 5.0 * max(err_good, 1e-3))
+    assert sw and err_bad > err_good
+    d = les.step(2)
+    assert d["sgs_active"] == S.SGS_DYN
