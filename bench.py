@@ -76,8 +76,9 @@ def run_gpu(n, precision, steps, warmup, u0, S):
     prof = sam.prof_read()
     sam.prof_enable(False)
     spec = sam.spectra()
+    state = sam.get_state()
     sam.close()
-    return dict(wall=t1 - t0, diag=diag, prof=prof, spec=spec)
+    return dict(wall=t1 - t0, diag=diag, prof=prof, spec=spec, state=state)
 
 
 def dominant_roofline(n, bc, prof, steps):
@@ -115,6 +116,55 @@ def measured_traffic(n, kernel, prec):
         return None
 
 
+def les_comparison(S, ic, dns_u, n_dns, steps, warmup):
+    """config-4/5 learned closure: train on the box-filtered DNS state
+    (ratio n_dns/128), then time the 128^3 LES step with the learned closure
+    against the dynamic model (same IC, FP32 transforms, FP64 state)."""
+    from sam_amd import les as L
+    n_les = 128
+    ratio = n_dns // n_les
+    dns = S.Sam(make_params(S, n_dns, S.PREC_SPLIT))
+    dns.set_state(dns_u)
+    mlp = S.Mlp(seed=1, pool_capacity=2 * n_les ** 3)
+    mlp.add_samples(dns, ratio)
+    dns.step(10)
+    mlp.add_samples(dns, ratio)
+    dns.close()
+    t0 = time.perf_counter()
+    losses = L.train(mlp, 200, batch=1 << 20, lr=1e-3, seed=7)
+    t_train = time.perf_counter() - t0
+    kw = dict(i_strat=1, grav=64.0, to=1.0, lapse=1.0, vis=1e-5, pr=1.0, i_cfl=1, cfl0=0.5, dt0=0.01,
+              precision=S.PREC_SPLIT)
+    u_les = ic.random_solenoidal(n_les, k0=6.0, seed=20261203, i_strat=1)
+    res = {}
+    for name, model in (("dynamic", S.SGS_DYN), ("learned", S.SGS_MLP)):
+        les = S.Sam(S.make_params(n_les, sgs=model, **kw))
+        if model == S.SGS_MLP:
+            S.attach_mlp(les, mlp)
+        les.set_state(u_les)
+        les.step(warmup)
+        les.sync()
+        t0 = time.perf_counter()
+        les.step(steps, want_spectra=True)
+        les.sync()
+        dt = time.perf_counter() - t0
+        res[name] = dict(ms_per_step=round(dt / steps * 1e3, 3), ek=L.shell_energy(les.spectra()))
+        les.close()
+    ek_d, ek_l = res["dynamic"]["ek"], res["learned"]["ek"]
+    k1 = len(ek_d) - 1
+    return {
+        "workload": "stratified LES 128^3, N=8, nu=1e-5, seeded IC k0=6, FP32 transforms / FP64 state",
+        "dynamic_ms_per_step": res["dynamic"]["ms_per_step"],
+        "learned_ms_per_step": res["learned"]["ms_per_step"],
+        "speedup_learned_vs_dynamic": round(res["dynamic"]["ms_per_step"] / res["learned"]["ms_per_step"], 3),
+        "ek_rel_err_learned_vs_dynamic": float(np.linalg.norm(ek_l[1:k1] - ek_d[1:k1]) / np.linalg.norm(ek_d[1:k1])),
+        "steps": steps + warmup,
+        "training": {"samples": mlp.pool_size(), "source": f"{n_dns}^3 DNS box-filtered by {ratio}",
+                     "adam_steps": 200, "batch": 1 << 20, "seconds": round(t_train, 2),
+                     "loss_first": float(losses[0]), "loss_last": float(losses[-1])},
+    }
+
+
 def cpu_baseline(n_cpu, steps_cpu, u0_small):
     from oracle import oracle as O
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
@@ -137,6 +187,7 @@ def main():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-split", action="store_true")
+    ap.add_argument("--no-les", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=0, help="CPU baseline grid (default: same n if RAM allows)")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -216,6 +267,12 @@ def main():
             "after_steps": args.warmup + args.steps,
         }
         log(f"[bench] split {ms_sp:.2f} ms/step, KE err {out['fp32_split']['ke_rel_err_vs_fp64']:.2e}")
+
+    if not args.no_les and n % 128 == 0 and n >= 256:
+        log("[bench] LES 128^3: learned closure vs dynamic model")
+        out["les_128"] = les_comparison(S, ic, r64["state"], n, 20, 3)
+        log(f"[bench] LES dyn {out['les_128']['dynamic_ms_per_step']} ms, "
+            f"learned {out['les_128']['learned_ms_per_step']} ms")
 
     if not args.no_cpu:
         n_cpu = args.cpu_n or n

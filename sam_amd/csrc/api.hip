@@ -907,6 +907,7 @@ extern "C" int sam_device_copy(sam_handle *h, long long bytes, int reps)
 struct sam_mlp {
     hipStream_t st = nullptr;
     float *P = nullptr, *norms = nullptr, *m1 = nullptr, *m2 = nullptr;
+    void *img = nullptr;        // packed BF16 weight tiles, rebuilt after every parameter change
     int step = 0;
     float *feats = nullptr, *labels = nullptr;
     long long cap = 0, n = 0;
@@ -980,6 +981,7 @@ extern "C" int sam_mlp_create(sam_mlp **out, unsigned seed, long long cap)
                                                  {&m->m2, MLP_NPARAM}};
     for (auto &a : allocs)
         if (hipMalloc(a.pp, sizeof(float) * a.n) != hipSuccess) return fail(set_err(SAM_ERR_NOMEM, "hipMalloc failed"));
+    if (hipMalloc(&m->img, MLP_TILE_BYTES) != hipSuccess) return fail(set_err(SAM_ERR_NOMEM, "hipMalloc failed"));
     if (cap > 0) {
         if (hipMalloc(&m->feats, sizeof(float) * 16 * cap) != hipSuccess ||
             hipMalloc(&m->labels, sizeof(float) * 16 * cap) != hipSuccess)
@@ -991,6 +993,8 @@ extern "C" int sam_mlp_create(sam_mlp **out, unsigned seed, long long cap)
         hipMemset(m->m1, 0, sizeof(float) * MLP_NPARAM) != hipSuccess ||
         hipMemset(m->m2, 0, sizeof(float) * MLP_NPARAM) != hipSuccess)
         return fail(set_err(SAM_ERR_HIP, "MLP upload failed"));
+    if (launch_mlp_pack(m->P, m->img, m->st) != SAM_OK || hipStreamSynchronize(m->st) != hipSuccess)
+        return fail(set_err(SAM_ERR_HIP, "weight packing failed"));
     *out = m;
     return SAM_OK;
 }
@@ -999,7 +1003,7 @@ extern "C" int sam_mlp_destroy(sam_mlp *m)
 {
     if (!m) return SAM_OK;
     if (m->st) hipStreamSynchronize(m->st);
-    hipFree(m->P); hipFree(m->norms); hipFree(m->m1); hipFree(m->m2);
+    hipFree(m->P); hipFree(m->norms); hipFree(m->m1); hipFree(m->m2); hipFree(m->img);
     hipFree(m->feats); hipFree(m->labels);
     for (DevBuf *b : {&m->act, &m->gpart, &m->lpart, &m->npart, &m->gout, &m->tmp, &m->lossv}) hipFree(b->p);
     if (m->st) hipStreamDestroy(m->st);
@@ -1025,6 +1029,8 @@ extern "C" int sam_mlp_set_params(sam_mlp *m, const float *params)
     HIPCHK(hipMemset(m->m1, 0, sizeof(float) * MLP_NPARAM));
     HIPCHK(hipMemset(m->m2, 0, sizeof(float) * MLP_NPARAM));
     m->step = 0;
+    RCHK(launch_mlp_pack(m->P, m->img, m->st));
+    HIPCHK(hipStreamSynchronize(m->st));
     return SAM_OK;
 }
 
@@ -1134,13 +1140,14 @@ static int mlp_pass(sam_mlp *m, const float *feats, const float *labels, long lo
     int nblk = batch / 64;
     if (nblk > 1024) nblk = 1024;
     if (nblk < 1) nblk = 1;
-    RCHK(launch_mlp_fb(m->P, m->norms, feats, labels, npool, base, pa, pb, batch, tb, (double *)m->lpart.p, nblk,
-                       m->st));
+    RCHK(launch_mlp_fb(m->P, m->img, m->norms, feats, labels, npool, base, pa, pb, batch, tb, (double *)m->lpart.p,
+                       nblk, m->st));
     RCHK(launch_sum_loss((const double *)m->lpart.p, nblk, loss_dst, m->st));
     RCHK(launch_mlp_wgrad(tb, batch, (float *)m->gpart.p, nch, m->st));
     if (apply) m->step++;
     RCHK(launch_mlp_adam((const float *)m->gpart.p, nch, m->P, m->m1, m->m2, (float *)m->gout.p, lr, m->step > 0 ? m->step : 1,
                          apply, m->st));
+    if (apply) RCHK(launch_mlp_pack(m->P, m->img, m->st));
     return SAM_OK;
 }
 
@@ -1200,7 +1207,7 @@ extern "C" int sam_mlp_infer(sam_mlp *m, const float *feats, float *out, long lo
     if (rc) return rc;
     float *f = (float *)m->tmp.p, *o = f + 16 * (size_t)(n + 16);
     if ((rc = upload_padded(m, f, feats, n, MLP_IN))) return rc;
-    RCHK(launch_mlp_infer(m->P, m->norms, f, o, n, m->st));
+    RCHK(launch_mlp_infer(m->P, m->img, m->norms, f, o, n, m->st));
     std::vector<float> tmp((size_t)n * 16);
     HIPCHK(hipMemcpyAsync(tmp.data(), o, sizeof(float) * 16 * n, hipMemcpyDeviceToHost, m->st));
     HIPCHK(hipStreamSynchronize(m->st));
@@ -1231,8 +1238,8 @@ template <typename T> static int mlp_flux(sam_handle *h, const T *P, T *Q)
 {
     const Dims &d = h->d;
     HIPCHK(hipStreamSynchronize(h->mlp->st));   // parameters written on the network's stream
-    RCHK(launch_les_mlp_flux<T>(d, h->mlp->P, h->mlp->norms, P, Q, (float)(d.nx / h->p.xl), (float)(d.ny / h->p.yl),
-                                (float)(d.nz / h->p.zl), h->st));
+    RCHK(launch_les_mlp_flux<T>(d, h->mlp->P, h->mlp->img, h->mlp->norms, P, Q, (float)(d.nx / h->p.xl),
+                                (float)(d.ny / h->p.yl), (float)(d.nz / h->p.zl), h->st));
     return SAM_OK;
 }
 

@@ -109,17 +109,20 @@ constexpr int MLP_NPARAM = MLP_OFF_B4 + 16;
 struct MlpTrainBufs {
     short *a0, *a1, *a2, *a3, *dz1, *dz2, *dz3, *dout;
 };
-int launch_mlp_fb(const float *P, const float *norms, const float *feats, const float *labels, long long npool,
-                  long long base, long long pa, long long pb, int batch, const MlpTrainBufs &b, double *loss_part,
-                  int nblk, hipStream_t st);
+constexpr int MLP_TILE_BYTES = 76 * 64 * 8;   // packed BF16 weight-tile image
+int launch_mlp_pack(const float *P, void *img, hipStream_t st);
+int launch_mlp_fb(const float *P, const void *img, const float *norms, const float *feats, const float *labels,
+                  long long npool, long long base, long long pa, long long pb, int batch, const MlpTrainBufs &b,
+                  double *loss_part, int nblk, hipStream_t st);
 int launch_mlp_wgrad(const MlpTrainBufs &b, int batch, float *part, int nchunks, hipStream_t st);
 int launch_mlp_adam(const float *part, int nch, float *P, float *m1, float *m2, float *gout, float lr, int step,
                     int apply, hipStream_t st);
 int launch_sum_loss(const double *part, int n, double *out, hipStream_t st);
-int launch_mlp_infer(const float *P, const float *norms, const float *feats, float *out, long long n, hipStream_t st);
+int launch_mlp_infer(const float *P, const void *img, const float *norms, const float *feats, float *out, long long n,
+                     hipStream_t st);
 template <typename T>
-int launch_les_mlp_flux(const Dims &d, const float *Pm, const float *norms, const T *U, T *Q, float dxi, float dyi,
-                        float dzi, hipStream_t st);
+int launch_les_mlp_flux(const Dims &d, const float *Pm, const void *img, const float *norms, const T *U, T *Q,
+                        float dxi, float dyi, float dzi, hipStream_t st);
 template <typename T> int launch_box_filter(const Dims &d, const T *U, int r, float *coarse, float *labels, hipStream_t st);
 int launch_coarse_features(const float *C, int nx, int ny, int nz, int lu, float dxi, float dyi, float dzi,
                            float *feats, hipStream_t st);

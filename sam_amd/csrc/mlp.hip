@@ -55,7 +55,8 @@ __device__ __forceinline__ v4f mfma(v4s a, v4s b, v4f c) { return __builtin_amdg
 __device__ __forceinline__ float gelu_fwd(float x, float &dg)
 {
     const float c = 0.7978845608028654f, a = 0.044715f;
-    const float t = tanhf(c * (x + a * x * x * x));
+    // tanh(u) = 1 - 2/(1 + exp(2u)); exact limits +-1 for large |u|
+    const float t = 1.0f - __fdividef(2.0f, 1.0f + __expf(2.0f * c * (x + a * x * x * x)));
     dg = 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * c * (1.0f + 3.0f * a * x * x);
     return 0.5f * x * (1.0f + t);
 }
@@ -68,10 +69,11 @@ constexpr int T_W1 = 0, T_W2 = 4, T_W3 = 20, T_W4 = 36, T_W4T = 40, T_W3T = 44, 
 
 __device__ __forceinline__ float wparam(const float *P, int off, int ld, int row, int col) { return P[off + row * ld + col]; }
 
-// fill `ntiles` tiles (forward only: 40, with transposed: 76) from FP32 params
-__device__ void load_weight_tiles(const float *__restrict__ P, v4s *tiles, int ntiles)
+// the packed image: tile t, lane l at index t*64 + l, built once per
+// parameter update (k_pack_tiles) and copied into LDS by each block
+__global__ __launch_bounds__(256) void k_pack_tiles(const float *__restrict__ P, v4s *__restrict__ img)
 {
-    for (int e = threadIdx.x; e < ntiles * 64; e += blockDim.x) {
+    for (int e = blockIdx.x * 256 + threadIdx.x; e < T_COUNT * 64; e += gridDim.x * 256) {
         const int t = e / 64, l = e % 64, i = l & 15, g = l >> 4;
         float w[4];
         for (int kk = 0; kk < 4; kk++) {
@@ -86,8 +88,16 @@ __device__ void load_weight_tiles(const float *__restrict__ P, v4s *tiles, int n
             else { const int b = t - T_W2T; v = wparam(P, MLP_OFF_W2, 64, 16 * (b % 4) + k, 16 * (b / 4) + i); }
             w[kk] = v;
         }
-        tiles[e] = v4s{f2bf(w[0]), f2bf(w[1]), f2bf(w[2]), f2bf(w[3])};
+        img[e] = v4s{f2bf(w[0]), f2bf(w[1]), f2bf(w[2]), f2bf(w[3])};
     }
+}
+
+// copy the first `ntiles` tiles of the packed image into LDS (16 B per thread)
+__device__ __forceinline__ void copy_tiles(const v4s *__restrict__ img, v4s *tiles, int ntiles)
+{
+    const uint4 *src = reinterpret_cast<const uint4 *>(img);
+    uint4 *dst = reinterpret_cast<uint4 *>(tiles);
+    for (int e = threadIdx.x; e < ntiles * 32; e += blockDim.x) dst[e] = src[e];
 }
 
 __device__ __forceinline__ v4f bias4(const float *__restrict__ P, int off, int u0)
@@ -202,14 +212,15 @@ __device__ __forceinline__ void store4(short *base, int ld, int p, int u0, v4s v
     *reinterpret_cast<v4s *>(base + (long long)p * ld + u0) = v;
 }
 
-__global__ __launch_bounds__(256) void k_mlp_fb(const float *__restrict__ P, const float *__restrict__ norms,
+__global__ __launch_bounds__(256) void k_mlp_fb(const float *__restrict__ P, const v4s *__restrict__ img,
+                                                 const float *__restrict__ norms,
                                                  const float *__restrict__ feats, const float *__restrict__ labels,
                                                  long long npool, long long base, long long pa, long long pb,
                                                  int batch, TrainBufs tb, double *__restrict__ loss_part)
 {
     __shared__ v4s tiles[T_COUNT * 64];
     __shared__ double lred[4];
-    load_weight_tiles(P, tiles, T_COUNT);
+    copy_tiles(img, tiles, T_COUNT);
     __syncthreads();
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, i = lane & 15, g = lane >> 4;
     const int ntiles = batch / 16;
@@ -365,11 +376,12 @@ __device__ __forceinline__ void deviatoric(int g, v4f &o)
     }
 }
 
-__global__ __launch_bounds__(256) void k_mlp_infer(const float *__restrict__ P, const float *__restrict__ norms,
-                                                    const float *__restrict__ feats, float *__restrict__ out, long long n)
+__global__ __launch_bounds__(256) void k_mlp_infer(const float *__restrict__ P, const v4s *__restrict__ img,
+                                                    const float *__restrict__ norms, const float *__restrict__ feats,
+                                                    float *__restrict__ out, long long n)
 {
     __shared__ v4s tiles[T_W4T * 64];
-    load_weight_tiles(P, tiles, T_W4T);
+    copy_tiles(img, tiles, T_W4T);
     __syncthreads();
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, i = lane & 15, g = lane >> 4;
     const long long ntile = (n + 15) / 16;
@@ -410,40 +422,79 @@ __device__ __forceinline__ int out_slot(int o)
     return o == 0 ? 0 : o == 1 ? 2 : o == 2 ? 5 : o == 3 ? 1 : o == 4 ? 4 : o == 5 ? 3 : o;
 }
 
+// inputs one lane needs for a tile: the two neighbours of its 4 gradient
+// features and the point's u, v, w, T'
+template <typename T> struct FluxIn {
+    T nb[8];
+    T c[4];
+};
+
 template <typename T>
-__global__ __launch_bounds__(256) void k_les_mlp_flux(const float *__restrict__ Pm, const float *__restrict__ norms,
-                                                       const T *__restrict__ U, T *__restrict__ Q, int nx, int ny,
-                                                       int nz, int lu, float dxi, float dyi, float dzi)
+__device__ __forceinline__ void flux_load(const T *__restrict__ U, int nx, int ny, int nz, int lu, long long np,
+                                          long long p, int g, FluxIn<T> &in)
+{
+    const int x = (int)(p % nx), y = (int)((p / nx) % ny), z = (int)(p / ((long long)nx * ny));
+#pragma unroll
+    for (int kk = 0; kk < 4; kk++) {
+        const int k = 4 * g + kk;                 // 0..8: du_i/dx_j (i = k/3, j = k%3); 9..11: dT/dx_j
+        const int fld = k < 9 ? k / 3 : 3, dir = k < 9 ? k % 3 : k - 9;
+        T a = (T)0, b = (T)0;
+        if (g < 3 && fld < lu) {
+            int xp = x, xm = x, yp = y, ym = y, zp = z, zm = z;
+            if (dir == 0) { xp = (x + 1) & (nx - 1); xm = (x - 1) & (nx - 1); }
+            if (dir == 1) { yp = (y + 1) & (ny - 1); ym = (y - 1) & (ny - 1); }
+            if (dir == 2) { zp = (z + 1) & (nz - 1); zm = (z - 1) & (nz - 1); }
+            const T *f = U + fld * np;
+            a = f[((long long)zp * ny + yp) * nx + xp];
+            b = f[((long long)zm * ny + ym) * nx + xm];
+        }
+        in.nb[2 * kk] = a;
+        in.nb[2 * kk + 1] = b;
+    }
+    in.c[0] = U[p];
+    in.c[1] = U[np + p];
+    in.c[2] = U[2 * np + p];
+    in.c[3] = lu == 4 ? U[3 * np + p] : (T)0;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_les_mlp_flux(const float *__restrict__ Pm, const v4s *__restrict__ img,
+                                                       const float *__restrict__ norms, const T *__restrict__ U,
+                                                       T *__restrict__ Q, int nx, int ny, int nz, int lu, float dxi,
+                                                       float dyi, float dzi)
 {
     __shared__ v4s tiles[T_W4T * 64];
-    load_weight_tiles(Pm, tiles, T_W4T);
+    copy_tiles(img, tiles, T_W4T);
     __syncthreads();
     const long long np = (long long)nx * ny * nz;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, i = lane & 15, g = lane >> 4;
     const long long ntile = np / 16;
+    const long long stride = (long long)gridDim.x * 4;
     const float dd[3] = {dxi, dyi, dzi};
-    for (long long t = (long long)blockIdx.x * 4 + w; t < ntile; t += (long long)gridDim.x * 4) {
+    long long t = (long long)blockIdx.x * 4 + w;
+    FluxIn<T> cur;
+    if (t < ntile) flux_load(U, nx, ny, nz, lu, np, t * 16 + i, g, cur);
+    for (; t < ntile; t += stride) {
         const long long p = t * 16 + i;
-        const int x = (int)(p % nx), y = (int)((p / nx) % ny), z = (int)(p / ((long long)nx * ny));
+        FluxIn<T> nxt;                      // prefetch the next tile while this one is computed
+        const bool more = t + stride < ntile;
+        if (more) flux_load(U, nx, ny, nz, lu, np, (t + stride) * 16 + i, g, nxt);
         v4f xin = {0, 0, 0, 0};
-        if (g < 3) {
 #pragma unroll
-            for (int kk = 0; kk < 4; kk++) {
-                const int k = 4 * g + kk;                 // 0..8: du_i/dx_j (i = k/3, j = k%3); 9..11: dT/dx_j
-                const int fld = k < 9 ? k / 3 : 3, dir = k < 9 ? k % 3 : k - 9;
-                float v = 0.f;
-                if (fld < lu) v = cdiff(U + fld * np, nx, ny, nz, x, y, z, dir, dd[dir]);
-                xin[kk] = v * norms[k];
-            }
+        for (int kk = 0; kk < 4; kk++) {
+            const int k = 4 * g + kk;
+            const int dir = k < 9 ? k % 3 : (k < 12 ? k - 9 : 0);
+            const float v = 0.5f * (float)(cur.nb[2 * kk] - cur.nb[2 * kk + 1]) * dd[dir];
+            xin[kk] = g < 3 ? v * norms[k] : 0.f;
         }
         const int zo = opaque_zero();
         Acts A;
         mlp_forward(Pm + zo, tiles + zo, lane, xin, A);
         v4f o = A.out;
+#pragma unroll
         for (int r = 0; r < 4; r++) o[r] *= (4 * g + r < MLP_OUT) ? norms[16 + 4 * g + r] : 0.f;
         deviatoric(g, o);
-        const double uu[4] = {(double)U[p], (double)U[np + p], (double)U[2 * np + p],
-                              lu == 4 ? (double)U[3 * np + p] : 0.0};
+        const double uu[4] = {(double)cur.c[0], (double)cur.c[1], (double)cur.c[2], (double)cur.c[3]};
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             const int oi = 4 * g + r;
@@ -456,6 +507,7 @@ __global__ __launch_bounds__(256) void k_les_mlp_flux(const float *__restrict__ 
             const int b = slot == 0 ? 0 : slot == 1 ? 1 : slot == 2 ? 1 : slot <= 5 ? 2 : 3;
             Q[slot * np + p] = (T)(uu[a] * uu[b] + (double)o[r]);
         }
+        if (more) cur = nxt;
     }
 }
 
@@ -554,13 +606,19 @@ __global__ void k_set_norms(const double *__restrict__ part, int nb, long long n
 // ---------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------
-int launch_mlp_fb(const float *P, const float *norms, const float *feats, const float *labels, long long npool,
-                  long long base, long long pa, long long pb, int batch, const MlpTrainBufs &b, double *loss_part,
-                  int nblk, hipStream_t st)
+int launch_mlp_pack(const float *P, void *img, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_pack_tiles, dim3((T_COUNT * 64 + 255) / 256), dim3(256), 0, st, P, (v4s *)img);
+    return launched_mlp();
+}
+
+int launch_mlp_fb(const float *P, const void *img, const float *norms, const float *feats, const float *labels,
+                  long long npool, long long base, long long pa, long long pb, int batch, const MlpTrainBufs &b,
+                  double *loss_part, int nblk, hipStream_t st)
 {
     TrainBufs tb{b.a0, b.a1, b.a2, b.a3, b.dz1, b.dz2, b.dz3, b.dout};
-    hipLaunchKernelGGL(k_mlp_fb, dim3(nblk), dim3(256), 0, st, P, norms, feats, labels, npool, base, pa, pb, batch, tb,
-                       loss_part);
+    hipLaunchKernelGGL(k_mlp_fb, dim3(nblk), dim3(256), 0, st, P, (const v4s *)img, norms, feats, labels, npool, base,
+                       pa, pb, batch, tb, loss_part);
     return launched_mlp();
 }
 
@@ -585,24 +643,25 @@ int launch_sum_loss(const double *part, int n, double *out, hipStream_t st)
     return launched_mlp();
 }
 
-int launch_mlp_infer(const float *P, const float *norms, const float *feats, float *out, long long n, hipStream_t st)
+int launch_mlp_infer(const float *P, const void *img, const float *norms, const float *feats, float *out, long long n,
+                     hipStream_t st)
 {
     long long nt = (n + 15) / 16;
     unsigned nb = (unsigned)((nt + 3) / 4);
-    if (nb > 4096) nb = 4096;
-    hipLaunchKernelGGL(k_mlp_infer, dim3(nb), dim3(256), 0, st, P, norms, feats, out, n);
+    if (nb > 2048) nb = 2048;
+    hipLaunchKernelGGL(k_mlp_infer, dim3(nb), dim3(256), 0, st, P, (const v4s *)img, norms, feats, out, n);
     return launched_mlp();
 }
 
 template <typename T>
-int launch_les_mlp_flux(const Dims &d, const float *Pm, const float *norms, const T *U, T *Q, float dxi, float dyi,
-                        float dzi, hipStream_t st)
+int launch_les_mlp_flux(const Dims &d, const float *Pm, const void *img, const float *norms, const T *U, T *Q,
+                        float dxi, float dyi, float dzi, hipStream_t st)
 {
     long long nt = d.pfield / 16;
     unsigned nb = (unsigned)((nt + 3) / 4);
-    if (nb > 8192) nb = 8192;
-    hipLaunchKernelGGL((k_les_mlp_flux<T>), dim3(nb), dim3(256), 0, st, Pm, norms, U, Q, d.nx, d.ny, d.nz, d.lu, dxi,
-                       dyi, dzi);
+    if (nb > 2048) nb = 2048;
+    hipLaunchKernelGGL((k_les_mlp_flux<T>), dim3(nb), dim3(256), 0, st, Pm, (const v4s *)img, norms, U, Q, d.nx, d.ny,
+                       d.nz, d.lu, dxi, dyi, dzi);
     return launched_mlp();
 }
 
@@ -632,10 +691,10 @@ int launch_pool_norms(const float *feats, const float *labels, long long n, doub
     return launched_mlp();
 }
 
-template int launch_les_mlp_flux<double>(const Dims &, const float *, const float *, const double *, double *, float,
-                                         float, float, hipStream_t);
-template int launch_les_mlp_flux<float>(const Dims &, const float *, const float *, const float *, float *, float,
-                                        float, float, hipStream_t);
+template int launch_les_mlp_flux<double>(const Dims &, const float *, const void *, const float *, const double *,
+                                         double *, float, float, float, hipStream_t);
+template int launch_les_mlp_flux<float>(const Dims &, const float *, const void *, const float *, const float *,
+                                        float *, float, float, float, hipStream_t);
 template int launch_box_filter<double>(const Dims &, const double *, int, float *, float *, hipStream_t);
 template int launch_box_filter<float>(const Dims &, const float *, int, float *, float *, hipStream_t);
 
