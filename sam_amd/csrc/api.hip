@@ -1238,8 +1238,8 @@ template <typename T> static int mlp_flux(sam_handle *h, const T *P, T *Q)
 {
     const Dims &d = h->d;
     HIPCHK(hipStreamSynchronize(h->mlp->st));   // parameters written on the network's stream
-    RCHK(launch_les_mlp_flux<T>(d, h->mlp->P, h->mlp->img, h->mlp->norms, P, Q, (float)(d.nx / h->p.xl),
-                                (float)(d.ny / h->p.yl), (float)(d.nz / h->p.zl), h->st));
+    RCHK(launch_les_mlp_flux<T>(d, h->mlp->img, h->mlp->norms, P, Q, (float)(d.nx / h->p.xl), (float)(d.ny / h->p.yl),
+                                (float)(d.nz / h->p.zl), h->st));
     return SAM_OK;
 }
 

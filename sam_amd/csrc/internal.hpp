@@ -30,8 +30,7 @@ struct DevDiag {
     double cfl_x, cfl_y, cfl_z;
     double c_delta_sq;
     long long nt;
-    unsigned long long umax_bits[4];   // running max |u| as IEEE bits
-    unsigned long long divmax_bits;
+    unsigned long long divmax_bits;    // running max |div u| as IEEE bits
     double spare[6];
 };
 
@@ -109,7 +108,7 @@ constexpr int MLP_NPARAM = MLP_OFF_B4 + 16;
 struct MlpTrainBufs {
     short *a0, *a1, *a2, *a3, *dz1, *dz2, *dz3, *dout;
 };
-constexpr int MLP_TILE_BYTES = 76 * 64 * 8;   // packed BF16 weight-tile image
+constexpr int MLP_TILE_BYTES = 76 * 64 * 8 + 208 * 4;   // packed BF16 weight tiles + FP32 biases
 int launch_mlp_pack(const float *P, void *img, hipStream_t st);
 int launch_mlp_fb(const float *P, const void *img, const float *norms, const float *feats, const float *labels,
                   long long npool, long long base, long long pa, long long pb, int batch, const MlpTrainBufs &b,
@@ -121,8 +120,8 @@ int launch_sum_loss(const double *part, int n, double *out, hipStream_t st);
 int launch_mlp_infer(const float *P, const void *img, const float *norms, const float *feats, float *out, long long n,
                      hipStream_t st);
 template <typename T>
-int launch_les_mlp_flux(const Dims &d, const float *Pm, const void *img, const float *norms, const T *U, T *Q,
-                        float dxi, float dyi, float dzi, hipStream_t st);
+int launch_les_mlp_flux(const Dims &d, const void *img, const float *norms, const T *U, T *Q, float dxi, float dyi,
+                        float dzi, hipStream_t st);
 template <typename T> int launch_box_filter(const Dims &d, const T *U, int r, float *coarse, float *labels, hipStream_t st);
 int launch_coarse_features(const float *C, int nx, int ny, int nz, int lu, float dxi, float dyi, float dzi,
                            float *feats, hipStream_t st);

@@ -286,7 +286,7 @@ __global__ __launch_bounds__(CP * N / 8, 2) void k_xrow(T *__restrict__ Yr, Dims
     constexpr int NW = (NT + 63) / 64;
     __shared__ cpx<T> lds[CP * N];
     __shared__ cpx<T> stg[NT >= 64 ? (N / 2 + 8) * 2 * CP : 1];
-    __shared__ double red[NW][2 * LU];
+    __shared__ double red[NW][3 * LU];
     const int nyt = d.ny / (2 * CP);
     const int yt = blockIdx.x % nyt, z = blockIdx.x / nyt;
     const int y0 = yt * 2 * CP;
@@ -337,14 +337,15 @@ __global__ __launch_bounds__(CP * N / 8, 2) void k_xrow(T *__restrict__ Yr, Dims
             if (lane == 0) {
                 red[w][f] = s;
                 red[w][LU + f] = s2;
-                atomicMax(&diag->umax_bits[f], (unsigned long long)__double_as_longlong(mx));
+                red[w][2 * LU + f] = mx;
             }
         }
         __syncthreads();
-        if (threadIdx.x < 2 * LU) {
-            double t = 0;
-            for (int i = 0; i < NW; i++) t += red[i][threadIdx.x];
-            partials[(long long)blockIdx.x * 2 * LU + threadIdx.x] = t;
+        if (threadIdx.x < 3 * LU) {
+            const bool is_max = threadIdx.x >= 2 * LU;
+            double t = red[0][threadIdx.x];
+            for (int i = 1; i < NW; i++) t = is_max ? fmax(t, red[i][threadIdx.x]) : t + red[i][threadIdx.x];
+            partials[(long long)blockIdx.x * 3 * LU + threadIdx.x] = t;
         }
     }
 
@@ -661,33 +662,38 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_stats_phys(const T *__restrict__ P, long long np, int nvar, long long chunk,
                                                      double *__restrict__ partials, DevDiag *__restrict__ diag)
 {
-    __shared__ double red[4][8];
+    __shared__ double red[4][12];
     const long long b0 = (long long)blockIdx.x * chunk;
     const long long b1 = b0 + chunk < np ? b0 + chunk : np;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    for (int f = 0; f < nvar; f++) {
-        double s = 0, s2 = 0, mx = 0;
-        for (long long i = b0 + threadIdx.x; i < b1; i += 256) {
-            const double a = (double)P[f * np + i];
-            s += a;
-            s2 += a * a;
-            mx = fmax(mx, fabs(a));
+    double s[4] = {0, 0, 0, 0}, s2[4] = {0, 0, 0, 0}, mx[4] = {0, 0, 0, 0};
+    for (long long i = b0 + threadIdx.x; i < b1; i += 256) {
+#pragma unroll
+        for (int f = 0; f < 4; f++) {
+            if (f < nvar) {
+                const double a = (double)P[f * np + i];
+                s[f] += a;
+                s2[f] += a * a;
+                mx[f] = fmax(mx[f], fabs(a));
+            }
         }
-        s = wave_sum(s);
-        s2 = wave_sum(s2);
-        mx = wave_max(mx);
+    }
+#pragma unroll
+    for (int f = 0; f < 4; f++) {
+        if (f >= nvar) break;
+        const double ts = wave_sum(s[f]), ts2 = wave_sum(s2[f]), tm = wave_max(mx[f]);
         if (lane == 0) {
-            red[w][f] = s;
-            red[w][4 + f] = s2;
-            atomicMax(&diag->umax_bits[f], (unsigned long long)__double_as_longlong(mx));
+            red[w][f] = ts;
+            red[w][nvar + f] = ts2;
+            red[w][2 * nvar + f] = tm;
         }
     }
     __syncthreads();
-    if (threadIdx.x < 2 * nvar) {
-        const int f = threadIdx.x % nvar, which = threadIdx.x / nvar;
-        double t = 0;
-        for (int i = 0; i < 4; i++) t += red[i][which * 4 + f];
-        partials[(long long)blockIdx.x * 2 * nvar + threadIdx.x] = t;
+    if (threadIdx.x < 3 * nvar) {
+        const bool is_max = threadIdx.x >= 2 * nvar;
+        double t = red[0][threadIdx.x];
+        for (int i = 1; i < 4; i++) t = is_max ? fmax(t, red[i][threadIdx.x]) : t + red[i][threadIdx.x];
+        partials[(long long)blockIdx.x * 3 * nvar + threadIdx.x] = t;
     }
 }
 
@@ -695,7 +701,7 @@ int launch_stats_phys(const Dims &d, const void *P, int is_double, int nvar, dou
                       hipStream_t st, int *nblocks_out)
 {
     const long long np = d.pfield;
-    const long long chunk = 8192;
+    const long long chunk = 2048;
     const unsigned nb = (unsigned)((np + chunk - 1) / chunk);
     *nblocks_out = (int)nb;
     if (is_double)
@@ -707,35 +713,48 @@ int launch_stats_phys(const Dims &d, const void *P, int is_double, int nvar, dou
     return launched();
 }
 
-// vel_max.f:28-49 from per-block partials (rows of 2*Lu doubles)
+// vel_max.f:28-49 from per-block partials: rows of 3*Lu doubles (sum, sum of
+// squares, max |u| of each field)
 __global__ __launch_bounds__(256) void k_finalize(const double *__restrict__ partials, int nblocks, int lu,
                                                    DevDiag *__restrict__ diag, StepConsts k, Dims d, int update_dt)
 {
-    __shared__ double red[256];
-    __shared__ double tot[8];
-    for (int f = 0; f < 2 * lu; f++) {
-        double s = 0;
-        for (int b = threadIdx.x; b < nblocks; b += 256) s += partials[(long long)b * 2 * lu + f];
-        red[threadIdx.x] = s;
-        __syncthreads();
-        for (int o = 128; o > 0; o >>= 1) {
-            if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) tot[f] = red[0];
-        __syncthreads();
+    __shared__ double red[4][12];
+    __shared__ double tot[12];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, width = 3 * lu;
+    double acc[12];
+#pragma unroll
+    for (int f = 0; f < 12; f++) acc[f] = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += 256) {
+        const double *row = partials + (long long)b * width;
+#pragma unroll
+        for (int f = 0; f < 12; f++)
+            if (f < width) acc[f] = f < 2 * lu ? acc[f] + row[f] : fmax(acc[f], row[f]);
     }
+#pragma unroll
+    for (int f = 0; f < 12; f++) {
+        if (f < width) {
+            const double v = f < 2 * lu ? wave_sum(acc[f]) : wave_max(acc[f]);
+            if (lane == 0) red[w][f] = v;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < width) {
+        const bool is_max = threadIdx.x >= 2 * lu;
+        double t = red[0][threadIdx.x];
+        for (int i = 1; i < 4; i++) t = is_max ? fmax(t, red[i][threadIdx.x]) : t + red[i][threadIdx.x];
+        tot[threadIdx.x] = t;
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
         const double inv = 1.0 / ((double)d.nx * (double)d.ny * (double)d.nz);
         for (int f = 0; f < 4; f++) {
             if (f < lu) {
                 diag->u_avg[f] = tot[f] * inv;
                 diag->u_var[f] = tot[lu + f] * inv - diag->u_avg[f] * diag->u_avg[f];
-                diag->u_max[f] = __longlong_as_double((long long)diag->umax_bits[f]);
+                diag->u_max[f] = tot[2 * lu + f];
             } else {
                 diag->u_avg[f] = diag->u_var[f] = diag->u_max[f] = 0.0;
             }
-            diag->umax_bits[f] = 0ull;
         }
         diag->energy = 0.5 * (diag->u_var[0] + diag->u_var[1] + diag->u_var[2]);
         const double cx = diag->u_max[0] * d.kxm * d.wave_x;
@@ -967,9 +986,9 @@ int launch_copy16(const void *src, void *dst, long long n16, hipStream_t st)
 
 long long partials_capacity(const Dims &d)
 {
-    long long a = (long long)d.nz * d.ny;                            // x-row blocks x 2Lu
+    long long a = (long long)d.nz * d.ny * 2;                        // x-row blocks x 3Lu
     long long b = (long long)d.nxm * SPEC_YCH * (5 * d.lm + 1);       // spectra
-    long long c = (d.pfield / 8192 + 1) * 8;                          // physical stats
+    long long c = (d.pfield / 2048 + 1) * 12;                         // physical stats
     long long e = (d.pfield / 1024 + 1) * 2;                          // dm reductions
     long long m = a > b ? a : b;
     m = m > c ? m : c;

@@ -29,12 +29,6 @@ static inline int launched_mlp()
     return e == hipSuccess ? SAM_OK : SAM_ERR_HIP;
 }
 
-__device__ __forceinline__ short f2bf(float f)
-{
-    unsigned u = __float_as_uint(f);
-    u += 0x7fffu + ((u >> 16) & 1u);   // round to nearest even
-    return (short)(u >> 16);
-}
 __device__ __forceinline__ float bf2f(short h) { return __uint_as_float(((unsigned)(unsigned short)h) << 16); }
 
 // a zero the compiler cannot see through: re-basing the LDS weight tiles and
@@ -47,18 +41,32 @@ __device__ __forceinline__ int opaque_zero()
     return z;
 }
 
-__device__ __forceinline__ v4s bf4(v4f x) { return v4s{f2bf(x[0]), f2bf(x[1]), f2bf(x[2]), f2bf(x[3])}; }
+typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ unsigned pk2(float a, float b) { return __builtin_bit_cast(unsigned, v2bf{(__bf16)a, (__bf16)b}); }
+__device__ __forceinline__ v4s bf4(v4f x)
+{
+    typedef unsigned v2u __attribute__((ext_vector_type(2)));
+    return __builtin_bit_cast(v4s, v2u{pk2(x[0], x[1]), pk2(x[2], x[3])});
+}
 
 __device__ __forceinline__ v4f mfma(v4s a, v4s b, v4f c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
 
-// GELU, tanh form, and its derivative
-__device__ __forceinline__ float gelu_fwd(float x, float &dg)
+// GELU, tanh form: 0.5 x (1 + tanh u) with u = c (x + a x^3), evaluated as
+// x * sigmoid(2u) = x / (1 + 2^(-2u log2 e)) (one exp2, one reciprocal);
+// e -> inf for large negative x gives exactly -0, e -> 0 gives x
+constexpr float GELU_C = 0.7978845608028654f, GELU_A = 0.044715f;
+constexpr float GELU_K1 = -2.0f * GELU_C * 1.4426950408889634f, GELU_K2 = GELU_K1 * GELU_A;
+__device__ __forceinline__ float gelu_sig(float x)
 {
-    const float c = 0.7978845608028654f, a = 0.044715f;
-    // tanh(u) = 1 - 2/(1 + exp(2u)); exact limits +-1 for large |u|
-    const float t = 1.0f - __fdividef(2.0f, 1.0f + __expf(2.0f * c * (x + a * x * x * x)));
-    dg = 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * c * (1.0f + 3.0f * a * x * x);
-    return 0.5f * x * (1.0f + t);
+    const float e = __builtin_amdgcn_exp2f(x * __builtin_fmaf(GELU_K2, x * x, GELU_K1));
+    return __builtin_amdgcn_rcpf(1.0f + e);
+}
+__device__ __forceinline__ float gelu(float x) { return x * gelu_sig(x); }
+// derivative: s + g (1 - s) 2c (1 + 3 a x^2), s = sigmoid(2u), g = x s
+__device__ __forceinline__ float gelu_deriv(float x)
+{
+    const float s = gelu_sig(x), g = x * s;
+    return __builtin_fmaf(g * (1.0f - s), __builtin_fmaf(6.0f * GELU_C * GELU_A, x * x, 2.0f * GELU_C), s);
 }
 
 // ---- weight tiles in LDS ---------------------------------------------
@@ -69,8 +77,12 @@ constexpr int T_W1 = 0, T_W2 = 4, T_W3 = 20, T_W4 = 36, T_W4T = 40, T_W3T = 44, 
 
 __device__ __forceinline__ float wparam(const float *P, int off, int ld, int row, int col) { return P[off + row * ld + col]; }
 
-// the packed image: tile t, lane l at index t*64 + l, built once per
-// parameter update (k_pack_tiles) and copied into LDS by each block
+// biases in the packed image, after the tiles (FP32): b1, b2, b3, b4
+constexpr int NB_BIAS = 208, BO_1 = 0, BO_2 = 64, BO_3 = 128, BO_4 = 192;
+
+// the packed image: tile t, lane l at index t*64 + l, then the 208 biases;
+// built once per parameter update (k_pack_tiles) and copied into LDS by
+// each block
 __global__ __launch_bounds__(256) void k_pack_tiles(const float *__restrict__ P, v4s *__restrict__ img)
 {
     for (int e = blockIdx.x * 256 + threadIdx.x; e < T_COUNT * 64; e += gridDim.x * 256) {
@@ -88,21 +100,30 @@ __global__ __launch_bounds__(256) void k_pack_tiles(const float *__restrict__ P,
             else { const int b = t - T_W2T; v = wparam(P, MLP_OFF_W2, 64, 16 * (b % 4) + k, 16 * (b / 4) + i); }
             w[kk] = v;
         }
-        img[e] = v4s{f2bf(w[0]), f2bf(w[1]), f2bf(w[2]), f2bf(w[3])};
+        img[e] = bf4(v4f{w[0], w[1], w[2], w[3]});
     }
+    float *bimg = reinterpret_cast<float *>(img + T_COUNT * 64);
+    const int boff[3] = {MLP_OFF_B1, MLP_OFF_B2, MLP_OFF_B3};
+    for (int e = blockIdx.x * 256 + threadIdx.x; e < NB_BIAS; e += gridDim.x * 256)
+        bimg[e] = e < BO_4 ? P[boff[e / 64] + e % 64] : P[MLP_OFF_B4 + e - BO_4];
 }
 
-// copy the first `ntiles` tiles of the packed image into LDS (16 B per thread)
-__device__ __forceinline__ void copy_tiles(const v4s *__restrict__ img, v4s *tiles, int ntiles)
+// copy the first `ntiles` tiles of the packed image and the biases into LDS
+// (16 B per thread)
+__device__ __forceinline__ void copy_consts(const v4s *__restrict__ img, v4s *tiles, int ntiles, v4f *bias)
 {
     const uint4 *src = reinterpret_cast<const uint4 *>(img);
     uint4 *dst = reinterpret_cast<uint4 *>(tiles);
     for (int e = threadIdx.x; e < ntiles * 32; e += blockDim.x) dst[e] = src[e];
+    const uint4 *bs = reinterpret_cast<const uint4 *>(img + T_COUNT * 64);
+    uint4 *bd = reinterpret_cast<uint4 *>(bias);
+    for (int e = threadIdx.x; e < NB_BIAS / 4; e += blockDim.x) bd[e] = bs[e];
 }
 
-__device__ __forceinline__ v4f bias4(const float *__restrict__ P, int off, int u0)
+// biases u0..u0+3 of the bias block at `off` (LDS)
+__device__ __forceinline__ v4f bias4(const float *bias, int off, int u0)
 {
-    return v4f{P[off + u0], P[off + u0 + 1], P[off + u0 + 2], P[off + u0 + 3]};
+    return *reinterpret_cast<const v4f *>(bias + off + u0);
 }
 
 struct Acts {
@@ -114,39 +135,46 @@ struct Acts {
 
 __device__ __forceinline__ v4s gelu_bf4(v4f z)
 {
-    float d;
-    return v4s{f2bf(gelu_fwd(z[0], d)), f2bf(gelu_fwd(z[1], d)), f2bf(gelu_fwd(z[2], d)), f2bf(gelu_fwd(z[3], d))};
+    return bf4(v4f{gelu(z[0]), gelu(z[1]), gelu(z[2]), gelu(z[3])});
 }
 
-// one hidden layer 64 -> 64: z = W a + b, tiles tb..tb+15 (mb*4+kb)
-__device__ __forceinline__ void fw_hidden(const float *__restrict__ P, const v4s *tiles, int tb, int boff, int lane,
+// one hidden layer 64 -> 64: z = W a + b, tiles tb..tb+15 (mb*4+kb); the
+// four output blocks' MFMA chains are interleaved (kb outer)
+__device__ __forceinline__ void fw_hidden(const float *bias, const v4s *tiles, int tb, int boff, int lane,
                                           const v4s *ain, v4f *z, v4s *aout)
 {
     const int g = lane >> 4;
+    v4f acc[4];
+#pragma unroll
+    for (int mb = 0; mb < 4; mb++) acc[mb] = bias4(bias, boff, 16 * mb + 4 * g);
+#pragma unroll
+    for (int kb = 0; kb < 4; kb++)
+#pragma unroll
+        for (int mb = 0; mb < 4; mb++) acc[mb] = mfma(tiles[(tb + mb * 4 + kb) * 64 + lane], ain[kb], acc[mb]);
 #pragma unroll
     for (int mb = 0; mb < 4; mb++) {
-        v4f acc = bias4(P, boff, 16 * mb + 4 * g);
-#pragma unroll
-        for (int kb = 0; kb < 4; kb++) acc = mfma(tiles[(tb + mb * 4 + kb) * 64 + lane], ain[kb], acc);
-        z[mb] = acc;
-        aout[mb] = gelu_bf4(acc);
+        z[mb] = acc[mb];
+        aout[mb] = gelu_bf4(acc[mb]);
     }
 }
 
-// forward pass of one 16-point tile; x: this lane's 4 normalised inputs
-__device__ __forceinline__ void mlp_forward(const float *__restrict__ P, const v4s *tiles, int lane, v4f x, Acts &A)
+// forward pass of one 16-point tile; x: this lane's 4 normalised inputs;
+// bias and tiles in LDS
+__device__ __forceinline__ void mlp_forward(const float *bias, const v4s *tiles, int lane, v4f x, Acts &A)
 {
     const int g = lane >> 4;
     A.a0 = bf4(x);
+    v4f z[4];
+#pragma unroll
+    for (int mb = 0; mb < 4; mb++) z[mb] = mfma(tiles[(T_W1 + mb) * 64 + lane], A.a0, bias4(bias, BO_1, 16 * mb + 4 * g));
 #pragma unroll
     for (int mb = 0; mb < 4; mb++) {
-        const v4f z = mfma(tiles[(T_W1 + mb) * 64 + lane], A.a0, bias4(P, MLP_OFF_B1, 16 * mb + 4 * g));
-        A.z[0][mb] = z;
-        A.a[0][mb] = gelu_bf4(z);
+        A.z[0][mb] = z[mb];
+        A.a[0][mb] = gelu_bf4(z[mb]);
     }
-    fw_hidden(P, tiles, T_W2, MLP_OFF_B2, lane, A.a[0], A.z[1], A.a[1]);
-    fw_hidden(P, tiles, T_W3, MLP_OFF_B3, lane, A.a[1], A.z[2], A.a[2]);
-    v4f o = bias4(P, MLP_OFF_B4, 4 * g);
+    fw_hidden(bias, tiles, T_W2, BO_2, lane, A.a[0], A.z[1], A.a[1]);
+    fw_hidden(bias, tiles, T_W3, BO_3, lane, A.a[1], A.z[2], A.a[2]);
+    v4f o = bias4(bias, BO_4, 4 * g);
 #pragma unroll
     for (int kb = 0; kb < 4; kb++) o = mfma(tiles[(T_W4 + kb) * 64 + lane], A.a[2][kb], o);
     A.out = o;
@@ -154,13 +182,7 @@ __device__ __forceinline__ void mlp_forward(const float *__restrict__ P, const v
 
 __device__ __forceinline__ v4f gelu_grad4(v4f z)
 {
-    v4f d;
-    float t;
-    gelu_fwd(z[0], t); d[0] = t;
-    gelu_fwd(z[1], t); d[1] = t;
-    gelu_fwd(z[2], t); d[2] = t;
-    gelu_fwd(z[3], t); d[3] = t;
-    return d;
+    return v4f{gelu_deriv(z[0]), gelu_deriv(z[1]), gelu_deriv(z[2]), gelu_deriv(z[3])};
 }
 
 __device__ __forceinline__ void bw_gelu(const v4f *z, const v4f *da, v4s *dz)
@@ -219,8 +241,9 @@ __global__ __launch_bounds__(256) void k_mlp_fb(const float *__restrict__ P, con
                                                  int batch, TrainBufs tb, double *__restrict__ loss_part)
 {
     __shared__ v4s tiles[T_COUNT * 64];
+    __shared__ v4f biasv[NB_BIAS / 4];
     __shared__ double lred[4];
-    copy_tiles(img, tiles, T_COUNT);
+    copy_consts(img, tiles, T_COUNT, biasv);
     __syncthreads();
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, i = lane & 15, g = lane >> 4;
     const int ntiles = batch / 16;
@@ -236,9 +259,8 @@ __global__ __launch_bounds__(256) void k_mlp_fb(const float *__restrict__ P, con
         }
         const int zo = opaque_zero();
         const v4s *tl = tiles + zo;
-        const float *Pl = P + zo;
         Acts A;
-        mlp_forward(Pl, tl, lane, x, A);
+        mlp_forward(reinterpret_cast<const float *>(biasv) + zo, tl, lane, x, A);
         // MSE over the 9 normalised outputs: L = mean over batch*9 of (out - y/os)^2
         v4f dout;
 #pragma unroll
@@ -381,7 +403,8 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const float *__restrict__ P, 
                                                     float *__restrict__ out, long long n)
 {
     __shared__ v4s tiles[T_W4T * 64];
-    copy_tiles(img, tiles, T_W4T);
+    __shared__ v4f biasv[NB_BIAS / 4];
+    copy_consts(img, tiles, T_W4T, biasv);
     __syncthreads();
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, i = lane & 15, g = lane >> 4;
     const long long ntile = (n + 15) / 16;
@@ -392,7 +415,7 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const float *__restrict__ P, 
             for (int kk = 0; kk < 4; kk++) x[kk] = feats[p * 16 + 4 * g + kk] * norms[4 * g + kk];
         const int zo = opaque_zero();
         Acts A;
-        mlp_forward(P + zo, tiles + zo, lane, x, A);
+        mlp_forward(reinterpret_cast<const float *>(biasv) + zo, tiles + zo, lane, x, A);
         v4f o = A.out;
         for (int r = 0; r < 4; r++) o[r] *= (4 * g + r < MLP_OUT) ? norms[16 + 4 * g + r] : 0.f;
         deviatoric(g, o);
@@ -422,92 +445,105 @@ __device__ __forceinline__ int out_slot(int o)
     return o == 0 ? 0 : o == 1 ? 2 : o == 2 ? 5 : o == 3 ? 1 : o == 4 ? 4 : o == 5 ? 3 : o;
 }
 
-// inputs one lane needs for a tile: the two neighbours of its 4 gradient
-// features and the point's u, v, w, T'
-template <typename T> struct FluxIn {
-    T nb[8];
-    T c[4];
+// product of flux slot s (uu, uv, vv, uw, vw, ww, uT, vT, wT): fields a(s) b(s)
+__device__ __forceinline__ int slot_a(int s) { return s < 6 ? (s == 0 || s == 1 || s == 3 ? 0 : s == 5 ? 2 : 1) : s - 6; }
+__device__ __forceinline__ int slot_b(int s) { return s < 6 ? (s == 0 ? 0 : s <= 2 ? 1 : 2) : 3; }
+
+// the seven values lane (i, g) reads for a tile: field g at its point and the
+// point's x, y, z neighbours (periodic)
+template <typename T> struct Stencil {
+    T c, xp, xm, yp, ym, zp, zm;
 };
 
 template <typename T>
-__device__ __forceinline__ void flux_load(const T *__restrict__ U, int nx, int ny, int nz, int lu, long long np,
-                                          long long p, int g, FluxIn<T> &in)
+__device__ __forceinline__ void stencil_load(const T *__restrict__ F, int lx, int ly, int lz, unsigned p, Stencil<T> &s)
 {
-    const int x = (int)(p % nx), y = (int)((p / nx) % ny), z = (int)(p / ((long long)nx * ny));
-#pragma unroll
-    for (int kk = 0; kk < 4; kk++) {
-        const int k = 4 * g + kk;                 // 0..8: du_i/dx_j (i = k/3, j = k%3); 9..11: dT/dx_j
-        const int fld = k < 9 ? k / 3 : 3, dir = k < 9 ? k % 3 : k - 9;
-        T a = (T)0, b = (T)0;
-        if (g < 3 && fld < lu) {
-            int xp = x, xm = x, yp = y, ym = y, zp = z, zm = z;
-            if (dir == 0) { xp = (x + 1) & (nx - 1); xm = (x - 1) & (nx - 1); }
-            if (dir == 1) { yp = (y + 1) & (ny - 1); ym = (y - 1) & (ny - 1); }
-            if (dir == 2) { zp = (z + 1) & (nz - 1); zm = (z - 1) & (nz - 1); }
-            const T *f = U + fld * np;
-            a = f[((long long)zp * ny + yp) * nx + xp];
-            b = f[((long long)zm * ny + ym) * nx + xm];
-        }
-        in.nb[2 * kk] = a;
-        in.nb[2 * kk + 1] = b;
-    }
-    in.c[0] = U[p];
-    in.c[1] = U[np + p];
-    in.c[2] = U[2 * np + p];
-    in.c[3] = lu == 4 ? U[3 * np + p] : (T)0;
+    const unsigned mx = (1u << lx) - 1, my = (1u << ly) - 1, mz = (1u << lz) - 1;
+    const unsigned x = p & mx, y = (p >> lx) & my, z = p >> (lx + ly);
+    s.c = F[p];
+    s.xp = F[p - x + ((x + 1) & mx)];
+    s.xm = F[p - x + ((x - 1) & mx)];
+    s.yp = F[p + ((((y + 1) & my) - y) << lx)];
+    s.ym = F[p + ((((y - 1) & my) - y) << lx)];
+    s.zp = F[p + ((((z + 1) & mz) - z) << (lx + ly))];
+    s.zm = F[p + ((((z - 1) & mz) - z) << (lx + ly))];
 }
 
+// Fused LES flux: features (centred differences), the network and the SGS
+// fluxes F_ij = u_i u_j + tau_ij, F_Tj = u_j T' + q_j, one 16-point x-segment
+// per wavefront and tile.  Lane (i, g) loads field g's 7-point stencil at
+// point i and forms that field's 3 gradient features (k = 3g + j); the 12
+// features and 4 point values go through a per-wave LDS slot to the MFMA
+// layout (lane (i, g) holds features 4g..4g+3).  The next tile's stencil is
+// loaded while the current one is computed.
 template <typename T>
-__global__ __launch_bounds__(256) void k_les_mlp_flux(const float *__restrict__ Pm, const v4s *__restrict__ img,
-                                                       const float *__restrict__ norms, const T *__restrict__ U,
-                                                       T *__restrict__ Q, int nx, int ny, int nz, int lu, float dxi,
-                                                       float dyi, float dzi)
+__global__ __launch_bounds__(256) void k_les_mlp_flux(const v4s *__restrict__ img, const float *__restrict__ norms,
+                                                       const T *__restrict__ U, T *__restrict__ Q, int lx, int ly,
+                                                       int lz, int lu, float dxi, float dyi, float dzi)
 {
     __shared__ v4s tiles[T_W4T * 64];
-    copy_tiles(img, tiles, T_W4T);
+    __shared__ v4f biasv[NB_BIAS / 4];
+    __shared__ v4f featv[4][16 * 3];        // per wave: 16 points x 12 features
+    __shared__ T ctr[4][16 * 4];            // per wave: 16 points x (u, v, w, T')
+    copy_consts(img, tiles, T_W4T, biasv);
     __syncthreads();
-    const long long np = (long long)nx * ny * nz;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, i = lane & 15, g = lane >> 4;
-    const long long ntile = np / 16;
-    const long long stride = (long long)gridDim.x * 4;
+    const unsigned np = 1u << (lx + ly + lz);
+    const unsigned ntile = np >> 4;
+    float *feat = reinterpret_cast<float *>(featv[w]);
+    T *cw = ctr[w];
+
+    // lane constants
+    const bool has_f = g < lu;
+    const T *F = U + (size_t)(has_f ? g : 0) * np;
     const float dd[3] = {dxi, dyi, dzi};
-    long long t = (long long)blockIdx.x * 4 + w;
-    FluxIn<T> cur;
-    if (t < ntile) flux_load(U, nx, ny, nz, lu, np, t * 16 + i, g, cur);
-    for (; t < ntile; t += stride) {
-        const long long p = t * 16 + i;
-        FluxIn<T> nxt;                      // prefetch the next tile while this one is computed
-        const bool more = t + stride < ntile;
-        if (more) flux_load(U, nx, ny, nz, lu, np, (t + stride) * 16 + i, g, nxt);
-        v4f xin = {0, 0, 0, 0};
+    float nsc[3];
 #pragma unroll
-        for (int kk = 0; kk < 4; kk++) {
-            const int k = 4 * g + kk;
-            const int dir = k < 9 ? k % 3 : (k < 12 ? k - 9 : 0);
-            const float v = 0.5f * (float)(cur.nb[2 * kk] - cur.nb[2 * kk + 1]) * dd[dir];
-            xin[kk] = g < 3 ? v * norms[k] : 0.f;
-        }
+    for (int j = 0; j < 3; j++) nsc[j] = has_f ? norms[3 * g + j] : 0.f;
+    T *qp[4];
+    float osc[4];
+    int ca[4], cb[4];
+    bool ok[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int oi = 4 * g + r, sl = oi < MLP_OUT ? out_slot(oi) : 0;
+        ok[r] = oi < MLP_OUT && (sl < 6 || lu == 4);
+        osc[r] = ok[r] ? norms[16 + oi] : 0.f;
+        qp[r] = Q + (size_t)sl * np;
+        ca[r] = 4 * i + slot_a(sl);
+        cb[r] = 4 * i + slot_b(sl);
+    }
+
+    unsigned t = blockIdx.x * 4 + w;
+    const unsigned stride = gridDim.x * 4;
+    Stencil<T> cur;
+    if (t < ntile) stencil_load(F, lx, ly, lz, t * 16 + i, cur);
+    for (; t < ntile; t += stride) {
+        const unsigned p = t * 16 + i;
+        // this lane's 3 features and point value -> LDS
+        tile_sync<true>();
+        feat[12 * i + 3 * g + 0] = 0.5f * (float)(cur.xp - cur.xm) * dd[0] * nsc[0];
+        feat[12 * i + 3 * g + 1] = 0.5f * (float)(cur.yp - cur.ym) * dd[1] * nsc[1];
+        feat[12 * i + 3 * g + 2] = 0.5f * (float)(cur.zp - cur.zm) * dd[2] * nsc[2];
+        cw[4 * i + g] = has_f ? cur.c : (T)0;
+        // prefetch the next tile
+        const bool more = t + stride < ntile;
+        if (more) stencil_load(F, lx, ly, lz, (t + stride) * 16 + i, cur);
+        tile_sync<true>();
+        const v4f xin = g < 3 ? *reinterpret_cast<const v4f *>(feat + 12 * i + 4 * g) : v4f{0, 0, 0, 0};
+
         const int zo = opaque_zero();
         Acts A;
-        mlp_forward(Pm + zo, tiles + zo, lane, xin, A);
+        mlp_forward(reinterpret_cast<const float *>(biasv) + zo, tiles + zo, lane, xin, A);
         v4f o = A.out;
 #pragma unroll
-        for (int r = 0; r < 4; r++) o[r] *= (4 * g + r < MLP_OUT) ? norms[16 + 4 * g + r] : 0.f;
+        for (int r = 0; r < 4; r++) o[r] *= osc[r];
         deviatoric(g, o);
-        const double uu[4] = {(double)cur.c[0], (double)cur.c[1], (double)cur.c[2], (double)cur.c[3]};
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            const int oi = 4 * g + r;
-            if (oi >= MLP_OUT) continue;
-            const int slot = out_slot(oi);
-            if (slot >= 6 && lu < 4) continue;
-            // product of the slot: (uu,uv,vv,uw,vw,ww,uT,vT,wT)
-            const int a = slot == 0 ? 0 : slot == 1 ? 0 : slot == 2 ? 1 : slot == 3 ? 0 : slot == 4 ? 1 : slot == 5 ? 2
-                        : slot == 6 ? 0 : slot == 7 ? 1 : 2;
-            const int b = slot == 0 ? 0 : slot == 1 ? 1 : slot == 2 ? 1 : slot <= 5 ? 2 : 3;
-            Q[slot * np + p] = (T)(uu[a] * uu[b] + (double)o[r]);
+            if (!ok[r]) continue;
+            qp[r][p] = (T)((double)cw[ca[r]] * (double)cw[cb[r]] + (double)o[r]);
         }
-        if (more) cur = nxt;
     }
 }
 
@@ -654,14 +690,16 @@ int launch_mlp_infer(const float *P, const void *img, const float *norms, const 
 }
 
 template <typename T>
-int launch_les_mlp_flux(const Dims &d, const float *Pm, const void *img, const float *norms, const T *U, T *Q,
-                        float dxi, float dyi, float dzi, hipStream_t st)
+int launch_les_mlp_flux(const Dims &d, const void *img, const float *norms, const T *U, T *Q, float dxi, float dyi,
+                        float dzi, hipStream_t st)
 {
+    // 32-bit point indices; x-rows of at least one 16-point tile
+    if (d.pfield >= (1ll << 31) || d.nx < 16) return SAM_ERR_UNSUPPORTED;
     long long nt = d.pfield / 16;
     unsigned nb = (unsigned)((nt + 3) / 4);
     if (nb > 2048) nb = 2048;
-    hipLaunchKernelGGL((k_les_mlp_flux<T>), dim3(nb), dim3(256), 0, st, Pm, (const v4s *)img, norms, U, Q, d.nx, d.ny,
-                       d.nz, d.lu, dxi, dyi, dzi);
+    hipLaunchKernelGGL((k_les_mlp_flux<T>), dim3(nb), dim3(256), 0, st, (const v4s *)img, norms, U, Q,
+                       __builtin_ctz(d.nx), __builtin_ctz(d.ny), __builtin_ctz(d.nz), d.lu, dxi, dyi, dzi);
     return launched_mlp();
 }
 
@@ -691,10 +729,10 @@ int launch_pool_norms(const float *feats, const float *labels, long long n, doub
     return launched_mlp();
 }
 
-template int launch_les_mlp_flux<double>(const Dims &, const float *, const void *, const float *, const double *,
-                                         double *, float, float, float, hipStream_t);
-template int launch_les_mlp_flux<float>(const Dims &, const float *, const void *, const float *, const float *,
-                                        float *, float, float, float, hipStream_t);
+template int launch_les_mlp_flux<double>(const Dims &, const void *, const float *, const double *, double *, float,
+                                         float, float, hipStream_t);
+template int launch_les_mlp_flux<float>(const Dims &, const void *, const float *, const float *, float *, float,
+                                        float, float, hipStream_t);
 template int launch_box_filter<double>(const Dims &, const double *, int, float *, float *, hipStream_t);
 template int launch_box_filter<float>(const Dims &, const float *, int, float *, float *, hipStream_t);
 
