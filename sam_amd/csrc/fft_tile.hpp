@@ -17,7 +17,8 @@
 
 namespace samhip {
 
-template <typename T> struct cpx { T x, y; };
+// aligned to its full size so LDS and global accesses use single 8- or 16-byte ops
+template <typename T> struct alignas(2 * sizeof(T)) cpx { T x, y; };
 
 template <typename T> __device__ __forceinline__ cpx<T> operator+(cpx<T> a, cpx<T> b) { return {a.x + b.x, a.y + b.y}; }
 template <typename T> __device__ __forceinline__ cpx<T> operator-(cpx<T> a, cpx<T> b) { return {a.x - b.x, a.y - b.y}; }
@@ -122,9 +123,10 @@ template <bool WAVE> __device__ __forceinline__ void tile_sync()
 }
 
 // twiddles w^r, r = 0..R-1, of w = exp(S 2 pi i e0 / N) from the table
-// tw[k] = exp(-2 pi i k/N): w^1, w^2, w^4 are read, the rest multiplied
+// tw[k] = exp(-2 pi i k/N): w^1, w^2, w^4 are read (tw_read), the rest
+// multiplied (tw_expand)
 template <int R, int S, typename T>
-__device__ __forceinline__ void twiddles(const cpx<T> *__restrict__ tw, int e0, cpx<T> *w)
+__device__ __forceinline__ void tw_read(const cpx<T> *__restrict__ tw, int e0, cpx<T> *w)
 {
     auto ld = [&](int e) {
         cpx<T> t = tw[e];
@@ -132,16 +134,23 @@ __device__ __forceinline__ void twiddles(const cpx<T> *__restrict__ tw, int e0, 
         return t;
     };
     w[1] = ld(e0);
-    if constexpr (R >= 4) {
-        w[2] = ld(2 * e0);
-        w[3] = cmul(w[1], w[2]);
-    }
+    if constexpr (R >= 4) w[2] = ld(2 * e0);
+    if constexpr (R == 8) w[4] = ld(4 * e0);
+}
+template <int R, typename T> __device__ __forceinline__ void tw_expand(cpx<T> *w)
+{
+    if constexpr (R >= 4) w[3] = cmul(w[1], w[2]);
     if constexpr (R == 8) {
-        w[4] = ld(4 * e0);
         w[5] = cmul(w[1], w[4]);
         w[6] = cmul(w[2], w[4]);
         w[7] = cmul(w[3], w[4]);
     }
+}
+template <int R, int S, typename T>
+__device__ __forceinline__ void twiddles(const cpx<T> *__restrict__ tw, int e0, cpx<T> *w)
+{
+    tw_read<R, S>(tw, e0, w);
+    tw_expand<R>(w);
 }
 
 // Butterfly of radix R on v[0..R-1] (inputs in stride order r)
@@ -182,21 +191,37 @@ template <typename T, int N, int EPT> struct Tile {
         }
     }
 
+    // per-thread twiddles of one pass: butterfly m, power r
+    struct Tw {
+        cpx<T> w[8][8];
+    };
+
+    // read the pass-p twiddle bases (issued ahead of the pass so the table
+    // reads overlap the preceding LDS exchange)
+    template <int p, int S> __device__ static __forceinline__ void tw_fetch(const cpx<T> *__restrict__ tw, int j0, Tw &t)
+    {
+        constexpr int R = SC::radix(p);
+        constexpr int Ns = SC::ns(p);
+        if constexpr (Ns > 1) {
+#pragma unroll
+            for (int m = 0; m < EPT / R; m++) {
+                const int jl = (j0 + m * TPC) & (Ns - 1);
+                tw_read<R, S>(tw, jl * (N / (Ns * R)), t.w[m]);
+            }
+        }
+    }
+
     // twiddle + butterflies of pass p
-    template <int p, int S> __device__ static __forceinline__ void compute(const cpx<T> *__restrict__ tw, int j0, cpx<T> *v)
+    template <int p, int S> __device__ static __forceinline__ void compute(Tw &t, cpx<T> *v)
     {
         constexpr int R = SC::radix(p);
         constexpr int Ns = SC::ns(p);
 #pragma unroll
         for (int m = 0; m < EPT / R; m++) {
             if constexpr (Ns > 1) {
-                const int b = j0 + m * TPC;
-                const int jl = b & (Ns - 1);
-                constexpr int unit = N / (Ns * R);
-                cpx<T> w[R];
-                twiddles<R, S>(tw, jl * unit, w);
+                tw_expand<R>(t.w[m]);
 #pragma unroll
-                for (int r = 1; r < R; r++) v[m * R + r] = cmul(v[m * R + r], w[r]);
+                for (int r = 1; r < R; r++) v[m * R + r] = cmul(v[m * R + r], t.w[m][r]);
             }
             dftR<R, S>(v + m * R);
         }
@@ -216,6 +241,25 @@ template <typename T, int N, int EPT> struct Tile {
         }
     }
 
+    // passes p..P-1 with the pass-p twiddles already read
+    template <int S, int p>
+    __device__ static __forceinline__ void run_passes(cpx<T> *lds, const cpx<T> *__restrict__ tw, int c, int j0,
+                                                      cpx<T> *v, Tw &cur)
+    {
+        if constexpr (p + 1 < SC::P) {
+            Tw nxt;
+            tw_fetch<p + 1, S>(tw, j0, nxt);
+            compute<p, S>(cur, v);
+            lds_write<p>(lds, c, j0, v);
+            tile_sync<WAVE>();
+            lds_read<p + 1>(lds, c, j0, v);
+            tile_sync<WAVE>();
+            run_passes<S, p + 1>(lds, tw, c, j0, v, nxt);
+        } else {
+            compute<p, S>(cur, v);
+        }
+    }
+
     // passes p0..P-1 starting from registers already holding pass-p0
     // inputs; the final pass leaves its outputs in v (positions kout).
     // Requires a __syncthreads() before the call if other threads may still
@@ -223,14 +267,9 @@ template <typename T, int N, int EPT> struct Tile {
     template <int S, int p = 0>
     __device__ static __forceinline__ void run_regs(cpx<T> *lds, const cpx<T> *__restrict__ tw, int c, int j0, cpx<T> *v)
     {
-        compute<p, S>(tw, j0, v);
-        if constexpr (p + 1 < SC::P) {
-            lds_write<p>(lds, c, j0, v);
-            tile_sync<WAVE>();
-            lds_read<p + 1>(lds, c, j0, v);
-            tile_sync<WAVE>();
-            run_regs<S, p + 1>(lds, tw, c, j0, v);
-        }
+        Tw cur;
+        tw_fetch<p, S>(tw, j0, cur);
+        run_passes<S, p>(lds, tw, c, j0, v, cur);
     }
 
     // LDS -> registers (pass-0 inputs) -> FFT -> registers (final outputs).

@@ -36,7 +36,7 @@ static inline int launched()
 // columns whose strided side is read/written in 128-byte rows
 template <typename T> struct Row128 { static constexpr int C = 128 / (int)sizeof(cpx<T>); };
 
-template <typename T> struct Pair { cpx<T> a, b; };
+template <typename T> struct alignas(2 * sizeof(cpx<T>)) Pair { cpx<T> a, b; };
 
 // ---------------------------------------------------------------------
 // k_zb: z inverse transform of contiguous spectral columns
@@ -102,12 +102,19 @@ __global__ __launch_bounds__(C * N / EPT) void k_yb(const T *__restrict__ Zr, T 
     const int ix = b % d.nxm, f = b / d.nxm;
     const int z0 = zt * C;
     const cpx<T> *Z = reinterpret_cast<const cpx<T> *>(Zr) + ((long long)f * d.nxm + ix) * d.nym * d.nz + z0;
-    for (int e = threadIdx.x; e < C * N; e += NT) {
-        const int p = e / C, c = e % C;
+    // EPT items per thread: all loads issued before the first LDS write
+    cpx<T> x[EPT];
+#pragma unroll
+    for (int s = 0; s < EPT; s++) {
+        const int e = threadIdx.x + s * NT, p = e / C, c = e % C;
         const int iy = (p <= d.kym) ? p : ((p >= N - d.kym) ? p - N + d.nym : -1);
-        cpx<T> x = {(T)0, (T)0};
-        if (iy >= 0) x = Z[(long long)iy * d.nz + c];
-        lds[lds_idx<T, N>(c, p)] = x;
+        x[s] = {(T)0, (T)0};
+        if (iy >= 0) x[s] = Z[(long long)iy * d.nz + c];
+    }
+#pragma unroll
+    for (int s = 0; s < EPT; s++) {
+        const int e = threadIdx.x + s * NT;
+        lds[lds_idx<T, N>(e % C, e / C)] = x[s];
     }
     __syncthreads();
     const int c = threadIdx.x / TL::TPC, j0 = threadIdx.x % TL::TPC;
@@ -157,10 +164,19 @@ __global__ __launch_bounds__(C * N / EPT) void k_yf(const T *__restrict__ Yr, T 
     TL::store_natural(lds, c, j0, v);
     __syncthreads();
     cpx<T> *Z = reinterpret_cast<cpx<T> *>(Zr) + ((long long)q * d.nxm + ix) * d.nym * d.nz + z0;
-    for (int e = threadIdx.x; e < C * d.nym; e += NT) {
-        const int iy = e / C, cc = e % C;
+    // at most EPT items per thread (nym <= N): LDS reads first, then the stores
+    cpx<T> o[EPT];
+#pragma unroll
+    for (int s = 0; s < EPT; s++) {
+        const int e = threadIdx.x + s * NT;
+        const int iy = e < C * d.nym ? e / C : 0, cc = e % C;   // items past the end read row 0, unused
         const int p = (iy <= d.kym) ? iy : iy + N - d.nym;
-        Z[(long long)iy * d.nz + cc] = lds[lds_idx<T, N>(cc, p)];
+        o[s] = lds[lds_idx<T, N>(cc, p)];
+    }
+#pragma unroll
+    for (int s = 0; s < EPT; s++) {
+        const int e = threadIdx.x + s * NT;
+        if (e < C * d.nym) Z[(long long)(e / C) * d.nz + e % C] = o[s];
     }
 }
 
@@ -590,6 +606,14 @@ __global__ __launch_bounds__(LR * N / EPT) void k_zrhs(const T *__restrict__ Zr,
     }
     for (int kz = threadIdx.x; kz < d.nzm; kz += NT) {
         const int p = (kz <= d.kzm) ? kz : kz + N - d.nzm;
+        const long long si = col * d.nzm + kz;
+        // state and previous RHS first: their HBM reads overlap the LDS reads and the RHS
+        cpx<double> us[LU], rp[LU];
+#pragma unroll
+        for (int n = 0; n < LU; n++) {
+            us[n] = (mode == 0 || n >= 2) ? U[n * ss + si] : cpx<double>{0, 0};
+            rp[n] = (mode == 0 && m > 0) ? R[n * ss + si] : cpx<double>{0, 0};
+        }
         cpx<double> F[LR];
 #pragma unroll
         for (int qq = 0; qq < LR; qq++) {
@@ -598,11 +622,10 @@ __global__ __launch_bounds__(LR * N / EPT) void k_zrhs(const T *__restrict__ Zr,
         }
         const double rkz = d.wave_z * (double)kwave(kz, d.kzm, d.nzm);
         const double rk_sq = rk_sq2 + rkz * rkz;
-        const long long si = col * d.nzm + kz;
         cpx<double> wsav = {0, 0}, tsav = {0, 0};
         if constexpr (LU == 4) {
-            wsav = U[2 * ss + si];
-            tsav = U[3 * ss + si];
+            wsav = us[2];
+            tsav = us[3];
         }
         cpx<double> r[4], pr;
         rhs_mode<LU>(F, rkx, rky, rkz, rk_sq, wsav, tsav, k, r, pr);
@@ -611,11 +634,8 @@ __global__ __launch_bounds__(LR * N / EPT) void k_zrhs(const T *__restrict__ Zr,
             for (int n = 0; n < LU; n++) {
                 const double vfn = (n == 3) ? vf * k.pr_inv : vf;
                 const double vis_fac = exp(-vfn * rk_sq);
-                cpx<double> u = U[n * ss + si];
-                if (m > 0) {
-                    const cpx<double> rp = R[n * ss + si];
-                    u = {u.x + dtf2 * rp.x, u.y + dtf2 * rp.y};
-                }
+                cpx<double> u = us[n];
+                if (m > 0) u = {u.x + dtf2 * rp[n].x, u.y + dtf2 * rp[n].y};
                 u = {u.x + dtf1 * r[n].x, u.y + dtf1 * r[n].y};
                 Un[n * ss + si] = {u.x * vis_fac, u.y * vis_fac};
                 R[n * ss + si] = r[n];
