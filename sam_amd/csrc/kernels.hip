@@ -218,18 +218,38 @@ __device__ __forceinline__ void xrow_load_pairs(cpx<T> *lds, const cpx<T> *__res
     }
 }
 
+// x R2C outputs of the two rows of each column pair -> Y rows k = 0..kxm.
+// The first MI items per thread (all of them when kx_max is the dealiased
+// N/3) read their LDS pairs before any store.
 template <typename T, int N, int CP, int NT>
 __device__ __forceinline__ void xrow_store_pairs(const cpx<T> *lds, cpx<T> *__restrict__ Yq, int ny, int y0, int kxm,
                                                  T h)
 {
-    for (int e = threadIdx.x; e < CP * (kxm + 1); e += NT) {
-        const int k = e / CP, c = e % CP;
-        const cpx<T> zk = lds[lds_idx<T, N>(c, k)];
-        const cpx<T> zm = lds[lds_idx<T, N>(c, (N - k) & (N - 1))];
+    // FP64: one item ahead (registers are held by the four transformed fields)
+    constexpr int MI = sizeof(T) == 8 ? 1 : (CP * (N / 3 + 1) + NT - 1) / NT;
+    const int nitem = CP * (kxm + 1);
+    auto put = [&](int e, cpx<T> zk, cpx<T> zm) {
         Pair<T> pr;
         pr.a = {(zk.x + zm.x) * h, (zk.y - zm.y) * h};
         pr.b = {(zk.y + zm.y) * h, (zm.x - zk.x) * h};
-        *reinterpret_cast<Pair<T> *>(Yq + (long long)k * ny + y0 + 2 * c) = pr;
+        *reinterpret_cast<Pair<T> *>(Yq + (long long)(e / CP) * ny + y0 + 2 * (e % CP)) = pr;
+    };
+    cpx<T> zk[MI], zm[MI];
+#pragma unroll
+    for (int i = 0; i < MI; i++) {
+        const int e = threadIdx.x + i * NT, ee = e < nitem ? e : 0;
+        const int k = ee / CP, c = ee % CP;
+        zk[i] = lds[lds_idx<T, N>(c, k)];
+        zm[i] = lds[lds_idx<T, N>(c, (N - k) & (N - 1))];
+    }
+#pragma unroll
+    for (int i = 0; i < MI; i++) {
+        const int e = threadIdx.x + i * NT;
+        if (e < nitem) put(e, zk[i], zm[i]);
+    }
+    for (int e = threadIdx.x + MI * NT; e < nitem; e += NT) {
+        const int k = e / CP, c = e % CP;
+        put(e, lds[lds_idx<T, N>(c, k)], lds[lds_idx<T, N>(c, (N - k) & (N - 1))]);
     }
 }
 
@@ -278,13 +298,29 @@ __device__ __forceinline__ void wait_vmem() { asm volatile("s_waitcnt vmcnt(0)" 
 template <typename T, int N, int CP, int NT>
 __device__ __forceinline__ void xrow_unpack_stage(cpx<T> *lds, const cpx<T> *stg, int kxm)
 {
-    for (int e = threadIdx.x; e < CP * (kxm + 1); e += NT) {
+    constexpr int MI = sizeof(T) == 8 ? 1 : (CP * (N / 3 + 1) + NT - 1) / NT;
+    const int nitem = CP * (kxm + 1);
+    auto put = [&](int e, cpx<T> X, cpx<T> Yv) {
         const int k = e / CP, c = e % CP;
-        cpx<T> X = stg[k * 2 * CP + 2 * c], Yv = stg[k * 2 * CP + 2 * c + 1];
         if (k == 0) { X.y = (T)0; Yv.y = (T)0; }
         lds[lds_idx<T, N>(c, k)] = {X.x - Yv.y, X.y + Yv.x};
         if (k > 0) lds[lds_idx<T, N>(c, N - k)] = {X.x + Yv.y, Yv.x - X.y};
+    };
+    // staged row k: X, Y of column pair c at 2c, 2c + 1 (first MI items: reads first)
+    cpx<T> X[MI], Yv[MI];
+#pragma unroll
+    for (int i = 0; i < MI; i++) {
+        const int e = threadIdx.x + i * NT, ee = e < nitem ? e : 0;
+        X[i] = stg[(ee / CP) * 2 * CP + 2 * (ee % CP)];
+        Yv[i] = stg[(ee / CP) * 2 * CP + 2 * (ee % CP) + 1];
     }
+#pragma unroll
+    for (int i = 0; i < MI; i++) {
+        const int e = threadIdx.x + i * NT;
+        if (e < nitem) put(e, X[i], Yv[i]);
+    }
+    for (int e = threadIdx.x + MI * NT; e < nitem; e += NT)
+        put(e, stg[(e / CP) * 2 * CP + 2 * (e % CP)], stg[(e / CP) * 2 * CP + 2 * (e % CP) + 1]);
     const int nzero = N - 2 * kxm - 1;
     for (int e = threadIdx.x; e < CP * nzero; e += NT) {
         const int p = kxm + 1 + e / CP, c = e % CP;
@@ -292,17 +328,26 @@ __device__ __forceinline__ void xrow_unpack_stage(cpx<T> *lds, const cpx<T> *stg
     }
 }
 
+// waves per SIMD the x-row kernel is built for (FP64: two 4-wave blocks per
+// CU; FP32: one 8-wave block, the 4 transformed fields need > 128 VGPRs)
+template <typename T> constexpr int xrow_waves() { return 2; }
+
 template <typename T, int N, int CP, int LU, int LR>
-__global__ __launch_bounds__(CP * N / 8, 2) void k_xrow(T *__restrict__ Yr, Dims d, const cpx<T> *__restrict__ tw,
+__global__ __launch_bounds__(CP * N / 8, xrow_waves<T>()) void k_xrow(T *__restrict__ Yr, Dims d, const cpx<T> *__restrict__ tw,
                                                          int want_stats, double *__restrict__ partials,
                                                          DevDiag *__restrict__ diag)
 {
     using TL = Tile<T, N, 8>;
     constexpr int NT = CP * N / 8;
     constexpr int NW = (NT + 63) / 64;
-    __shared__ cpx<T> lds[CP * N];
-    __shared__ cpx<T> stg[NT >= 64 ? (N / 2 + 8) * 2 * CP : 1];
+    // FFT tile, then the staging buffer (>= one tile) at a fixed offset: in the
+    // product phase the two serve as alternating tiles addressed from the same
+    // per-lane offsets
+    constexpr int TILE = CP * N, STG = NT >= 64 ? (N / 2 + 8) * 2 * CP : 1;
+    static_assert(NT < 64 || STG >= TILE, "staging buffer holds a tile");
+    __shared__ cpx<T> smem[TILE + STG];
     __shared__ double red[NW][3 * LU];
+    cpx<T> *const lds = smem, *const stg = smem + TILE;
     const int nyt = d.ny / (2 * CP);
     const int yt = blockIdx.x % nyt, z = blockIdx.x / nyt;
     const int y0 = yt * 2 * CP;
@@ -375,11 +420,19 @@ __global__ __launch_bounds__(CP * N / 8, 2) void k_xrow(T *__restrict__ Yr, Dims
             po[k] = {a.x * b.x, a.y * b.y};
         }
         TL::out_to_in(po, pi);
+        // staged path: products alternate between the FFT tile and the (idle,
+        // at least as large) staging buffer, so the store of product q-1 and
+        // the transform of product q overlap and one barrier per product suffices
+        cpx<T> *tile = lds;
+        if constexpr (NT >= 64) {
+            if (q & 1) tile = stg;
+        } else {
+            __syncthreads();
+        }
+        TL::template run_regs<-1>(tile, tw, c, j0, pi);
+        TL::store_natural(tile, c, j0, pi);
         __syncthreads();
-        TL::template run_regs<-1>(lds, tw, c, j0, pi);
-        TL::store_natural(lds, c, j0, pi);
-        __syncthreads();
-        xrow_store_pairs<T, N, CP, NT>(lds, Y + q * fstride + zoff, d.ny, y0, d.kxm, h);
+        xrow_store_pairs<T, N, CP, NT>(tile, Y + q * fstride + zoff, d.ny, y0, d.kxm, h);
     }
 }
 
