@@ -276,6 +276,18 @@ __host__ __device__ constexpr int prod_j(int q) { return q == 0 ? 0 : q == 1 ? 1
 // next field's tile arrives while the current one is transformed.
 using lds_void_t = __attribute__((address_space(3))) void;
 
+// One global_load_lds_dwordx4: lane l's 16 bytes at gsrc land at
+// lds_dst + 16 l.  Issued from inline assembly so the compiler's wait-count
+// tracking does not see an LDS write in flight and drain it before every
+// unrelated LDS read; the callers wait (s_waitcnt vmcnt) and barrier before
+// they read the staged data.  The instruction has no register outputs.
+__device__ __forceinline__ void lds_dma16(const void *gsrc, const void *lds_dst)
+{
+    const unsigned m0 = __builtin_amdgcn_readfirstlane((unsigned)(uintptr_t)(const lds_void_t *)lds_dst);
+    asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(gsrc), "s"(m0)
+                 : "memory", "m0");
+}
+
 template <typename T, int CP, int NT>
 __device__ __forceinline__ void xrow_issue_stage(cpx<T> *stg, const cpx<T> *__restrict__ Yf, int ny, int y0, int kxm)
 {
@@ -288,7 +300,7 @@ __device__ __forceinline__ void xrow_issue_stage(cpx<T> *stg, const cpx<T> *__re
         int k = i * 8 + (lane >> 3);
         if (k > kxm) k = kxm;      // rows past kx_max: re-read a valid row, ignored
         const char *src = reinterpret_cast<const char *>(Yf + (long long)k * ny + y0) + (lane & 7) * 16;
-        __builtin_amdgcn_global_load_lds(src, (lds_void_t *)(stg + i * 16 * CP), 16, 0, 0);
+        lds_dma16(src, stg + i * 16 * CP);
     }
 }
 
@@ -328,12 +340,37 @@ __device__ __forceinline__ void xrow_unpack_stage(cpx<T> *lds, const cpx<T> *stg
     }
 }
 
+// Inverse x transforms of the LU fields, whole wavefronts: the next field is
+// streamed into the staging buffer during this field's transform.  The tile
+// and staging pointers are restrict-qualified so the compiler knows the
+// transform's LDS traffic is independent of the in-flight LDS DMA and does
+// not drain it before every tile read.
+template <typename T, int N, int CP, int LU>
+__device__ __forceinline__ void xrow_phase_a(cpx<T> *__restrict__ lds, cpx<T> *__restrict__ stg,
+                                             const cpx<T> *__restrict__ tw, const cpx<T> *__restrict__ Yz,
+                                             long long fstride, int ny, int y0, int kxm, int c, int j0,
+                                             cpx<T> (&uf)[LU][8])
+{
+    using TL = Tile<T, N, 8>;
+    constexpr int NT = CP * N / 8;
+    xrow_issue_stage<T, CP, NT>(stg, Yz, ny, y0, kxm);
+#pragma unroll
+    for (int f = 0; f < LU; f++) {
+        wait_vmem();
+        __syncthreads();   // staged tile complete; every wave is done with the FFT tile
+        xrow_unpack_stage<T, N, CP, NT>(lds, stg, kxm);
+        __syncthreads();   // FFT tile ready; staging buffer free
+        if (f + 1 < LU) xrow_issue_stage<T, CP, NT>(stg, Yz + (f + 1) * fstride, ny, y0, kxm);
+        TL::template run_from_lds<+1>(lds, tw, c, j0, uf[f]);
+    }
+}
+
 // waves per SIMD the x-row kernel is built for (FP64: two 4-wave blocks per
 // CU; FP32: one 8-wave block, the 4 transformed fields need > 128 VGPRs)
 template <typename T> constexpr int xrow_waves() { return 2; }
 
 template <typename T, int N, int CP, int LU, int LR>
-__global__ __launch_bounds__(CP * N / 8, xrow_waves<T>()) void k_xrow(T *__restrict__ Yr, Dims d, const cpx<T> *__restrict__ tw,
+__global__ __launch_bounds__(CP * N / 8, xrow_waves<T>()) void k_xrow(T *__restrict__ Yr, Dims d, const cpx<T> *__restrict__ twg,
                                                          int want_stats, double *__restrict__ partials,
                                                          DevDiag *__restrict__ diag)
 {
@@ -347,6 +384,11 @@ __global__ __launch_bounds__(CP * N / 8, xrow_waves<T>()) void k_xrow(T *__restr
     static_assert(NT < 64 || STG >= TILE, "staging buffer holds a tile");
     __shared__ cpx<T> smem[TILE + STG];
     __shared__ double red[NW][3 * LU];
+    // the x twiddles (the passes read indices < N/2) in LDS: the transforms then
+    // issue no vector-memory reads, so the staged loads and the product stores
+    // stay in flight across them instead of being drained by a twiddle wait
+    __shared__ cpx<T> tw[N / 2];
+    for (int e = threadIdx.x; e < N / 2; e += NT) tw[e] = twg[e];
     cpx<T> *const lds = smem, *const stg = smem + TILE;
     const int nyt = d.ny / (2 * CP);
     const int yt = blockIdx.x % nyt, z = blockIdx.x / nyt;
@@ -358,17 +400,7 @@ __global__ __launch_bounds__(CP * N / 8, xrow_waves<T>()) void k_xrow(T *__restr
 
     cpx<T> uf[LU][8];
     if constexpr (NT >= 64) {
-        // whole wavefronts: stream the next field into LDS during this field's FFT
-        xrow_issue_stage<T, CP, NT>(stg, Y + zoff, d.ny, y0, d.kxm);
-#pragma unroll
-        for (int f = 0; f < LU; f++) {
-            wait_vmem();
-            __syncthreads();   // staged tile complete; every wave is done with the FFT tile
-            xrow_unpack_stage<T, N, CP, NT>(lds, stg, d.kxm);
-            __syncthreads();   // FFT tile ready; staging buffer free
-            if (f + 1 < LU) xrow_issue_stage<T, CP, NT>(stg, Y + (f + 1) * fstride + zoff, d.ny, y0, d.kxm);
-            TL::template run_from_lds<+1>(lds, tw, c, j0, uf[f]);
-        }
+        xrow_phase_a<T, N, CP, LU>(lds, stg, tw, Y + zoff, fstride, d.ny, y0, d.kxm, c, j0, uf);
     } else {
         // sub-wavefront groups (short rows): direct cooperative loads
 #pragma unroll
