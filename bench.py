@@ -1,8 +1,8 @@
 #!/usr/bin/env python3
 """Throughput of SAM's RK3 pseudo-spectral step on one MI355X.
 
-Workload (default): the stratified Boussinesq DNS of the 256^3 config run at
-512^3 -- seeded random-phase solenoidal velocity, E(k) ~ k^4 exp(-2(k/4)^2),
+Workload (default): stratified Boussinesq DNS at 512^3 -- seeded
+random-phase solenoidal velocity, E(k) ~ k^4 exp(-2(k/4)^2),
 u_rms = 1, T' = 0, N = 4 (grav*lapse/To = 16), nu = kappa = 5e-4, Pr = 1,
 RK3 with CFL 0.5, 2/3 dealiasing.  One "step" is one full RK3 step (three
 substeps) of the whole 512^3 grid with the state resident in HBM.
@@ -117,7 +117,7 @@ def measured_traffic(n, kernel, prec):
 
 
 def les_comparison(S, ic, dns_u, n_dns, steps, warmup):
-    """config-4/5 learned closure: train on the box-filtered DNS state
+    """Learned closure: train on the box-filtered DNS state
     (ratio n_dns/128), then time the 128^3 LES step with the learned closure
     against the dynamic model (same IC, FP32 transforms, FP64 state)."""
     from sam_amd import les as L

@@ -365,11 +365,118 @@ __device__ __forceinline__ void xrow_phase_a(cpx<T> *__restrict__ lds, cpx<T> *_
     }
 }
 
+// ---- double-staged inverse phase (dealiased rows: kx_max + 1 = N/3 + 1) ----
+// Two staging buffers of exactly N/3 + 1 rows; field f + 2 streams in while
+// field f + 1 is in flight and field f is transformed.  Staged rows keep a
+// 16-byte chunk swizzle (chunk j of row r at position j ^ (r & 7)) so the
+// pass-0 reads straight from the staged rows (eight consecutive rows per
+// 8-lane group) are conflict-free.
+template <int N> constexpr int xrow_srows() { return N / 3 + 1; }
+
+template <typename T, int N, int CP, int NT>
+__device__ __forceinline__ void xrow_issue_stage_ds(cpx<T> *stg, const cpx<T> *__restrict__ Yf, int ny, int y0)
+{
+    static_assert(2 * CP * (int)sizeof(cpx<T>) == 128, "one k-row is 128 bytes");
+    constexpr int SR = xrow_srows<N>(), NI = (SR + 7) / 8, NW = NT / 64;
+    static_assert(SR >= 8, "at least one full instruction of rows");
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int i = w; i < NI; i += NW) {
+        const int r0 = i * 8 < SR - 8 ? i * 8 : SR - 8;   // the last group overlaps: no row past the buffer
+        const int k = r0 + (lane >> 3);
+        const int chunk = (lane & 7) ^ (k & 7);
+        const char *src = reinterpret_cast<const char *>(Yf + (long long)k * ny + y0) + chunk * 16;
+        lds_dma16(src, stg + r0 * 2 * CP);
+    }
+}
+
+// number of staging instructions wave w issues per field
+template <int N, int NT> __device__ __forceinline__ int xrow_stage_count(int w)
+{
+    constexpr int NI = (xrow_srows<N>() + 7) / 8, NW = NT / 64;
+    return w < NI ? (NI - w + NW - 1) / NW : 0;
+}
+
+// wait until at most n of this wave's vector-memory operations are in flight
+__device__ __forceinline__ void wait_vmem_le(int n)
+{
+    switch (n) {
+    case 1: asm volatile("s_waitcnt vmcnt(1)" ::: "memory"); break;
+    case 2: asm volatile("s_waitcnt vmcnt(2)" ::: "memory"); break;
+    case 3: asm volatile("s_waitcnt vmcnt(3)" ::: "memory"); break;
+    case 4: asm volatile("s_waitcnt vmcnt(4)" ::: "memory"); break;
+    case 5: asm volatile("s_waitcnt vmcnt(5)" ::: "memory"); break;
+    case 6: asm volatile("s_waitcnt vmcnt(6)" ::: "memory"); break;
+    case 7: asm volatile("s_waitcnt vmcnt(7)" ::: "memory"); break;
+    case 8: asm volatile("s_waitcnt vmcnt(8)" ::: "memory"); break;
+    default: asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); break;
+    }
+}
+
+// pass-0 inputs of column pair c (element j0 + s N/8 in slot s) straight
+// from the staged rows: packed two-row spectrum, zero band in the middle
+template <typename T, int N, int CP>
+__device__ __forceinline__ void xrow_pass0_from_stage(const cpx<T> *stg, int c, int j0, int kxm, cpx<T> *v)
+{
+    static_assert(Tile<T, N, 8>::kin(0, 3) == 3, "pass-0 slot s holds element j0 + s N/8");
+    auto rowpair = [&](int k, cpx<T> &X, cpx<T> &Yv) {
+        const cpx<T> *row = stg + k * 2 * CP;
+        if constexpr (sizeof(T) == 8) {
+            X = row[(2 * c) ^ (k & 7)];
+            Yv = row[(2 * c + 1) ^ (k & 7)];
+        } else {
+            const Pair<T> pr = reinterpret_cast<const Pair<T> *>(row)[c ^ (k & 7)];
+            X = pr.a;
+            Yv = pr.b;
+        }
+    };
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+        const int p = j0 + (N / 8) * s;
+        cpx<T> out = {(T)0, (T)0};
+        if (p <= kxm) {
+            cpx<T> X, Yv;
+            rowpair(p, X, Yv);
+            if (p == 0) { X.y = (T)0; Yv.y = (T)0; }
+            out = {X.x - Yv.y, X.y + Yv.x};
+        } else if (p >= N - kxm) {
+            cpx<T> X, Yv;
+            rowpair(N - p, X, Yv);
+            out = {X.x + Yv.y, Yv.x - X.y};
+        }
+        v[s] = out;
+    }
+}
+
+template <typename T, int N, int CP, int LU>
+__device__ __forceinline__ void xrow_phase_a_ds(cpx<T> *lds, cpx<T> *stg, const cpx<T> *tw,
+                                                const cpx<T> *__restrict__ Yz, long long fstride, int ny, int y0,
+                                                int kxm, int c, int j0, cpx<T> (&uf)[LU][8])
+{
+    using TL = Tile<T, N, 8>;
+    constexpr int NT = CP * N / 8, SR = xrow_srows<N>();
+    cpx<T> *const st[2] = {stg, stg + SR * 2 * CP};
+    const int mine = xrow_stage_count<N, NT>(threadIdx.x >> 6);
+    xrow_issue_stage_ds<T, N, CP, NT>(st[0], Yz, ny, y0);
+    if (LU > 1) xrow_issue_stage_ds<T, N, CP, NT>(st[1], Yz + fstride, ny, y0);
+#pragma unroll
+    for (int f = 0; f < LU; f++) {
+        if (f + 1 < LU) wait_vmem_le(mine);   // field f landed; field f + 1 may still be in flight
+        else wait_vmem();
+        __syncthreads();                       // ... for every wave's share of the rows
+        xrow_pass0_from_stage<T, N, CP>(st[f & 1], c, j0, kxm, uf[f]);
+        if (f + 2 < LU) {
+            __syncthreads();                   // every wave has read field f's buffer
+            xrow_issue_stage_ds<T, N, CP, NT>(st[f & 1], Yz + (f + 2) * fstride, ny, y0);
+        }
+        TL::template run_regs<+1>(lds, tw, c, j0, uf[f]);
+    }
+}
+
 // waves per SIMD the x-row kernel is built for (FP64: two 4-wave blocks per
 // CU; FP32: one 8-wave block, the 4 transformed fields need > 128 VGPRs)
 template <typename T> constexpr int xrow_waves() { return 2; }
 
-template <typename T, int N, int CP, int LU, int LR>
+template <typename T, int N, int CP, int LU, int LR, bool DS>
 __global__ __launch_bounds__(CP * N / 8, xrow_waves<T>()) void k_xrow(T *__restrict__ Yr, Dims d, const cpx<T> *__restrict__ twg,
                                                          int want_stats, double *__restrict__ partials,
                                                          DevDiag *__restrict__ diag)
@@ -380,7 +487,8 @@ __global__ __launch_bounds__(CP * N / 8, xrow_waves<T>()) void k_xrow(T *__restr
     // FFT tile, then the staging buffer (>= one tile) at a fixed offset: in the
     // product phase the two serve as alternating tiles addressed from the same
     // per-lane offsets
-    constexpr int TILE = CP * N, STG = NT >= 64 ? (N / 2 + 8) * 2 * CP : 1;
+    constexpr int TILE = CP * N;
+    constexpr int STG = DS ? 2 * xrow_srows<N>() * 2 * CP : NT >= 64 ? (N / 2 + 8) * 2 * CP : 1;
     static_assert(NT < 64 || STG >= TILE, "staging buffer holds a tile");
     __shared__ cpx<T> smem[TILE + STG];
     __shared__ double red[NW][3 * LU];
@@ -399,7 +507,9 @@ __global__ __launch_bounds__(CP * N / 8, xrow_waves<T>()) void k_xrow(T *__restr
     const long long zoff = (long long)z * d.nxm * d.ny;
 
     cpx<T> uf[LU][8];
-    if constexpr (NT >= 64) {
+    if constexpr (DS) {
+        xrow_phase_a_ds<T, N, CP, LU>(lds, stg, tw, Y + zoff, fstride, d.ny, y0, d.kxm, c, j0, uf);
+    } else if constexpr (NT >= 64) {
         xrow_phase_a<T, N, CP, LU>(lds, stg, tw, Y + zoff, fstride, d.ny, y0, d.kxm, c, j0, uf);
     } else {
         // sub-wavefront groups (short rows): direct cooperative loads
@@ -480,12 +590,27 @@ int launch_xrow(const Dims &d, T *Y, int want_stats, double *partials, DevDiag *
     if (d.ny % (2 * CP)) return SAM_ERR_UNSUPPORTED;
     SAM_SWITCH_N(d.nx, {
         if constexpr (CP * N / 8 <= 1024) {
-            if (d.lu == 4)
-                hipLaunchKernelGGL((k_xrow<T, N, CP, 4, 9>), dim3(nb), dim3(CP * N / 8), 0, st, Y, d,
-                                   (const cpx<T> *)twx, want_stats, partials, diag);
-            else
-                hipLaunchKernelGGL((k_xrow<T, N, CP, 3, 6>), dim3(nb), dim3(CP * N / 8), 0, st, Y, d,
-                                   (const cpx<T> *)twx, want_stats, partials, diag);
+            // double staging: whole-wave columns, dealiased rows, and the tile, both
+            // staging buffers and the twiddles within the 160 KiB of LDS
+            constexpr bool ds_ok = N / 8 <= 64 && (CP * N / 8) % 64 == 0 &&
+                                   (CP * N + 2 * xrow_srows<N>() * 2 * CP + N / 2) * sizeof(cpx<T>) + 512 <= 160 * 1024;
+            const bool ds = ds_ok && d.kxm + 1 == xrow_srows<N>();
+            const cpx<T> *tw = (const cpx<T> *)twx;
+            if (d.lu == 4) {
+                if (ds)
+                    hipLaunchKernelGGL((k_xrow<T, N, CP, 4, 9, ds_ok>), dim3(nb), dim3(CP * N / 8), 0, st, Y, d, tw,
+                                       want_stats, partials, diag);
+                else
+                    hipLaunchKernelGGL((k_xrow<T, N, CP, 4, 9, false>), dim3(nb), dim3(CP * N / 8), 0, st, Y, d, tw,
+                                       want_stats, partials, diag);
+            } else {
+                if (ds)
+                    hipLaunchKernelGGL((k_xrow<T, N, CP, 3, 6, ds_ok>), dim3(nb), dim3(CP * N / 8), 0, st, Y, d, tw,
+                                       want_stats, partials, diag);
+                else
+                    hipLaunchKernelGGL((k_xrow<T, N, CP, 3, 6, false>), dim3(nb), dim3(CP * N / 8), 0, st, Y, d, tw,
+                                       want_stats, partials, diag);
+            }
         } else {
             return SAM_ERR_UNSUPPORTED;
         }

@@ -1,5 +1,5 @@
 """The Fortran driver (sam_amd/driver) over the C ABI: input.dat parsing and
-checks (CPU; check_inputs.f error codes), and the config-1 run end to end
+checks (CPU; check_inputs.f error codes), and a 64^3 Taylor-Green run end to end
 from SAM restart files (GPU)."""
 import os
 import subprocess
@@ -15,7 +15,7 @@ DRIVER = os.path.join(ROOT, "sam_amd", "driver", "samhip_driver")
 
 needs_driver = pytest.mark.skipif(not os.path.exists(DRIVER), reason="driver not built (make -C sam_amd/driver)")
 
-CONFIG1 = dict(Nx=64, Ny=64, Nz=64, xL=2 * np.pi, yL=2 * np.pi, zL=2 * np.pi, vis=0.0025,
+TG64 = dict(Nx=64, Ny=64, Nz=64, xL=2 * np.pi, yL=2 * np.pi, zL=2 * np.pi, vis=0.0025,
                n_steps=20, i_restart=1, nt_restart=0, i_cfl=0, dt0=0.01, nrk_max=3,
                n_dealias=1, i_fs=0, n_skip_h=5)
 
@@ -27,7 +27,7 @@ def run(tmp_path, **params):
 
 @needs_driver
 def test_missing_required_input(tmp_path):
-    p = dict(CONFIG1)
+    p = dict(TG64)
     del p["vis"]
     r = run(tmp_path, **p)
     assert r.returncode == 6 and "vis" in r.stdout
@@ -35,7 +35,7 @@ def test_missing_required_input(tmp_path):
 
 @needs_driver
 def test_dt0_required_when_fixed_dt(tmp_path):
-    p = dict(CONFIG1)
+    p = dict(TG64)
     del p["dt0"]
     r = run(tmp_path, **p)
     assert r.returncode == 6 and "dt0" in r.stdout
@@ -43,7 +43,7 @@ def test_dt0_required_when_fixed_dt(tmp_path):
 
 @needs_driver
 def test_invalid_value(tmp_path):
-    r = run(tmp_path, **dict(CONFIG1, nrk_max=4))
+    r = run(tmp_path, **dict(TG64, nrk_max=4))
     assert r.returncode == 9 and "nrk_max" in r.stdout
 
 
@@ -65,12 +65,12 @@ def test_python_parser_multipliers(tmp_path):
 
 @pytest.mark.gpu
 @needs_driver
-def test_config1_end_to_end(tmp_path, oracle_mod):
+def test_taylor_green_run_end_to_end(tmp_path, oracle_mod):
     """Taylor-Green 64^3, nu=1/400, dt=0.01, 20 RK3 steps from vel.000000"""
     u0 = ic.taylor_green(64)
     sio.write_vel(str(tmp_path / "vel.000000"), u0)
-    sio.write_header(str(tmp_path / "header.000000"), CONFIG1, time=0.0, dt=0.01, nt=0)
-    r = run(tmp_path, **CONFIG1)
+    sio.write_header(str(tmp_path / "header.000000"), TG64, time=0.0, dt=0.01, nt=0)
+    r = run(tmp_path, **TG64)
     assert r.returncode == 0, r.stdout + r.stderr
     o = oracle_mod.Oracle(oracle_mod.make_cfg(64, vis=0.0025, dt0=0.01))
     u = sio.read_vel(str(tmp_path / "vel.000020"), o.nzm, o.nym, o.nxm, o.lu)

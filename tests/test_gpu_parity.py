@@ -124,7 +124,7 @@ def test_rhs_full(oracle_mod, strat):
 
 
 def test_taylor_green_64_steps(oracle_mod):
-    """config 1 (64^3 TG, nu = 1/400, dt = 0.01): N steps, FP64 parity 1e-12"""
+    """64^3 Taylor-Green, nu = 1/400, dt = 0.01: N steps, FP64 parity 1e-12"""
     nsteps = 20
     o, s = pair(oracle_mod, 64, vis=0.0025, dt0=0.01)
     u0 = ic.taylor_green(64)
@@ -138,7 +138,7 @@ def test_taylor_green_64_steps(oracle_mod):
 
 
 def test_stratified_cfl_steps(oracle_mod):
-    """config-2 physics at 32^3: N = 4 (N^2 = grav*lapse/To = 16), CFL dt"""
+    """stratified Boussinesq at 32^3: N = 4 (N^2 = grav*lapse/To = 16), CFL dt"""
     nsteps = 10
     o, s = pair(oracle_mod, 32, i_strat=1, grav=16.0, lapse0=1.0, vis=5e-3, i_cfl=1, cfl0=0.5)
     u0 = ic.random_solenoidal(32, k0=4, seed=20261203, i_strat=1)
@@ -149,6 +149,24 @@ def test_stratified_cfl_steps(oracle_mod):
     assert rel_l2(got, ref) < TOL64
     assert abs(d["dt"] - dt) < 1e-12 * dt
     assert abs(d["time"] - t) < 1e-12
+
+
+@pytest.mark.parametrize("prec,n,tol", [(S.PREC_FP64, 128, TOL64), (S.PREC_SPLIT, 64, 1e-5)],
+                         ids=["fp64-128", "split-64"])
+def test_double_staged_x_rows(oracle_mod, prec, n, tol):
+    """grids whose x rows take the double-staged inverse phase (dealiased,
+    whole-wavefront columns: FP64 from 128, FP32 from 64): full stratified
+    steps against the oracle"""
+    nsteps = 2
+    o, s = pair(oracle_mod, n, i_strat=1, grav=16.0, lapse0=1.0, vis=5e-3, i_cfl=1, cfl0=0.5, prec=prec)
+    u0 = ic.random_solenoidal(n, k0=4, seed=7, i_strat=1)
+    u0[..., 3] = 0.1 * ic.random_solenoidal(n, k0=3, seed=8)[..., 0]
+    s.set_state(u0)
+    d = s.step(nsteps)
+    ref, t, dt, st = o.step(u0, nsteps)
+    err = rel_l2(s.get_state(), ref)
+    assert err < tol, err
+    np.testing.assert_allclose(d["energy"], st[12], rtol=10 * tol)
 
 
 def test_split_precision_bounded(oracle_mod):
