@@ -13,6 +13,9 @@
 // The unfused k_xb / k_xf / k_zf serve the stage entry points and the LES path.
 #include "internal.hpp"
 
+#include <map>
+#include <mutex>
+
 namespace samhip {
 
 #define SAM_SWITCH_N(n, ...)                                     \
@@ -31,6 +34,32 @@ static inline int launched()
 {
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? SAM_OK : SAM_ERR_HIP;
+}
+
+// Grid of a persistent kernel: the blocks the device holds at once (capped by
+// the number of work items).  Occupancy per kernel and device is computed
+// once and cached.
+static unsigned persistent_grid(const void *kernel, int threads, long long nwork)
+{
+    static std::mutex mu;
+    static std::map<std::pair<const void *, int>, long long> resident;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+    long long nres = 1;
+    {
+        std::lock_guard<std::mutex> g(mu);
+        const auto key = std::make_pair(kernel, dev);
+        auto it = resident.find(key);
+        if (it == resident.end()) {
+            int occ = 0, ncu = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, threads, 0) != hipSuccess || occ < 1) occ = 1;
+            if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
+                ncu = 1;
+            it = resident.emplace(key, (long long)occ * ncu).first;
+        }
+        nres = it->second;
+    }
+    return (unsigned)(nwork < nres ? nwork : nres);
 }
 
 // columns whose strided side is read/written in 128-byte rows
@@ -306,6 +335,13 @@ __device__ __forceinline__ void xrow_issue_stage(cpx<T> *stg, const cpx<T> *__re
 
 __device__ __forceinline__ void wait_vmem() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
+// Workgroup barrier for LDS data only: this wave's LDS accesses complete,
+// then s_barrier.  __syncthreads() also waits for every outstanding global
+// access (s_waitcnt vmcnt(0)), which would drain the staged-field DMA and the
+// product stores at each barrier.  The asm memory clobber keeps the compiler
+// from moving memory accesses across it.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
 // staged raw tile -> packed two-row spectra in the FFT tile
 template <typename T, int N, int CP, int NT>
 __device__ __forceinline__ void xrow_unpack_stage(cpx<T> *lds, const cpx<T> *stg, int kxm)
@@ -357,9 +393,9 @@ __device__ __forceinline__ void xrow_phase_a(cpx<T> *__restrict__ lds, cpx<T> *_
 #pragma unroll
     for (int f = 0; f < LU; f++) {
         wait_vmem();
-        __syncthreads();   // staged tile complete; every wave is done with the FFT tile
+        lds_barrier();   // staged tile complete; every wave is done with the FFT tile
         xrow_unpack_stage<T, N, CP, NT>(lds, stg, kxm);
-        __syncthreads();   // FFT tile ready; staging buffer free
+        lds_barrier();   // FFT tile ready; staging buffer free
         if (f + 1 < LU) xrow_issue_stage<T, CP, NT>(stg, Yz + (f + 1) * fstride, ny, y0, kxm);
         TL::template run_from_lds<+1>(lds, tw, c, j0, uf[f]);
     }
@@ -429,21 +465,44 @@ __device__ __forceinline__ void xrow_pass0_from_stage(const cpx<T> *stg, int c, 
             Yv = pr.b;
         }
     };
+    // kx_max = N/3 on this path: slots whose whole element range lies in one
+    // band are resolved at compile time (the zero band then drops out of
+    // the first butterflies)
+    constexpr int KXM = xrow_srows<N>() - 1, TPC = N / 8;
 #pragma unroll
     for (int s = 0; s < 8; s++) {
-        const int p = j0 + (N / 8) * s;
+        const int p = j0 + TPC * s;
+        const bool all_low = TPC * s + TPC - 1 <= KXM, all_zero = TPC * s > KXM && TPC * s + TPC - 1 < N - KXM;
+        const bool all_high = TPC * s >= N - KXM;
         cpx<T> out = {(T)0, (T)0};
-        if (p <= kxm) {
+        if (all_zero) {
+        } else if (all_low || (!all_high && p <= kxm)) {
             cpx<T> X, Yv;
             rowpair(p, X, Yv);
-            if (p == 0) { X.y = (T)0; Yv.y = (T)0; }
+            if (s == 0 && p == 0) { X.y = (T)0; Yv.y = (T)0; }
             out = {X.x - Yv.y, X.y + Yv.x};
-        } else if (p >= N - kxm) {
+        } else if (all_high || p >= N - kxm) {
             cpx<T> X, Yv;
             rowpair(N - p, X, Yv);
             out = {X.x + Yv.y, Yv.x - X.y};
         }
         v[s] = out;
+    }
+}
+
+// final forward outputs to LDS in natural order, leaving out the slots whose
+// positions all fall in the truncated band kx_max < k < N - kx_max (kx_max =
+// N/3): nothing reads them, so their butterfly outputs are never formed
+template <typename T, int N>
+__device__ __forceinline__ void xrow_store_kept(cpx<T> *lds, int c, int j0, const cpx<T> *v)
+{
+    using TL = Tile<T, N, 8>;
+    constexpr int KXM = xrow_srows<N>() - 1;
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+        const int lo = TL::kout(s) * TL::TPC, hi = lo + TL::TPC - 1;
+        if (lo > KXM && hi < N - KXM) continue;
+        lds[lds_idx<T, N>(c, j0 + TL::kout(s) * TL::TPC)] = v[s];
     }
 }
 
@@ -462,13 +521,143 @@ __device__ __forceinline__ void xrow_phase_a_ds(cpx<T> *lds, cpx<T> *stg, const 
     for (int f = 0; f < LU; f++) {
         if (f + 1 < LU) wait_vmem_le(mine);   // field f landed; field f + 1 may still be in flight
         else wait_vmem();
-        __syncthreads();                       // ... for every wave's share of the rows
+        lds_barrier();                       // ... for every wave's share of the rows
         xrow_pass0_from_stage<T, N, CP>(st[f & 1], c, j0, kxm, uf[f]);
         if (f + 2 < LU) {
-            __syncthreads();                   // every wave has read field f's buffer
+            lds_barrier();                   // every wave has read field f's buffer
             xrow_issue_stage_ds<T, N, CP, NT>(st[f & 1], Yz + (f + 2) * fstride, ny, y0);
         }
         TL::template run_regs<+1>(lds, tw, c, j0, uf[f]);
+    }
+}
+
+// vel_max.f partials of the block's rows: sum, sum of squares, max |u| per field
+template <typename T, int NT, int LU>
+__device__ __forceinline__ void xrow_stats(const cpx<T> (&uf)[LU][8], double (*red)[3 * LU], double *__restrict__ out)
+{
+    constexpr int NW = (NT + 63) / 64;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int f = 0; f < LU; f++) {
+        double s = 0, s2 = 0, mx = 0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const double a = (double)uf[f][k].x, b = (double)uf[f][k].y;
+            s += a + b;
+            s2 += a * a + b * b;
+            mx = fmax(mx, fmax(fabs(a), fabs(b)));
+        }
+        s = wave_sum(s);
+        s2 = wave_sum(s2);
+        mx = wave_max(mx);
+        if (lane == 0) {
+            red[w][f] = s;
+            red[w][LU + f] = s2;
+            red[w][2 * LU + f] = mx;
+        }
+    }
+    lds_barrier();
+    if (threadIdx.x < 3 * LU) {
+        const bool is_max = threadIdx.x >= 2 * LU;
+        double t = red[0][threadIdx.x];
+        for (int i = 1; i < NW; i++) t = is_max ? fmax(t, red[i][threadIdx.x]) : t + red[i][threadIdx.x];
+        out[threadIdx.x] = t;
+    }
+}
+
+// The LR products of the transformed fields, forward x transforms and the
+// R2C stores.  DBUF: products alternate between the FFT tile and the (idle,
+// at least as large) staging buffer, so the store of product q-1 and the
+// transform of product q overlap and one barrier per product suffices.
+// KEPT: kx_max = N/3, outputs in the truncated band are not formed.
+template <typename T, int N, int CP, int LU, int LR, bool DBUF, bool KEPT>
+__device__ __forceinline__ void xrow_products(cpx<T> *lds, cpx<T> *stg, const cpx<T> *tw, const cpx<T> (&uf)[LU][8],
+                                              cpx<T> *__restrict__ Yz, long long fstride, const Dims &d, int y0, int c,
+                                              int j0)
+{
+    using TL = Tile<T, N, 8>;
+    constexpr int NT = CP * N / 8;
+    const T h = (T)(0.5 / ((double)d.nx * (double)d.ny));
+#pragma unroll
+    for (int q = 0; q < LR; q++) {
+        cpx<T> po[8], pi[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const cpx<T> a = uf[prod_i(q)][k], b = uf[prod_j(q)][k];
+            po[k] = {a.x * b.x, a.y * b.y};
+        }
+        TL::out_to_in(po, pi);
+        cpx<T> *tile = lds;
+        if constexpr (DBUF) {
+            if (q & 1) tile = stg;
+        } else {
+            lds_barrier();
+        }
+        TL::template run_regs<-1>(tile, tw, c, j0, pi);
+        if constexpr (KEPT) xrow_store_kept<T, N>(tile, c, j0, pi);
+        else TL::store_natural(tile, c, j0, pi);
+        lds_barrier();
+        xrow_store_pairs<T, N, CP, NT>(tile, Yz + q * fstride, d.ny, y0, d.kxm, h);
+    }
+}
+
+// Persistent double-staged x-row kernel (dealiased rows, whole-wave columns):
+// a block walks row groups b = blockIdx.x + k*gridDim.x.  The staging
+// buffers are free during the product phase (one tile, two barriers per
+// product), so the next group's first two fields stream in behind the last
+// two transforms and the product stores of this one.
+template <typename T, int N, int CP, int LU, int LR>
+__global__ __launch_bounds__(CP * N / 8, 2) void k_xrow_p(T *__restrict__ Yr, Dims d, const cpx<T> *__restrict__ twg,
+                                                         int want_stats, double *__restrict__ partials, int nblk)
+{
+    using TL = Tile<T, N, 8>;
+    constexpr int NT = CP * N / 8, NW = NT / 64;
+    constexpr int TILE = CP * N, SR = xrow_srows<N>();
+    __shared__ cpx<T> smem[TILE + 2 * SR * 2 * CP];
+    __shared__ double red[NW][3 * LU];
+    __shared__ cpx<T> tw[N / 2];
+    for (int e = threadIdx.x; e < N / 2; e += NT) tw[e] = twg[e];
+    cpx<T> *const lds = smem;
+    auto stage = [&](int i) { return smem + TILE + (i & 1) * (SR * 2 * CP); };
+    const int nyt = d.ny / (2 * CP);
+    const int c = threadIdx.x / TL::TPC, j0 = threadIdx.x % TL::TPC;
+    const int mine = xrow_stage_count<N, NT>(threadIdx.x >> 6);
+    cpx<T> *Y = reinterpret_cast<cpx<T> *>(Yr);
+    const long long fstride = d.yfield;
+    auto zoff = [&](int b) { return (long long)(b / nyt) * d.nxm * d.ny; };
+    auto yof = [&](int b) { return (b % nyt) * 2 * CP; };
+
+    int b = blockIdx.x;
+    if (b < nblk) {
+        xrow_issue_stage_ds<T, N, CP, NT>(stage(0), Y + zoff(b), d.ny, yof(b));
+        if (LU > 1) xrow_issue_stage_ds<T, N, CP, NT>(stage(1), Y + zoff(b) + fstride, d.ny, yof(b));
+    }
+    int par = 0;   // field f of the current group sits in stage(f + par)
+    for (; b < nblk; b += gridDim.x) {
+        const int bn = b + gridDim.x;
+        const bool has_next = bn < nblk;
+        const int y0 = yof(b);
+        cpx<T> *const Yz = Y + zoff(b);
+        cpx<T> uf[LU][8];
+#pragma unroll
+        for (int f = 0; f < LU; f++) {
+            // in flight, oldest first: field f, then the item issued after it
+            if (f + 1 < LU || has_next) wait_vmem_le(mine);
+            else wait_vmem();
+            lds_barrier();
+            cpx<T> *const sf = stage(f + par);
+            xrow_pass0_from_stage<T, N, CP>(sf, c, j0, d.kxm, uf[f]);
+            const int g = f + 2;   // next item into the buffer just read
+            if (g < LU || (has_next && g < LU + 2)) {
+                lds_barrier();
+                if (g < LU) xrow_issue_stage_ds<T, N, CP, NT>(sf, Yz + g * fstride, d.ny, y0);
+                else xrow_issue_stage_ds<T, N, CP, NT>(sf, Y + zoff(bn) + (g - LU) * fstride, d.ny, yof(bn));
+            }
+            TL::template run_regs<+1>(lds, tw, c, j0, uf[f]);
+        }
+        if (want_stats) xrow_stats<T, NT, LU>(uf, red, partials + (long long)b * 3 * LU);
+        xrow_products<T, N, CP, LU, LR, false, true>(lds, nullptr, tw, uf, Yz, fstride, d, y0, c, j0);
+        par ^= LU & 1;
     }
 }
 
@@ -515,67 +704,15 @@ __global__ __launch_bounds__(CP * N / 8, xrow_waves<T>()) void k_xrow(T *__restr
         // sub-wavefront groups (short rows): direct cooperative loads
 #pragma unroll
         for (int f = 0; f < LU; f++) {
-            if (f > 0) __syncthreads();
+            if (f > 0) lds_barrier();
             xrow_load_pairs<T, N, CP, NT>(lds, Y + f * fstride + zoff, d.ny, y0, d.kxm);
-            __syncthreads();
+            lds_barrier();
             TL::template run_from_lds<+1>(lds, tw, c, j0, uf[f]);
         }
     }
 
-    if (want_stats) {
-        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-#pragma unroll
-        for (int f = 0; f < LU; f++) {
-            double s = 0, s2 = 0, mx = 0;
-#pragma unroll
-            for (int k = 0; k < 8; k++) {
-                const double a = (double)uf[f][k].x, b = (double)uf[f][k].y;
-                s += a + b;
-                s2 += a * a + b * b;
-                mx = fmax(mx, fmax(fabs(a), fabs(b)));
-            }
-            s = wave_sum(s);
-            s2 = wave_sum(s2);
-            mx = wave_max(mx);
-            if (lane == 0) {
-                red[w][f] = s;
-                red[w][LU + f] = s2;
-                red[w][2 * LU + f] = mx;
-            }
-        }
-        __syncthreads();
-        if (threadIdx.x < 3 * LU) {
-            const bool is_max = threadIdx.x >= 2 * LU;
-            double t = red[0][threadIdx.x];
-            for (int i = 1; i < NW; i++) t = is_max ? fmax(t, red[i][threadIdx.x]) : t + red[i][threadIdx.x];
-            partials[(long long)blockIdx.x * 3 * LU + threadIdx.x] = t;
-        }
-    }
-
-    const T h = (T)(0.5 / ((double)d.nx * (double)d.ny));
-#pragma unroll
-    for (int q = 0; q < LR; q++) {
-        cpx<T> po[8], pi[8];
-#pragma unroll
-        for (int k = 0; k < 8; k++) {
-            const cpx<T> a = uf[prod_i(q)][k], b = uf[prod_j(q)][k];
-            po[k] = {a.x * b.x, a.y * b.y};
-        }
-        TL::out_to_in(po, pi);
-        // staged path: products alternate between the FFT tile and the (idle,
-        // at least as large) staging buffer, so the store of product q-1 and
-        // the transform of product q overlap and one barrier per product suffices
-        cpx<T> *tile = lds;
-        if constexpr (NT >= 64) {
-            if (q & 1) tile = stg;
-        } else {
-            __syncthreads();
-        }
-        TL::template run_regs<-1>(tile, tw, c, j0, pi);
-        TL::store_natural(tile, c, j0, pi);
-        __syncthreads();
-        xrow_store_pairs<T, N, CP, NT>(tile, Y + q * fstride + zoff, d.ny, y0, d.kxm, h);
-    }
+    if (want_stats) xrow_stats<T, NT, LU>(uf, red, partials + (long long)blockIdx.x * 3 * LU);
+    xrow_products<T, N, CP, LU, LR, NT >= 64, DS>(lds, stg, tw, uf, Y + zoff, fstride, d, y0, c, j0);
 }
 
 template <typename T> struct XCP { static constexpr int CP = Row128<T>::C / 2; };
@@ -596,21 +733,35 @@ int launch_xrow(const Dims &d, T *Y, int want_stats, double *partials, DevDiag *
                                    (CP * N + 2 * xrow_srows<N>() * 2 * CP + N / 2) * sizeof(cpx<T>) + 512 <= 160 * 1024;
             const bool ds = ds_ok && d.kxm + 1 == xrow_srows<N>();
             const cpx<T> *tw = (const cpx<T> *)twx;
-            if (d.lu == 4) {
-                if (ds)
-                    hipLaunchKernelGGL((k_xrow<T, N, CP, 4, 9, ds_ok>), dim3(nb), dim3(CP * N / 8), 0, st, Y, d, tw,
-                                       want_stats, partials, diag);
-                else
-                    hipLaunchKernelGGL((k_xrow<T, N, CP, 4, 9, false>), dim3(nb), dim3(CP * N / 8), 0, st, Y, d, tw,
-                                       want_stats, partials, diag);
-            } else {
-                if (ds)
-                    hipLaunchKernelGGL((k_xrow<T, N, CP, 3, 6, ds_ok>), dim3(nb), dim3(CP * N / 8), 0, st, Y, d, tw,
-                                       want_stats, partials, diag);
-                else
-                    hipLaunchKernelGGL((k_xrow<T, N, CP, 3, 6, false>), dim3(nb), dim3(CP * N / 8), 0, st, Y, d, tw,
-                                       want_stats, partials, diag);
+            if constexpr (ds_ok) {
+                if constexpr (sizeof(T) == 4) {
+                    if (ds) {
+                        // FP32: persistent, row groups cycle through the resident blocks
+                        auto kern = d.lu == 4 ? k_xrow_p<T, N, CP, 4, 9> : k_xrow_p<T, N, CP, 3, 6>;
+                        const unsigned ng = persistent_grid((const void *)kern, CP * N / 8, nb);
+                        hipLaunchKernelGGL(kern, dim3(ng), dim3(CP * N / 8), 0, st, Y, d, tw, want_stats, partials,
+                                           (int)nb);
+                        return launched();
+                    }
+                }
+                if (ds) {
+                    // FP64: the persistent loop would push the kernel past 256 VGPRs (spills,
+                    // whose scratch traffic drains the staged DMA); one group per block
+                    if (d.lu == 4)
+                        hipLaunchKernelGGL((k_xrow<T, N, CP, 4, 9, true>), dim3(nb), dim3(CP * N / 8), 0, st, Y, d, tw,
+                                           want_stats, partials, diag);
+                    else
+                        hipLaunchKernelGGL((k_xrow<T, N, CP, 3, 6, true>), dim3(nb), dim3(CP * N / 8), 0, st, Y, d, tw,
+                                           want_stats, partials, diag);
+                    return launched();
+                }
             }
+            if (d.lu == 4)
+                hipLaunchKernelGGL((k_xrow<T, N, CP, 4, 9, false>), dim3(nb), dim3(CP * N / 8), 0, st, Y, d, tw,
+                                   want_stats, partials, diag);
+            else
+                hipLaunchKernelGGL((k_xrow<T, N, CP, 3, 6, false>), dim3(nb), dim3(CP * N / 8), 0, st, Y, d, tw,
+                                   want_stats, partials, diag);
         } else {
             return SAM_ERR_UNSUPPORTED;
         }
