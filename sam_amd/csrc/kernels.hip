@@ -1096,20 +1096,32 @@ int launch_stats_phys(const Dims &d, const void *P, int is_double, int nvar, dou
 
 // vel_max.f:28-49 from per-block partials: rows of 3*Lu doubles (sum, sum of
 // squares, max |u| of each field)
-__global__ __launch_bounds__(256) void k_finalize(const double *__restrict__ partials, int nblocks, int lu,
-                                                   DevDiag *__restrict__ diag, StepConsts k, Dims d, int update_dt)
+constexpr int FIN_THREADS = 1024;
+__global__ __launch_bounds__(FIN_THREADS) void k_finalize(const double *__restrict__ partials, int nblocks, int lu,
+                                                           DevDiag *__restrict__ diag, StepConsts k, Dims d,
+                                                           int update_dt)
 {
-    __shared__ double red[4][12];
+    constexpr int NW = FIN_THREADS / 64;
+    __shared__ double red[NW][12];
     __shared__ double tot[12];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, width = 3 * lu;
     double acc[12];
 #pragma unroll
     for (int f = 0; f < 12; f++) acc[f] = 0.0;
-    for (int b = threadIdx.x; b < nblocks; b += 256) {
-        const double *row = partials + (long long)b * width;
+    // two rows per iteration: their loads are in flight together
+    for (int b = threadIdx.x; b < nblocks; b += 2 * FIN_THREADS) {
+        const double *r0 = partials + (long long)b * width;
+        const double *r1 = r0 + (long long)FIN_THREADS * width;
+        const bool two = b + FIN_THREADS < nblocks;
+        double v0[12], v1[12];
+#pragma unroll
+        for (int f = 0; f < 12; f++) {
+            v0[f] = f < width ? r0[f] : 0.0;
+            v1[f] = (f < width && two) ? r1[f] : 0.0;
+        }
 #pragma unroll
         for (int f = 0; f < 12; f++)
-            if (f < width) acc[f] = f < 2 * lu ? acc[f] + row[f] : fmax(acc[f], row[f]);
+            if (f < width) acc[f] = f < 2 * lu ? acc[f] + v0[f] + v1[f] : fmax(acc[f], fmax(v0[f], v1[f]));
     }
 #pragma unroll
     for (int f = 0; f < 12; f++) {
@@ -1122,7 +1134,7 @@ __global__ __launch_bounds__(256) void k_finalize(const double *__restrict__ par
     if (threadIdx.x < width) {
         const bool is_max = threadIdx.x >= 2 * lu;
         double t = red[0][threadIdx.x];
-        for (int i = 1; i < 4; i++) t = is_max ? fmax(t, red[i][threadIdx.x]) : t + red[i][threadIdx.x];
+        for (int i = 1; i < NW; i++) t = is_max ? fmax(t, red[i][threadIdx.x]) : t + red[i][threadIdx.x];
         tot[threadIdx.x] = t;
     }
     __syncthreads();
@@ -1157,7 +1169,7 @@ __global__ __launch_bounds__(256) void k_finalize(const double *__restrict__ par
 int launch_finalize(const Dims &d, const StepConsts &k, const double *partials, int nblocks, DevDiag *diag,
                     int update_dt, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, partials, nblocks, d.lu, diag, k, d, update_dt);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(FIN_THREADS), 0, st, partials, nblocks, d.lu, diag, k, d, update_dt);
     return launched();
 }
 
@@ -1298,19 +1310,33 @@ __global__ __launch_bounds__(256) void k_spectra(const double *__restrict__ Ur, 
     for (int i = threadIdx.x; i < 5 * lm + 1; i += 256) partials[(long long)blockIdx.x * (5 * lm + 1) + i] = acc[i];
 }
 
+// column sums of the per-block spectra partials: one thread per entry, four
+// independent accumulators (fixed order, so the result is deterministic)
 __global__ __launch_bounds__(256) void k_spectra_reduce(const double *__restrict__ partials, int nblocks, int lm,
-                                                         double *__restrict__ out, DevDiag *__restrict__ diag)
+                                                         double *__restrict__ out)
 {
     const int w = 5 * lm + 1;
-    for (int i = threadIdx.x; i < w; i += 256) {
-        double s = 0.0;
-        for (int b = 0; b < nblocks; b++) s += partials[(long long)b * w + i];
-        out[i < 5 * lm ? i : 5 * lm + 1] = s;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= w) return;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int b = 0;
+    for (; b + 4 <= nblocks; b += 4) {
+        s0 += partials[(long long)b * w + i];
+        s1 += partials[(long long)(b + 1) * w + i];
+        s2 += partials[(long long)(b + 2) * w + i];
+        s3 += partials[(long long)(b + 3) * w + i];
     }
-    __syncthreads();
+    for (; b < nblocks; b++) s0 += partials[(long long)b * w + i];
+    out[i < 5 * lm ? i : 5 * lm + 1] = (s0 + s1) + (s2 + s3);
+}
+
+// total energy, rms and max divergence (spectra.f:80-91, div_check)
+__global__ void k_spectra_final(int lm, double *__restrict__ out, DevDiag *__restrict__ diag)
+{
+    double se = 0.0;
+    for (int i = threadIdx.x; i < lm; i += 64) se += out[i];
+    se = wave_sum(se);
     if (threadIdx.x == 0) {
-        double se = 0.0;
-        for (int i = 0; i < lm; i++) se += out[i];
         out[5 * lm] = 0.5 * se;
         out[5 * lm + 1] = sqrt(out[5 * lm + 1]);
         out[5 * lm + 2] = sqrt(__longlong_as_double((long long)diag->divmax_bits));
@@ -1324,7 +1350,8 @@ int launch_spectra(const Dims &d, double vis, const double *U, const double *R, 
     const int nb = d.nxm * SPEC_YCH;
     const size_t sh = sizeof(double) * (5 * d.lm + 1);
     hipLaunchKernelGGL(k_spectra, dim3(nb), dim3(256), sh, st, U, R, d, vis, partials, diag);
-    hipLaunchKernelGGL(k_spectra_reduce, dim3(1), dim3(256), 0, st, partials, nb, d.lm, out, diag);
+    hipLaunchKernelGGL(k_spectra_reduce, dim3((5 * d.lm + 1 + 255) / 256), dim3(256), 0, st, partials, nb, d.lm, out);
+    hipLaunchKernelGGL(k_spectra_final, dim3(1), dim3(64), 0, st, d.lm, out, diag);
     return launched();
 }
 
