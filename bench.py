@@ -62,7 +62,7 @@ def make_params(S, n, precision, sgs=0):
                          i_cfl=1, cfl0=0.5, dt0=0.01, precision=precision, sgs=sgs)
 
 
-def run_gpu(n, precision, steps, warmup, u0, S):
+def run_gpu(n, precision, steps, warmup, u0, S, keep_state=True):
     sam = S.Sam(make_params(S, n, precision))
     sam.set_state(u0)
     sam.step(warmup)
@@ -76,7 +76,7 @@ def run_gpu(n, precision, steps, warmup, u0, S):
     prof = sam.prof_read()
     sam.prof_enable(False)
     spec = sam.spectra()
-    state = sam.get_state()
+    state = sam.get_state() if keep_state else None
     sam.close()
     return dict(wall=t1 - t0, diag=diag, prof=prof, spec=spec, state=state)
 
@@ -165,6 +165,28 @@ def les_comparison(S, ic, dns_u, n_dns, steps, warmup):
     }
 
 
+def big_split(S, ic, u_coarse, n_big, steps, warmup):
+    """One or more RK3 steps of the FP32-split DNS at n_big^3 (the state of
+    the bench run, spectrally padded), timed like the main runs."""
+    u = ic.spectral_pad(u_coarse, n_big)
+    r = run_gpu(n_big, S.PREC_SPLIT, steps, warmup, u, S, keep_state=False)
+    del u
+    sb = step_bytes(n_big, 8)
+    _, pk = dominant_roofline(n_big, 8, r["prof"], steps)
+    return {
+        "workload": f"stratified Boussinesq DNS {n_big}^3, FP32-split (FP32 transforms/products, FP64 state)",
+        "ms_per_step": round(r["wall"] / steps * 1e3, 3),
+        "value": round(n_big ** 3 * steps / r["wall"] / 1e6, 2),
+        "unit": "Mpoint-steps/s",
+        "step_hbm_frac": round(sb / (r["wall"] / steps) / HBM_PEAK, 4),
+        "bytes_per_step": sb,
+        "kernels": pk,
+        "energy": float(r["spec"]["energy_w"]),
+        "steps": steps,
+        "warmup": warmup,
+    }
+
+
 def cpu_baseline(n_cpu, steps_cpu, u0_small):
     from oracle import oracle as O
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
@@ -188,6 +210,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-split", action="store_true")
     ap.add_argument("--no-les", action="store_true")
+    ap.add_argument("--no-big", action="store_true", help="skip the 1024^3 FP32-split step")
+    ap.add_argument("--big-n", type=int, default=1024)
     ap.add_argument("--cpu-n", type=int, default=0, help="CPU baseline grid (default: same n if RAM allows)")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -273,6 +297,11 @@ def main():
         out["les_128"] = les_comparison(S, ic, r64["state"], n, 20, 3)
         log(f"[bench] LES dyn {out['les_128']['dynamic_ms_per_step']} ms, "
             f"learned {out['les_128']['learned_ms_per_step']} ms")
+
+    if not args.no_big and args.big_n > n:
+        log(f"[bench] FP32-split {args.big_n}^3")
+        out[f"fp32_split_{args.big_n}"] = big_split(S, ic, r64["state"], args.big_n, 2, 1)
+        log(f"[bench] split {args.big_n}^3: {out[f'fp32_split_{args.big_n}']['ms_per_step']} ms/step")
 
     if not args.no_cpu:
         n_cpu = args.cpu_n or n

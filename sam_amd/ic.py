@@ -104,3 +104,20 @@ def random_solenoidal(nx, ny=None, nz=None, *, k0=4.0, urms=1.0, seed=20261203, 
     for c in range(3):
         u[:, :, :, c] = a[c]
     return u
+
+
+def spectral_pad(u, n_to, n_dealias=1):
+    """Place the modes of u (SAM's stored form on a coarser grid) into the
+    stored form of an n_to^3 grid, zero elsewhere -- the refinement direction
+    of remesh.f:97-178."""
+    nzm_f, nym_f, nxm_f, lu = u.shape
+    d = grid_dims(n_to, n_dealias=n_dealias, i_strat=1 if lu == 4 else 0)
+    kyf, kzf = (nym_f - 1) // 2, (nzm_f - 1) // 2
+    if d["kym"] < kyf or d["kzm"] < kzf or d["nxm"] < nxm_f:
+        raise ValueError("target grid is coarser than the source")
+    out = np.zeros((d["nzm"], d["nym"], d["nxm"], lu), np.complex128, order="F")
+    jy = np.array([j if j <= kyf else j - nym_f for j in range(nym_f)]) % d["nym"]
+    jz = np.array([k if k <= kzf else k - nzm_f for k in range(nzm_f)]) % d["nzm"]
+    out[np.ix_(jz, jy, np.arange(nxm_f), np.arange(lu))] = u
+    return out
+

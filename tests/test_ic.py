@@ -33,3 +33,14 @@ def test_taylor_green_physical():
     assert np.abs(phys[..., 0] - np.sin(X) * np.cos(Y) * np.cos(Z)).max() < 1e-14
     assert np.abs(phys[..., 1] + np.cos(X) * np.sin(Y) * np.cos(Z)).max() < 1e-14
     assert np.abs(phys[..., 2]).max() < 1e-14
+
+
+def test_spectral_pad_round_trip():
+    """padding to a finer grid keeps every mode (energy) and truncation undoes it"""
+    from sam_amd import les as L
+    u = ic.random_solenoidal(32, k0=4, seed=3, i_strat=1)
+    v = ic.spectral_pad(u, 64)
+    d = ic.grid_dims(64, i_strat=1)
+    assert v.shape == (d["nzm"], d["nym"], d["nxm"], 4)
+    assert np.isclose(np.linalg.norm(v), np.linalg.norm(u), rtol=1e-14, atol=0)
+    np.testing.assert_array_equal(L.spectral_truncate(v, 32), u)
