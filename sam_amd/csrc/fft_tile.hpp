@@ -112,13 +112,15 @@ template <int S, typename T> __device__ __forceinline__ void dft8(cpx<T> *v)
 // LDS ordering between passes.  When every column of the tile belongs to a
 // single wavefront (threads per column divide 64), the exchange only has to
 // order this wave's own LDS accesses: wait for them and stop the compiler
-// from moving LDS accesses across; otherwise a workgroup barrier.
+// from moving LDS accesses across.  Otherwise a workgroup barrier for LDS
+// data only (not __syncthreads, which also waits for every outstanding
+// global access and so would drain prefetched loads and pending stores).
 template <bool WAVE> __device__ __forceinline__ void tile_sync()
 {
     if constexpr (WAVE) {
         asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
     } else {
-        __syncthreads();
+        asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
     }
 }
 

@@ -320,16 +320,18 @@ __device__ __forceinline__ void lds_dma16(const void *gsrc, const void *lds_dst)
 template <typename T, int CP, int NT>
 __device__ __forceinline__ void xrow_issue_stage(cpx<T> *stg, const cpx<T> *__restrict__ Yf, int ny, int y0, int kxm)
 {
-    static_assert(2 * CP * (int)sizeof(cpx<T>) == 128, "one k-row is 128 bytes");
+    // k-rows of 2*CP complex values: whole 16-byte chunks, 64 lanes = RPI rows
+    constexpr int ROWB = 2 * CP * (int)sizeof(cpx<T>), CPR = ROWB / 16, RPI = 64 / CPR;
+    static_assert(ROWB % 16 == 0 && 64 % CPR == 0, "k-rows of whole 16-byte chunks");
     static_assert(NT >= 64 && NT % 64 == 0, "whole wavefronts issue the staged loads");
     constexpr int NW = NT / 64;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int ninstr = (kxm + 8) / 8;
+    const int ninstr = (kxm + RPI) / RPI;
     for (int i = w; i < ninstr; i += NW) {
-        int k = i * 8 + (lane >> 3);
+        int k = i * RPI + lane / CPR;
         if (k > kxm) k = kxm;      // rows past kx_max: re-read a valid row, ignored
-        const char *src = reinterpret_cast<const char *>(Yf + (long long)k * ny + y0) + (lane & 7) * 16;
-        lds_dma16(src, stg + i * 16 * CP);
+        const char *src = reinterpret_cast<const char *>(Yf + (long long)k * ny + y0) + (lane % CPR) * 16;
+        lds_dma16(src, stg + i * RPI * 2 * CP);
     }
 }
 
@@ -448,12 +450,13 @@ __device__ __forceinline__ void wait_vmem_le(int n)
     }
 }
 
-// pass-0 inputs of column pair c (element j0 + s N/8 in slot s) straight
-// from the staged rows: packed two-row spectrum, zero band in the middle
-template <typename T, int N, int CP>
+// pass-0 inputs of column pair c (slot s holds element j0 + kin(0, s) N/EPT)
+// straight from the staged rows: packed two-row spectrum, zero band in the
+// middle
+template <typename T, int N, int CP, int EPT = 8>
 __device__ __forceinline__ void xrow_pass0_from_stage(const cpx<T> *stg, int c, int j0, int kxm, cpx<T> *v)
 {
-    static_assert(Tile<T, N, 8>::kin(0, 3) == 3, "pass-0 slot s holds element j0 + s N/8");
+    using TL = Tile<T, N, EPT>;
     auto rowpair = [&](int k, cpx<T> &X, cpx<T> &Yv) {
         const cpx<T> *row = stg + k * 2 * CP;
         if constexpr (sizeof(T) == 8) {
@@ -468,18 +471,18 @@ __device__ __forceinline__ void xrow_pass0_from_stage(const cpx<T> *stg, int c, 
     // kx_max = N/3 on this path: slots whose whole element range lies in one
     // band are resolved at compile time (the zero band then drops out of
     // the first butterflies)
-    constexpr int KXM = xrow_srows<N>() - 1, TPC = N / 8;
+    constexpr int KXM = xrow_srows<N>() - 1, TPC = TL::TPC;
 #pragma unroll
-    for (int s = 0; s < 8; s++) {
-        const int p = j0 + TPC * s;
-        const bool all_low = TPC * s + TPC - 1 <= KXM, all_zero = TPC * s > KXM && TPC * s + TPC - 1 < N - KXM;
-        const bool all_high = TPC * s >= N - KXM;
+    for (int s = 0; s < EPT; s++) {
+        const int lo = TPC * TL::kin(0, s), p = j0 + lo;
+        const bool all_low = lo + TPC - 1 <= KXM, all_zero = lo > KXM && lo + TPC - 1 < N - KXM;
+        const bool all_high = lo >= N - KXM;
         cpx<T> out = {(T)0, (T)0};
         if (all_zero) {
         } else if (all_low || (!all_high && p <= kxm)) {
             cpx<T> X, Yv;
             rowpair(p, X, Yv);
-            if (s == 0 && p == 0) { X.y = (T)0; Yv.y = (T)0; }
+            if (lo == 0 && p == 0) { X.y = (T)0; Yv.y = (T)0; }
             out = {X.x - Yv.y, X.y + Yv.x};
         } else if (all_high || p >= N - kxm) {
             cpx<T> X, Yv;
@@ -493,13 +496,13 @@ __device__ __forceinline__ void xrow_pass0_from_stage(const cpx<T> *stg, int c, 
 // final forward outputs to LDS in natural order, leaving out the slots whose
 // positions all fall in the truncated band kx_max < k < N - kx_max (kx_max =
 // N/3): nothing reads them, so their butterfly outputs are never formed
-template <typename T, int N>
+template <typename T, int N, int EPT = 8>
 __device__ __forceinline__ void xrow_store_kept(cpx<T> *lds, int c, int j0, const cpx<T> *v)
 {
-    using TL = Tile<T, N, 8>;
+    using TL = Tile<T, N, EPT>;
     constexpr int KXM = xrow_srows<N>() - 1;
 #pragma unroll
-    for (int s = 0; s < 8; s++) {
+    for (int s = 0; s < EPT; s++) {
         const int lo = TL::kout(s) * TL::TPC, hi = lo + TL::TPC - 1;
         if (lo > KXM && hi < N - KXM) continue;
         lds[lds_idx<T, N>(c, j0 + TL::kout(s) * TL::TPC)] = v[s];
@@ -532,8 +535,8 @@ __device__ __forceinline__ void xrow_phase_a_ds(cpx<T> *lds, cpx<T> *stg, const 
 }
 
 // vel_max.f partials of the block's rows: sum, sum of squares, max |u| per field
-template <typename T, int NT, int LU>
-__device__ __forceinline__ void xrow_stats(const cpx<T> (&uf)[LU][8], double (*red)[3 * LU], double *__restrict__ out)
+template <typename T, int NT, int LU, int EPT = 8>
+__device__ __forceinline__ void xrow_stats(const cpx<T> (&uf)[LU][EPT], double (*red)[3 * LU], double *__restrict__ out)
 {
     constexpr int NW = (NT + 63) / 64;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -541,7 +544,7 @@ __device__ __forceinline__ void xrow_stats(const cpx<T> (&uf)[LU][8], double (*r
     for (int f = 0; f < LU; f++) {
         double s = 0, s2 = 0, mx = 0;
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
+        for (int k = 0; k < EPT; k++) {
             const double a = (double)uf[f][k].x, b = (double)uf[f][k].y;
             s += a + b;
             s2 += a * a + b * b;
@@ -570,19 +573,19 @@ __device__ __forceinline__ void xrow_stats(const cpx<T> (&uf)[LU][8], double (*r
 // at least as large) staging buffer, so the store of product q-1 and the
 // transform of product q overlap and one barrier per product suffices.
 // KEPT: kx_max = N/3, outputs in the truncated band are not formed.
-template <typename T, int N, int CP, int LU, int LR, bool DBUF, bool KEPT>
-__device__ __forceinline__ void xrow_products(cpx<T> *lds, cpx<T> *stg, const cpx<T> *tw, const cpx<T> (&uf)[LU][8],
+template <typename T, int N, int CP, int LU, int LR, bool DBUF, bool KEPT, int EPT = 8>
+__device__ __forceinline__ void xrow_products(cpx<T> *lds, cpx<T> *stg, const cpx<T> *tw, const cpx<T> (&uf)[LU][EPT],
                                               cpx<T> *__restrict__ Yz, long long fstride, const Dims &d, int y0, int c,
                                               int j0)
 {
-    using TL = Tile<T, N, 8>;
-    constexpr int NT = CP * N / 8;
+    using TL = Tile<T, N, EPT>;
+    constexpr int NT = CP * N / EPT;
     const T h = (T)(0.5 / ((double)d.nx * (double)d.ny));
 #pragma unroll
     for (int q = 0; q < LR; q++) {
-        cpx<T> po[8], pi[8];
+        cpx<T> po[EPT], pi[EPT];
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
+        for (int k = 0; k < EPT; k++) {
             const cpx<T> a = uf[prod_i(q)][k], b = uf[prod_j(q)][k];
             po[k] = {a.x * b.x, a.y * b.y};
         }
@@ -594,7 +597,7 @@ __device__ __forceinline__ void xrow_products(cpx<T> *lds, cpx<T> *stg, const cp
             lds_barrier();
         }
         TL::template run_regs<-1>(tile, tw, c, j0, pi);
-        if constexpr (KEPT) xrow_store_kept<T, N>(tile, c, j0, pi);
+        if constexpr (KEPT) xrow_store_kept<T, N, EPT>(tile, c, j0, pi);
         else TL::store_natural(tile, c, j0, pi);
         lds_barrier();
         xrow_store_pairs<T, N, CP, NT>(tile, Yz + q * fstride, d.ny, y0, d.kxm, h);
@@ -606,12 +609,12 @@ __device__ __forceinline__ void xrow_products(cpx<T> *lds, cpx<T> *stg, const cp
 // buffers are free during the product phase (one tile, two barriers per
 // product), so the next group's first two fields stream in behind the last
 // two transforms and the product stores of this one.
-template <typename T, int N, int CP, int LU, int LR>
-__global__ __launch_bounds__(CP * N / 8, 2) void k_xrow_p(T *__restrict__ Yr, Dims d, const cpx<T> *__restrict__ twg,
-                                                         int want_stats, double *__restrict__ partials, int nblk)
+template <typename T, int N, int CP, int LU, int LR, int EPT>
+__global__ __launch_bounds__(CP * N / EPT, 2) void k_xrow_p(T *__restrict__ Yr, Dims d, const cpx<T> *__restrict__ twg,
+                                                           int want_stats, double *__restrict__ partials, int nblk)
 {
-    using TL = Tile<T, N, 8>;
-    constexpr int NT = CP * N / 8, NW = NT / 64;
+    using TL = Tile<T, N, EPT>;
+    constexpr int NT = CP * N / EPT, NW = NT / 64;
     constexpr int TILE = CP * N, SR = xrow_srows<N>();
     __shared__ cpx<T> smem[TILE + 2 * SR * 2 * CP];
     __shared__ double red[NW][3 * LU];
@@ -638,7 +641,7 @@ __global__ __launch_bounds__(CP * N / 8, 2) void k_xrow_p(T *__restrict__ Yr, Di
         const bool has_next = bn < nblk;
         const int y0 = yof(b);
         cpx<T> *const Yz = Y + zoff(b);
-        cpx<T> uf[LU][8];
+        cpx<T> uf[LU][EPT];
 #pragma unroll
         for (int f = 0; f < LU; f++) {
             // in flight, oldest first: field f, then the item issued after it
@@ -646,7 +649,7 @@ __global__ __launch_bounds__(CP * N / 8, 2) void k_xrow_p(T *__restrict__ Yr, Di
             else wait_vmem();
             lds_barrier();
             cpx<T> *const sf = stage(f + par);
-            xrow_pass0_from_stage<T, N, CP>(sf, c, j0, d.kxm, uf[f]);
+            xrow_pass0_from_stage<T, N, CP, EPT>(sf, c, j0, d.kxm, uf[f]);
             const int g = f + 2;   // next item into the buffer just read
             if (g < LU || (has_next && g < LU + 2)) {
                 lds_barrier();
@@ -655,8 +658,8 @@ __global__ __launch_bounds__(CP * N / 8, 2) void k_xrow_p(T *__restrict__ Yr, Di
             }
             TL::template run_regs<+1>(lds, tw, c, j0, uf[f]);
         }
-        if (want_stats) xrow_stats<T, NT, LU>(uf, red, partials + (long long)b * 3 * LU);
-        xrow_products<T, N, CP, LU, LR, false, true>(lds, nullptr, tw, uf, Yz, fstride, d, y0, c, j0);
+        if (want_stats) xrow_stats<T, NT, LU, EPT>(uf, red, partials + (long long)b * 3 * LU);
+        xrow_products<T, N, CP, LU, LR, false, true, EPT>(lds, nullptr, tw, uf, Yz, fstride, d, y0, c, j0);
         par ^= LU & 1;
     }
 }
@@ -677,7 +680,7 @@ __global__ __launch_bounds__(CP * N / 8, xrow_waves<T>()) void k_xrow(T *__restr
     // product phase the two serve as alternating tiles addressed from the same
     // per-lane offsets
     constexpr int TILE = CP * N;
-    constexpr int STG = DS ? 2 * xrow_srows<N>() * 2 * CP : NT >= 64 ? (N / 2 + 8) * 2 * CP : 1;
+    constexpr int STG = DS ? 2 * xrow_srows<N>() * 2 * CP : NT >= 64 ? (N / 2 + 16) * 2 * CP : 1;
     static_assert(NT < 64 || STG >= TILE, "staging buffer holds a tile");
     __shared__ cpx<T> smem[TILE + STG];
     __shared__ double red[NW][3 * LU];
@@ -716,34 +719,43 @@ __global__ __launch_bounds__(CP * N / 8, xrow_waves<T>()) void k_xrow(T *__restr
 }
 
 template <typename T> struct XCP { static constexpr int CP = Row128<T>::C / 2; };
+// column pairs per x-row block: 128-byte k-rows, except FP32 at N >= 1024,
+// where 8 pairs (1024 threads) would cap the kernel at 128 VGPRs and spill;
+// 4 pairs (64-byte k-rows) keep 512 threads
+template <typename T, int N> constexpr int xrow_cp() { return (sizeof(T) == 4 && N >= 1024) ? 4 : XCP<T>::CP; }
 
 template <typename T>
 int launch_xrow(const Dims &d, T *Y, int want_stats, double *partials, DevDiag *diag, const void *twx, hipStream_t st,
                 int *nblocks_out)
 {
-    constexpr int CP = XCP<T>::CP;
-    const unsigned nb = (unsigned)((long long)d.nz * (d.ny / (2 * CP)));
-    if (nblocks_out) *nblocks_out = (int)nb;
-    if (d.ny % (2 * CP)) return SAM_ERR_UNSUPPORTED;
     SAM_SWITCH_N(d.nx, {
+        constexpr int CP = xrow_cp<T, N>();
+        const unsigned nb = (unsigned)((long long)d.nz * (d.ny / (2 * CP)));
+        if (nblocks_out) *nblocks_out = (int)nb;
+        if (d.ny % (2 * CP)) return SAM_ERR_UNSUPPORTED;
         if constexpr (CP * N / 8 <= 1024) {
             // double staging: whole-wave columns, dealiased rows, and the tile, both
             // staging buffers and the twiddles within the 160 KiB of LDS
             constexpr bool ds_ok = N / 8 <= 64 && (CP * N / 8) % 64 == 0 &&
                                    (CP * N + 2 * xrow_srows<N>() * 2 * CP + N / 2) * sizeof(cpx<T>) + 512 <= 160 * 1024;
+            // (16 elements per thread for FP32 at N = 1024 spills: one column per
+            // wavefront needs N / 8 <= 64)
+            constexpr int PEPT = 8;
+            constexpr bool ps_ok = sizeof(T) == 4 && N / PEPT <= 64 && (CP * N / PEPT) % 64 == 0 &&
+                                   (CP * N + 2 * xrow_srows<N>() * 2 * CP + N / 2) * sizeof(cpx<T>) + 512 <= 160 * 1024;
             const bool ds = ds_ok && d.kxm + 1 == xrow_srows<N>();
             const cpx<T> *tw = (const cpx<T> *)twx;
-            if constexpr (ds_ok) {
-                if constexpr (sizeof(T) == 4) {
-                    if (ds) {
-                        // FP32: persistent, row groups cycle through the resident blocks
-                        auto kern = d.lu == 4 ? k_xrow_p<T, N, CP, 4, 9> : k_xrow_p<T, N, CP, 3, 6>;
-                        const unsigned ng = persistent_grid((const void *)kern, CP * N / 8, nb);
-                        hipLaunchKernelGGL(kern, dim3(ng), dim3(CP * N / 8), 0, st, Y, d, tw, want_stats, partials,
-                                           (int)nb);
-                        return launched();
-                    }
+            if constexpr (ps_ok) {
+                if (d.kxm + 1 == xrow_srows<N>()) {
+                    // FP32: persistent, row groups cycle through the resident blocks
+                    auto kern = d.lu == 4 ? k_xrow_p<T, N, CP, 4, 9, PEPT> : k_xrow_p<T, N, CP, 3, 6, PEPT>;
+                    const unsigned ng = persistent_grid((const void *)kern, CP * N / PEPT, nb);
+                    hipLaunchKernelGGL(kern, dim3(ng), dim3(CP * N / PEPT), 0, st, Y, d, tw, want_stats, partials,
+                                       (int)nb);
+                    return launched();
                 }
+            }
+            if constexpr (ds_ok && sizeof(T) == 8) {
                 if (ds) {
                     // FP64: the persistent loop would push the kernel past 256 VGPRs (spills,
                     // whose scratch traffic drains the staged DMA); one group per block

@@ -169,6 +169,21 @@ def test_double_staged_x_rows(oracle_mod, prec, n, tol):
     np.testing.assert_allclose(d["energy"], st[12], rtol=10 * tol)
 
 
+@pytest.mark.parametrize("prec,tol", [(S.PREC_FP64, TOL64), (S.PREC_SPLIT, 1e-5)], ids=["fp64", "split"])
+def test_long_x_rows(oracle_mod, prec, tol):
+    """nx = 1024 (FP32 takes 64-byte k-rows, 4 column pairs per block) on a
+    thin 1024 x 32 x 32 box: full stratified steps against the oracle"""
+    nsteps = 2
+    o, s = pair(oracle_mod, 1024, 32, 32, xl=64.0, yl=2.0, zl=2.0, i_strat=1, grav=16.0, lapse0=1.0, vis=5e-3,
+                i_cfl=1, cfl0=0.5, prec=prec)
+    u0 = ic.random_solenoidal(1024, 32, 32, k0=4, seed=9, i_strat=1, xl=64.0, yl=2.0, zl=2.0)
+    s.set_state(u0)
+    s.step(nsteps)
+    ref, t, dt, st = o.step(u0, nsteps)
+    err = rel_l2(s.get_state(), ref)
+    assert err < tol, err
+
+
 def test_split_precision_bounded(oracle_mod):
     """FP32-split: transforms/products in FP32, state/RK in FP64"""
     o, s = pair(oracle_mod, 32, i_strat=1, grav=16.0, lapse0=1.0, vis=5e-3, i_cfl=1, cfl0=0.5,
