@@ -243,20 +243,24 @@ template <typename T, int N, int EPT> struct Tile {
         }
     }
 
-    // passes p..P-1 with the pass-p twiddles already read
-    template <int S, int p>
+    // passes p..P-1 with the pass-p twiddles already read.  TWPF: read the
+    // next pass's twiddles before this pass's LDS exchange (for a twiddle
+    // table in global memory, whose latency the exchange then covers); with
+    // an LDS table they are read at the pass itself, holding fewer VGPRs.
+    template <int S, int p, bool TWPF>
     __device__ static __forceinline__ void run_passes(cpx<T> *lds, const cpx<T> *__restrict__ tw, int c, int j0,
                                                       cpx<T> *v, Tw &cur)
     {
         if constexpr (p + 1 < SC::P) {
             Tw nxt;
-            tw_fetch<p + 1, S>(tw, j0, nxt);
+            if constexpr (TWPF) tw_fetch<p + 1, S>(tw, j0, nxt);
             compute<p, S>(cur, v);
             lds_write<p>(lds, c, j0, v);
             tile_sync<WAVE>();
             lds_read<p + 1>(lds, c, j0, v);
             tile_sync<WAVE>();
-            run_passes<S, p + 1>(lds, tw, c, j0, v, nxt);
+            if constexpr (!TWPF) tw_fetch<p + 1, S>(tw, j0, nxt);
+            run_passes<S, p + 1, TWPF>(lds, tw, c, j0, v, nxt);
         } else {
             compute<p, S>(cur, v);
         }
@@ -266,12 +270,12 @@ template <typename T, int N, int EPT> struct Tile {
     // inputs; the final pass leaves its outputs in v (positions kout).
     // Requires a __syncthreads() before the call if other threads may still
     // access the LDS tile region this thread's column uses.
-    template <int S, int p = 0>
+    template <int S, int p = 0, bool TWPF = true>
     __device__ static __forceinline__ void run_regs(cpx<T> *lds, const cpx<T> *__restrict__ tw, int c, int j0, cpx<T> *v)
     {
         Tw cur;
         tw_fetch<p, S>(tw, j0, cur);
-        run_passes<S, p>(lds, tw, c, j0, v, cur);
+        run_passes<S, p, TWPF>(lds, tw, c, j0, v, cur);
     }
 
     // LDS -> registers (pass-0 inputs) -> FFT -> registers (final outputs).

@@ -530,7 +530,7 @@ __device__ __forceinline__ void xrow_phase_a_ds(cpx<T> *lds, cpx<T> *stg, const 
             lds_barrier();                   // every wave has read field f's buffer
             xrow_issue_stage_ds<T, N, CP, NT>(st[f & 1], Yz + (f + 2) * fstride, ny, y0);
         }
-        TL::template run_regs<+1>(lds, tw, c, j0, uf[f]);
+        TL::template run_regs<+1, 0, false>(lds, tw, c, j0, uf[f]);
     }
 }
 
@@ -596,7 +596,7 @@ __device__ __forceinline__ void xrow_products(cpx<T> *lds, cpx<T> *stg, const cp
         } else {
             lds_barrier();
         }
-        TL::template run_regs<-1>(tile, tw, c, j0, pi);
+        TL::template run_regs<-1, 0, false>(tile, tw, c, j0, pi);
         if constexpr (KEPT) xrow_store_kept<T, N, EPT>(tile, c, j0, pi);
         else TL::store_natural(tile, c, j0, pi);
         lds_barrier();
@@ -656,7 +656,7 @@ __global__ __launch_bounds__(CP * N / EPT, 2) void k_xrow_p(T *__restrict__ Yr, 
                 if (g < LU) xrow_issue_stage_ds<T, N, CP, NT>(sf, Yz + g * fstride, d.ny, y0);
                 else xrow_issue_stage_ds<T, N, CP, NT>(sf, Y + zoff(bn) + (g - LU) * fstride, d.ny, yof(bn));
             }
-            TL::template run_regs<+1>(lds, tw, c, j0, uf[f]);
+            TL::template run_regs<+1, 0, false>(lds, tw, c, j0, uf[f]);
         }
         if (want_stats) xrow_stats<T, NT, LU, EPT>(uf, red, partials + (long long)b * 3 * LU);
         xrow_products<T, N, CP, LU, LR, false, true, EPT>(lds, nullptr, tw, uf, Yz, fstride, d, y0, c, j0);
