@@ -263,11 +263,21 @@ def main():
         "kernels": per_kernel,
     }
 
+    # measured copy-kernel bandwidth: the practical HBM ceiling next to the 8 TB/s spec
+    cal = S.Sam(make_params(S, 64, S.PREC_FP64))
+    cal.device_copy(1 << 30, 2)
+    t0 = time.perf_counter()
+    cal.device_copy(1 << 30, 10)
+    t_copy = time.perf_counter() - t0
+    out["hbm_copy"] = {"kernel": "k_copy16, 1 GiB read + 1 GiB written, 10 launches",
+                       "GBps": round(10 * (2 << 30) / t_copy / 1e9, 1),
+                       "frac_of_peak": round(10 * (2 << 30) / t_copy / HBM_PEAK, 4),
+                       "step_frac_of_copy": round(sb / (r64["wall"] / args.steps) / (10 * (2 << 30) / t_copy), 4)}
+    log(f"[bench] copy kernel {out['hbm_copy']['GBps']} GB/s")
     if args.calibrate_copy:
-        cal = S.Sam(make_params(S, 64, S.PREC_FP64))
         cal.device_copy(1 << 30, 3)
-        cal.close()
         out["copy_calibration_bytes"] = {"read": 1 << 30, "write": 1 << 30}
+    cal.close()
 
     if not args.no_split:
         log("[bench] FP32-split run")
