@@ -159,9 +159,23 @@ int  sam_mlp_destroy(sam_mlp *m);
 int  sam_mlp_get_params(sam_mlp *m, float *params);
 int  sam_mlp_set_params(sam_mlp *m, const float *params);
 /* training samples from the current state of a DNS handle: box filter by
- * `ratio` in each direction (labels tau, q) and gradient features of the
- * filtered field on the coarse grid */
+ * `ratio` in each direction and gradient features of the filtered field on
+ * the coarse grid; labels as chosen by sam_mlp_set_labels */
 int  sam_mlp_add_samples(sam_mlp *m, sam_handle *dns, int ratio);
+/* kind of label sam_mlp_add_samples writes (the pool must be empty to change
+ * it):
+ *   SAM_LABELS_FILTERED  the box filter's subfilter stress and flux,
+ *                        tau_ij = (u_i u_j)~ - u~_i u~_j (deviatoric),
+ *                        q_j = (u_j T')~ - u~_j T'~ (default);
+ *   SAM_LABELS_DYNAMIC   the dynamic model's stress and flux at unit
+ *                        coefficient (dm.f/sm.f form, tau_ij = -2|S|S_ij,
+ *                        q_j = -(|S|/pr_t) dT'/dx_j) on the filtered field.
+ *                        A network trained on these is scaled in the LES by
+ *                        C Delta^2 from the dynamic procedure, recomputed
+ *                        every sam_set_mlp_refresh steps. */
+#define SAM_LABELS_FILTERED 0
+#define SAM_LABELS_DYNAMIC  1
+int  sam_mlp_set_labels(sam_mlp *m, int mode, double pr_t);
 /* host samples: feats [n][12], labels [n][9] */
 int  sam_mlp_add_samples_host(sam_mlp *m, const float *feats, const float *labels, long long n);
 long long sam_mlp_pool_size(const sam_mlp *m);
@@ -182,6 +196,10 @@ int  sam_mlp_infer(sam_mlp *m, const float *feats, float *out, long long n);
 int  sam_attach_mlp(sam_handle *h, sam_mlp *m);
 /* switch the handle's SGS model between steps (e.g. fall back MLP -> DYN) */
 int  sam_set_sgs(sam_handle *h, int sgs);
+/* steps between updates of C Delta^2 by the dynamic procedure for a network
+ * trained on SAM_LABELS_DYNAMIC labels (default 20; the first step after
+ * sam_attach_mlp always updates) */
+int  sam_set_mlp_refresh(sam_handle *h, int nsteps);
 /* a-posteriori check of the learned closure on the current state, with the
  * dynamic model as judge: the shell spectra of the SGS energy transfer
  * T_m(k) = tk_m(k) - tk_none(k) (spectra.f's tk of the RHS with model m

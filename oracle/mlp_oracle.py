@@ -8,6 +8,12 @@ no reference counterpart; this restates the GPU network exactly:
   gradients) rounded to BF16 (round to nearest even), FP32 accumulation, FP32
   bias, GELU and Adam.
 
+  Scale-free form: the scaled features x = f * fs of a point enter as the unit
+  vector x / |x|, and the network's output is multiplied by |x|^2 (so the
+  closure is homogeneous of degree 2 in the resolved gradients, as the
+  gradient and eddy-viscosity stresses are); training targets are the scaled
+  labels divided by |x|^2.  A point with x = 0 gets a zero input and output.
+
 `bf16=False` gives the unrounded float64 network, used for finite-difference
 gradient checks.  Parameter layout (FP32 block of 10448 + 32 scales):
 W1[64][16] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[16][64] b4[16],
@@ -60,13 +66,22 @@ class Net:
     def r(self, x):
         return bf16(x) if self.bf else np.asarray(x, dtype=np.float64)
 
+    def radius2(self, feats):
+        """|x|^2 of the scaled features, computed in float32 like the GPU"""
+        x = (np.asarray(feats, np.float64) * self.fs[:NIN]).astype(self.dt)
+        return np.sum(x.astype(self.dt) * x.astype(self.dt), axis=1, dtype=self.dt)
+
     def forward(self, feats):
-        """feats [n][12] raw -> dict of activations; out [n][16] normalised"""
+        """feats [n][12] raw -> dict of activations; out [n][16] normalised
+        (scale-free: before the |x|^2 factor)"""
         dt = self.dt
         n = feats.shape[0]
         x = np.zeros((n, 16), dt)
         x[:, :NIN] = feats * self.fs[:NIN]
-        act = {"a0": self.r(x)}
+        r2 = self.radius2(feats)
+        inv = np.where(r2 > 0, 1.0 / np.sqrt(np.where(r2 > 0, r2, 1.0)), 0.0).astype(dt)
+        x = (x * inv[:, None]).astype(dt)
+        act = {"a0": self.r(x), "r2": r2}
         a = act["a0"]
         for L, (W, b) in enumerate((("W1", "b1"), ("W2", "b2"), ("W3", "b3")), start=1):
             z = (a.astype(dt) @ self.r(self.w[W].astype(dt)).T + self.w[b].astype(dt)).astype(dt)
@@ -80,7 +95,8 @@ class Net:
 
     def infer(self, feats):
         """physical outputs [n][9], tau made deviatoric"""
-        o = self.forward(feats)["out"][:, :NOUT].astype(np.float64) * self.os[:NOUT]
+        act = self.forward(feats)
+        o = act["out"][:, :NOUT].astype(np.float64) * act["r2"].astype(np.float64)[:, None] * self.os[:NOUT]
         tr = (o[:, 0] + o[:, 1] + o[:, 2]) / 3.0
         o[:, :3] -= tr[:, None]
         return o
@@ -90,8 +106,10 @@ class Net:
         dt = self.dt
         n = feats.shape[0]
         act = self.forward(feats)
+        r2 = act["r2"]
+        inv2 = np.where(r2 > 0, 1.0 / np.where(r2 > 0, r2, 1.0), 0.0).astype(dt)
         y = np.zeros((n, 16), dt)
-        y[:, :NOUT] = labels / self.os[:NOUT]
+        y[:, :NOUT] = labels / self.os[:NOUT] * inv2[:, None]
         d = np.zeros((n, 16), dt)
         d[:, :NOUT] = act["out"][:, :NOUT] - y[:, :NOUT]
         loss = float(np.sum(d.astype(np.float64) ** 2) / (n * NOUT))

@@ -73,6 +73,8 @@ struct sam_handle {
     int have_spectra = 0;
     int sgs_active = 0;
     sam_mlp *mlp = nullptr;      // network used when p.sgs == SAM_SGS_MLP
+    int mlp_refresh = 20;        // steps between dynamic-coefficient updates (unit-coefficient networks)
+    int mlp_since = 1 << 30;     // steps since the last update (large: update at the next step)
     // HIP-event timing of the step's kernels (sam_prof_*)
     bool prof = false;
     std::vector<hipEvent_t> ev_free;
@@ -459,11 +461,11 @@ static double delta_sq(const sam_handle *h)
 }
 
 // eddy-viscosity coefficient C Delta^2 into diag->c_delta_sq (and |S| into b.smag)
-template <typename T> static int sgs_coefficient(sam_handle *h, const T *P, LesBufs &b, DevDiag *dg, int report)
+template <typename T>
+static int sgs_coefficient(sam_handle *h, const T *P, LesBufs &b, DevDiag *dg, int report, bool dyn)
 {
     const Dims &d = h->d;
     const double dxi = d.nx / h->p.xl, dyi = d.ny / h->p.yl, dzi = d.nz / h->p.zl;   // 1/dx (init.f:22-27)
-    const bool dyn = h->p.sgs == SAM_SGS_DYN;
     RCHK(launch_strain_smag<T>(d, P, (T *)b.smag, dyn ? (T *)b.S6 : nullptr, dxi, dyi, dzi, h->st));
     if (!dyn) {
         RCHK(launch_dm_coef(h->partials, 0, h->p.c_smag * delta_sq(h), 0, report, dg, h->st));
@@ -482,14 +484,16 @@ template <typename T> static int sgs_coefficient(sam_handle *h, const T *P, LesB
 }
 
 template <typename T> static int mlp_flux(sam_handle *h, const T *P, T *Q);
+static bool mlp_unit_coefficient(const sam_mlp *m);
 
 template <typename T> static int les_fluxes(sam_handle *h, int stats)
 {
     const Dims &d = h->d;
     const int pi = h->prec;
     if (h->p.sgs == SAM_SGS_MLP && !h->mlp) return set_err(SAM_ERR_STATE, "sgs = MLP but no network attached");
+    const bool mlp_dyn = h->p.sgs == SAM_SGS_MLP && h->mlp && mlp_unit_coefficient(h->mlp);
     LesBufs b;
-    int rc = les_bufs<T>(h, b, h->p.sgs == SAM_SGS_DYN);
+    int rc = les_bufs<T>(h, b, h->p.sgs == SAM_SGS_DYN || mlp_dyn);
     if (rc) return rc;
     T *P = (T *)h->Pb.p, *Q = P + d.pfield * d.lu;
     T *Y = (T *)h->Yb;
@@ -506,10 +510,17 @@ template <typename T> static int les_fluxes(sam_handle *h, int stats)
     {
         PROF(SAM_K_SGS);
         if (h->p.sgs == SAM_SGS_MLP) {
+            // unit-coefficient network: the dynamic procedure supplies C Delta^2
+            // at the first substep of every mlp_refresh-th step
+            if (mlp_dyn && stats && ++h->mlp_since >= h->mlp_refresh) {
+                rc = sgs_coefficient<T>(h, P, b, h->diag, stats, true);
+                if (rc) return rc;
+                h->mlp_since = 0;
+            }
             rc = mlp_flux<T>(h, P, Q);
             if (rc) return rc;
         } else {
-            rc = sgs_coefficient<T>(h, P, b, h->diag, stats);
+            rc = sgs_coefficient<T>(h, P, b, h->diag, stats, h->p.sgs == SAM_SGS_DYN);
             if (rc) return rc;
             RCHK(launch_sgs_flux<T>(d, P, (const T *)b.smag, h->diag, Q, d.nx / h->p.xl, d.ny / h->p.yl,
                                     d.nz / h->p.zl, 1.0 / h->p.pr_t, h->st));
@@ -814,7 +825,7 @@ template <typename T> static int eddy_stage(sam_handle *h, const double *phys, d
     rc = upload_T<T>(h, P, phys, d.pfield * d.lu);
     if (rc) return rc;
     DevDiag *sd = h->diag + 1;
-    rc = sgs_coefficient<T>(h, P, b, sd, 1);
+    rc = sgs_coefficient<T>(h, P, b, sd, 1, h->p.sgs == SAM_SGS_DYN);
     if (rc) return rc;
     rc = ensure(h->stage0, sizeof(double) * d.pfield);
     if (rc) return rc;
@@ -911,6 +922,8 @@ struct sam_mlp {
     int step = 0;
     float *feats = nullptr, *labels = nullptr;
     long long cap = 0, n = 0;
+    int label_mode = SAM_LABELS_FILTERED;   // what the pool's labels are (see sam_mlp_set_labels)
+    float pr_t = 1.0f;
     DevBuf act, gpart, lpart, npart, gout, tmp, lossv;
 };
 
@@ -1090,8 +1103,32 @@ extern "C" int sam_mlp_add_samples(sam_mlp *m, sam_handle *dns, int r)
         RCHK(launch_box_filter<double>(d, (const double *)dns->Pb.p, r, coarse, m->labels + 16 * m->n, m->st));
     RCHK(launch_coarse_features(coarse, cx, cy, cz, d.lu, (float)(cx / dns->p.xl), (float)(cy / dns->p.yl),
                                 (float)(cz / dns->p.zl), m->feats + 16 * m->n, m->st));
+    if (m->label_mode == SAM_LABELS_DYNAMIC)
+        RCHK(launch_dyn_labels(m->feats + 16 * m->n, nc, 1.0f / m->pr_t, m->labels + 16 * m->n, m->st));
     HIPCHK(hipStreamSynchronize(m->st));
     m->n += nc;
+    return SAM_OK;
+}
+
+extern "C" int sam_mlp_set_labels(sam_mlp *m, int mode, double pr_t)
+{
+    if (!m) return set_err(SAM_ERR_VALUE, "null argument");
+    if (mode != SAM_LABELS_FILTERED && mode != SAM_LABELS_DYNAMIC) return set_err(SAM_ERR_VALUE, "unknown label mode %d", mode);
+    if (!(pr_t > 0)) return set_err(SAM_ERR_VALUE, "pr_t must be > 0");
+    if (m->n > 0 && mode != m->label_mode) return set_err(SAM_ERR_STATE, "the sample pool holds labels of the other kind");
+    m->label_mode = mode;
+    m->pr_t = (float)pr_t;
+    return SAM_OK;
+}
+
+static bool mlp_unit_coefficient(const sam_mlp *m) { return m->label_mode == SAM_LABELS_DYNAMIC; }
+
+extern "C" int sam_set_mlp_refresh(sam_handle *h, int nsteps)
+{
+    if (!h) return set_err(SAM_ERR_VALUE, "null handle");
+    if (nsteps < 1) return set_err(SAM_ERR_VALUE, "refresh interval must be >= 1 step");
+    h->mlp_refresh = nsteps;
+    h->mlp_since = 1 << 30;
     return SAM_OK;
 }
 
@@ -1220,6 +1257,7 @@ extern "C" int sam_attach_mlp(sam_handle *h, sam_mlp *m)
 {
     if (!h) return set_err(SAM_ERR_VALUE, "null handle");
     h->mlp = m;
+    h->mlp_since = 1 << 30;   // a unit-coefficient network gets C Delta^2 at the next step
     return SAM_OK;
 }
 
@@ -1238,8 +1276,9 @@ template <typename T> static int mlp_flux(sam_handle *h, const T *P, T *Q)
 {
     const Dims &d = h->d;
     HIPCHK(hipStreamSynchronize(h->mlp->st));   // parameters written on the network's stream
+    const double *cs = mlp_unit_coefficient(h->mlp) ? &h->diag->c_delta_sq : nullptr;
     RCHK(launch_les_mlp_flux<T>(d, h->mlp->img, h->mlp->norms, P, Q, (float)(d.nx / h->p.xl), (float)(d.ny / h->p.yl),
-                                (float)(d.nz / h->p.zl), h->st));
+                                (float)(d.nz / h->p.zl), cs, h->st));
     return SAM_OK;
 }
 

@@ -119,9 +119,11 @@ int launch_mlp_adam(const float *part, int nch, float *P, float *m1, float *m2, 
 int launch_sum_loss(const double *part, int n, double *out, hipStream_t st);
 int launch_mlp_infer(const float *P, const void *img, const float *norms, const float *feats, float *out, long long n,
                      hipStream_t st);
+// cscale: device pointer to C Delta^2 for unit-coefficient networks, else null
 template <typename T>
 int launch_les_mlp_flux(const Dims &d, const void *img, const float *norms, const T *U, T *Q, float dxi, float dyi,
-                        float dzi, hipStream_t st);
+                        float dzi, const double *cscale, hipStream_t st);
+int launch_dyn_labels(const float *feats, long long n, float pr_t_inv, float *labels, hipStream_t st);
 template <typename T> int launch_box_filter(const Dims &d, const T *U, int r, float *coarse, float *labels, hipStream_t st);
 int launch_coarse_features(const float *C, int nx, int ny, int nz, int lu, float dxi, float dyi, float dzi,
                            float *feats, hipStream_t st);

@@ -16,6 +16,11 @@
 // layer's B operand, so activations stay in registers from layer to layer;
 // the weight tiles (pre-arranged per lane, forward and transposed) sit in
 // LDS.  One wavefront processes 16 points per tile.
+//
+// Scale-free form: a point's scaled features x (12) enter as x/|x| and the
+// output is multiplied by |x|^2, so the closure is homogeneous of degree 2
+// in the resolved gradients (as gradient-model and eddy-viscosity stresses
+// are) and holds outside the gradient range it was trained on.
 #include "internal.hpp"
 
 namespace samhip {
@@ -50,6 +55,17 @@ __device__ __forceinline__ v4s bf4(v4f x)
 }
 
 __device__ __forceinline__ v4f mfma(v4s a, v4s b, v4f c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
+
+// |x|^2 of a point's scaled features, held 4 per lane in lanes i, i+16, i+32
+// (lane i+48 holds zeros); every lane of the point gets the sum
+__device__ __forceinline__ float point_r2(v4f x)
+{
+    float r = x[0] * x[0] + x[1] * x[1] + x[2] * x[2] + x[3] * x[3];
+    r += __shfl_xor(r, 16, 64);
+    r += __shfl_xor(r, 32, 64);
+    return r;
+}
+__device__ __forceinline__ float inv_radius(float r2) { return r2 > 0.f ? 1.0f / sqrtf(r2) : 0.f; }
 
 // GELU, tanh form: 0.5 x (1 + tanh u) with u = c (x + a x^3), evaluated as
 // x * sigmoid(2u) = x / (1 + 2^(-2u log2 e)) (one exp2, one reciprocal);
@@ -257,6 +273,8 @@ __global__ __launch_bounds__(256) void k_mlp_fb(const float *__restrict__ P, con
 #pragma unroll
             for (int kk = 0; kk < 4; kk++) x[kk] = feats[s * 16 + 4 * g + kk] * norms[4 * g + kk];
         }
+        const float r2 = point_r2(x), ir = inv_radius(r2), ir2 = ir * ir;
+        x *= ir;
         const int zo = opaque_zero();
         const v4s *tl = tiles + zo;
         Acts A;
@@ -268,7 +286,7 @@ __global__ __launch_bounds__(256) void k_mlp_fb(const float *__restrict__ P, con
             const int o = 4 * g + r;
             float d = 0.f;
             if (o < MLP_OUT) {
-                const float y = labels[s * 16 + o] / norms[16 + o];
+                const float y = labels[s * 16 + o] / norms[16 + o] * ir2;   // scale-free target
                 d = A.out[r] - y;
                 lsum += (double)d * d;
             }
@@ -413,11 +431,13 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const float *__restrict__ P, 
         v4f x = {0, 0, 0, 0};
         if (g < 3 && p < n)
             for (int kk = 0; kk < 4; kk++) x[kk] = feats[p * 16 + 4 * g + kk] * norms[4 * g + kk];
+        const float r2 = point_r2(x);
+        x *= inv_radius(r2);
         const int zo = opaque_zero();
         Acts A;
         mlp_forward(reinterpret_cast<const float *>(biasv) + zo, tiles + zo, lane, x, A);
         v4f o = A.out;
-        for (int r = 0; r < 4; r++) o[r] *= (4 * g + r < MLP_OUT) ? norms[16 + 4 * g + r] : 0.f;
+        for (int r = 0; r < 4; r++) o[r] *= (4 * g + r < MLP_OUT) ? norms[16 + 4 * g + r] * r2 : 0.f;
         deviatoric(g, o);
         if (p < n) *reinterpret_cast<v4f *>(out + p * 16 + 4 * g) = o;
     }
@@ -479,7 +499,8 @@ __device__ __forceinline__ void stencil_load(const T *__restrict__ F, int lx, in
 template <typename T>
 __global__ __launch_bounds__(256) void k_les_mlp_flux(const v4s *__restrict__ img, const float *__restrict__ norms,
                                                        const T *__restrict__ U, T *__restrict__ Q, int lx, int ly,
-                                                       int lz, int lu, float dxi, float dyi, float dzi)
+                                                       int lz, int lu, float dxi, float dyi, float dzi,
+                                                       const double *__restrict__ cscale)
 {
     __shared__ v4s tiles[T_W4T * 64];
     __shared__ v4f biasv[NB_BIAS / 4];
@@ -502,13 +523,15 @@ __global__ __launch_bounds__(256) void k_les_mlp_flux(const v4s *__restrict__ im
     for (int j = 0; j < 3; j++) nsc[j] = has_f ? norms[3 * g + j] : 0.f;
     T *qp[4];
     float osc[4];
+    // networks trained on unit-coefficient dynamic-model stresses: times C Delta^2
+    const float cs = cscale ? (float)*cscale : 1.0f;
     int ca[4], cb[4];
     bool ok[4];
 #pragma unroll
     for (int r = 0; r < 4; r++) {
         const int oi = 4 * g + r, sl = oi < MLP_OUT ? out_slot(oi) : 0;
         ok[r] = oi < MLP_OUT && (sl < 6 || lu == 4);
-        osc[r] = ok[r] ? norms[16 + oi] : 0.f;
+        osc[r] = ok[r] ? norms[16 + oi] * cs : 0.f;
         qp[r] = Q + (size_t)sl * np;
         ca[r] = 4 * i + slot_a(sl);
         cb[r] = 4 * i + slot_b(sl);
@@ -530,14 +553,16 @@ __global__ __launch_bounds__(256) void k_les_mlp_flux(const v4s *__restrict__ im
         const bool more = t + stride < ntile;
         if (more) stencil_load(F, lx, ly, lz, (t + stride) * 16 + i, cur);
         tile_sync<true>();
-        const v4f xin = g < 3 ? *reinterpret_cast<const v4f *>(feat + 12 * i + 4 * g) : v4f{0, 0, 0, 0};
+        v4f xin = g < 3 ? *reinterpret_cast<const v4f *>(feat + 12 * i + 4 * g) : v4f{0, 0, 0, 0};
+        const float r2 = point_r2(xin);
+        xin *= inv_radius(r2);
 
         const int zo = opaque_zero();
         Acts A;
         mlp_forward(reinterpret_cast<const float *>(biasv) + zo, tiles + zo, lane, xin, A);
         v4f o = A.out;
 #pragma unroll
-        for (int r = 0; r < 4; r++) o[r] *= osc[r];
+        for (int r = 0; r < 4; r++) o[r] *= osc[r] * r2;
         deviatoric(g, o);
 #pragma unroll
         for (int r = 0; r < 4; r++) {
@@ -581,6 +606,39 @@ __global__ __launch_bounds__(256) void k_box_filter(const T *__restrict__ U, int
     float *L = labels + c * 16;
     for (int k = 0; k < 6; k++) L[k] = (float)tau[k];
     for (int k = 0; k < 3; k++) L[6 + k] = lu == 4 ? (float)(pt[k] * inv - m[k] * m[3]) : 0.f;
+    for (int k = 9; k < 16; k++) L[k] = 0.f;
+}
+
+// Dynamic-model labels at unit coefficient from the raw gradient features
+// (dm.f/sm.f stress form): tau_ij = -2 |S| S_ij (deviatoric), q_j =
+// -(|S|/Pr_t) dT/dx_j, with S_ij = (du_i/dx_j + du_j/dx_i)/2 and |S| =
+// sqrt(2 S_ij S_ij) from the same centred differences as the features.  The
+// dynamic procedure supplies C Delta^2 at run time.
+__global__ __launch_bounds__(256) void k_dyn_labels(const float *__restrict__ feats, long long n, float pr_t_inv,
+                                                     float *__restrict__ labels)
+{
+    const long long p = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (p >= n) return;
+    const float *F = feats + p * 16;
+    double g[3][3], S[3][3];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) g[i][j] = F[3 * i + j];
+    double s2 = 0.0;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            S[i][j] = 0.5 * (g[i][j] + g[j][i]);
+            s2 += S[i][j] * S[i][j];
+        }
+    const double smag = sqrt(2.0 * s2);
+    const double tr = (S[0][0] + S[1][1] + S[2][2]) / 3.0;
+    float *L = labels + p * 16;
+    L[0] = (float)(-2.0 * smag * (S[0][0] - tr));
+    L[1] = (float)(-2.0 * smag * (S[1][1] - tr));
+    L[2] = (float)(-2.0 * smag * (S[2][2] - tr));
+    L[3] = (float)(-2.0 * smag * S[0][1]);
+    L[4] = (float)(-2.0 * smag * S[1][2]);
+    L[5] = (float)(-2.0 * smag * S[0][2]);
+    for (int j = 0; j < 3; j++) L[6 + j] = (float)(-smag * (double)F[9 + j] * (double)pr_t_inv);
     for (int k = 9; k < 16; k++) L[k] = 0.f;
 }
 
@@ -689,9 +747,15 @@ int launch_mlp_infer(const float *P, const void *img, const float *norms, const 
     return launched_mlp();
 }
 
+int launch_dyn_labels(const float *feats, long long n, float pr_t_inv, float *labels, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_dyn_labels, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, feats, n, pr_t_inv, labels);
+    return launched_mlp();
+}
+
 template <typename T>
 int launch_les_mlp_flux(const Dims &d, const void *img, const float *norms, const T *U, T *Q, float dxi, float dyi,
-                        float dzi, hipStream_t st)
+                        float dzi, const double *cscale, hipStream_t st)
 {
     // 32-bit point indices; x-rows of at least one 16-point tile
     if (d.pfield >= (1ll << 31) || d.nx < 16) return SAM_ERR_UNSUPPORTED;
@@ -699,7 +763,7 @@ int launch_les_mlp_flux(const Dims &d, const void *img, const float *norms, cons
     unsigned nb = (unsigned)((nt + 3) / 4);
     if (nb > 2048) nb = 2048;
     hipLaunchKernelGGL((k_les_mlp_flux<T>), dim3(nb), dim3(256), 0, st, (const v4s *)img, norms, U, Q,
-                       __builtin_ctz(d.nx), __builtin_ctz(d.ny), __builtin_ctz(d.nz), d.lu, dxi, dyi, dzi);
+                       __builtin_ctz(d.nx), __builtin_ctz(d.ny), __builtin_ctz(d.nz), d.lu, dxi, dyi, dzi, cscale);
     return launched_mlp();
 }
 
@@ -730,9 +794,9 @@ int launch_pool_norms(const float *feats, const float *labels, long long n, doub
 }
 
 template int launch_les_mlp_flux<double>(const Dims &, const void *, const float *, const double *, double *, float,
-                                         float, float, hipStream_t);
+                                         float, float, const double *, hipStream_t);
 template int launch_les_mlp_flux<float>(const Dims &, const void *, const float *, const float *, float *, float,
-                                        float, float, hipStream_t);
+                                        float, float, const double *, hipStream_t);
 template int launch_box_filter<double>(const Dims &, const double *, int, float *, float *, hipStream_t);
 template int launch_box_filter<float>(const Dims &, const float *, int, float *, float *, hipStream_t);
 

@@ -39,12 +39,13 @@ EXPORTS = (
     "sam_prof_enable", "sam_prof_read", "sam_sync", "sam_device_copy",
     "sam_mlp_create", "sam_mlp_destroy", "sam_mlp_get_params", "sam_mlp_set_params",
     "sam_mlp_add_samples", "sam_mlp_add_samples_host", "sam_mlp_pool_size", "sam_mlp_get_samples",
-    "sam_mlp_clear_pool",
+    "sam_mlp_clear_pool", "sam_mlp_set_labels",
     "sam_mlp_fit_norms", "sam_mlp_train", "sam_mlp_grad", "sam_mlp_infer",
-    "sam_attach_mlp", "sam_set_sgs", "sam_sgs_check",
+    "sam_attach_mlp", "sam_set_sgs", "sam_sgs_check", "sam_set_mlp_refresh",
 )
 
 MLP_NPARAM = 10448
+LABELS_FILTERED, LABELS_DYNAMIC = 0, 1
 
 # kernel classes of sam_prof_read (SAM_K_* in include/samhip.h)
 KERNEL_CLASSES = ("zb", "yb", "xrow", "final", "yf", "zrhs", "spec", "xb", "xf", "sgs", "misc")
@@ -143,11 +144,13 @@ def load():
     L.sam_mlp_clear_pool.argtypes = [_hp]
     L.sam_mlp_get_samples.argtypes = [_hp, ctypes.c_longlong, ctypes.c_longlong, _fp, _fp]
     L.sam_mlp_fit_norms.argtypes = [_hp]
+    L.sam_mlp_set_labels.argtypes = [_hp, ctypes.c_int, ctypes.c_double]
     L.sam_mlp_train.argtypes = [_hp, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_uint, _fp]
     L.sam_mlp_grad.argtypes = [_hp, _fp, _fp, ctypes.c_int, _fp, ctypes.POINTER(ctypes.c_double)]
     L.sam_mlp_infer.argtypes = [_hp, _fp, _fp, ctypes.c_longlong]
     L.sam_attach_mlp.argtypes = [_hp, _hp]
     L.sam_set_sgs.argtypes = [_hp, ctypes.c_int]
+    L.sam_set_mlp_refresh.argtypes = [_hp, ctypes.c_int]
     L.sam_sgs_check.argtypes = [_hp, ctypes.c_double, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]
     for name in EXPORTS:
         if name in ("sam_last_error", "sam_mlp_pool_size"):
@@ -380,6 +383,11 @@ class Mlp:
     def clear_pool(self):
         _chk(self.L.sam_mlp_clear_pool(self.h))
 
+    def set_labels(self, mode, pr_t=1.0):
+        """LABELS_FILTERED (subfilter stress of the box filter) or LABELS_DYNAMIC
+        (dynamic-model stress at unit coefficient); set before adding samples"""
+        _chk(self.L.sam_mlp_set_labels(self.h, int(mode), ctypes.c_double(pr_t)))
+
     def fit_norms(self):
         _chk(self.L.sam_mlp_fit_norms(self.h))
 
@@ -421,3 +429,8 @@ def sgs_check(sam, tol):
 
 def set_sgs(sam, model):
     _chk(load().sam_set_sgs(sam.h, int(model)))
+
+
+def set_mlp_refresh(sam, nsteps):
+    """steps between dynamic-coefficient updates for a unit-coefficient network"""
+    _chk(load().sam_set_mlp_refresh(sam.h, int(nsteps)))
