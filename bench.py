@@ -153,7 +153,8 @@ def les_comparison(S, ic, n_dns, n_les=128, les_steps=2000, refresh=20):
     res = {}
     runs = (("dynamic", S.SGS_DYN, u_les), ("learned", S.SGS_MLP, u_les),
             ("dynamic_perturbed", S.SGS_DYN, u_les * (1 + 1e-6 * rng.standard_normal(u_les.shape))))
-    half = les_steps // 2
+    half = max(1, les_steps // 2)
+    chunk = min(50, max(1, half // 2))    # steps between E(k) samples of the second half
     for name, model, u in runs:
         les = S.Sam(S.make_params(n_les, sgs=model, **phys))
         if model == S.SGS_MLP:
@@ -164,8 +165,8 @@ def les_comparison(S, ic, n_dns, n_les=128, les_steps=2000, refresh=20):
         t0 = time.perf_counter()
         les.step(half)
         acc, ns = 0.0, 0
-        for _ in range(half // 50):
-            les.step(50, want_spectra=True)
+        for _ in range(half // chunk):
+            les.step(chunk, want_spectra=True)
             acc = acc + L.shell_energy(les.spectra())
             ns += 1
         les.sync()
@@ -174,7 +175,7 @@ def les_comparison(S, ic, n_dns, n_les=128, les_steps=2000, refresh=20):
         if model == S.SGS_MLP:
             err, sw = S.sgs_check(les, 1.0)
             check = {"sgs_transfer_rel_err_vs_dynamic": round(float(err), 4), "fell_back": bool(sw)}
-        res[name] = dict(ms=wall / (half + 50 * (half // 50)) * 1e3, final=L.shell_energy(les.spectra()),
+        res[name] = dict(ms=wall / (half + chunk * (half // chunk)) * 1e3, final=L.shell_energy(les.spectra()),
                          avg=acc / ns, check=check)
         les.close()
 
@@ -251,6 +252,8 @@ def main():
     ap.add_argument("--no-les", action="store_true")
     ap.add_argument("--no-big", action="store_true", help="skip the 1024^3 FP32-split step")
     ap.add_argument("--big-n", type=int, default=1024)
+    ap.add_argument("--les-steps", type=int, default=2000,
+                    help="steps of each 128^3 LES run (a short count for profiler runs)")
     ap.add_argument("--cpu-n", type=int, default=0, help="CPU baseline grid (default: same n if RAM allows)")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -343,7 +346,7 @@ def main():
 
     if not args.no_les and n % 128 == 0 and n >= 256:
         log("[bench] LES 128^3: learned closure vs dynamic model")
-        out["les_128"] = les_comparison(S, ic, n)
+        out["les_128"] = les_comparison(S, ic, n, les_steps=args.les_steps)
         lr = out["les_128"]
         log(f"[bench] LES dyn {lr['dynamic_ms_per_step']} ms, learned {lr['learned_ms_per_step']} ms, "
             f"E(k) err {lr['ek_rel_err_learned_vs_dynamic']}")

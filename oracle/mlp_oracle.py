@@ -109,7 +109,9 @@ class Net:
         r2 = act["r2"]
         inv2 = np.where(r2 > 0, 1.0 / np.where(r2 > 0, r2, 1.0), 0.0).astype(dt)
         y = np.zeros((n, 16), dt)
-        y[:, :NOUT] = labels / self.os[:NOUT] * inv2[:, None]
+        os_ = np.broadcast_to(self.os[:NOUT], (n, NOUT))
+        scaled = np.divide(labels, os_, out=np.zeros((n, NOUT)), where=os_ > 0)   # zero column: target 0
+        y[:, :NOUT] = scaled * inv2[:, None]
         d = np.zeros((n, 16), dt)
         d[:, :NOUT] = act["out"][:, :NOUT] - y[:, :NOUT]
         loss = float(np.sum(d.astype(np.float64) ** 2) / (n * NOUT))

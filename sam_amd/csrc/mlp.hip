@@ -286,7 +286,10 @@ __global__ __launch_bounds__(256) void k_mlp_fb(const float *__restrict__ P, con
             const int o = 4 * g + r;
             float d = 0.f;
             if (o < MLP_OUT) {
-                const float y = labels[s * 16 + o] / norms[16 + o] * ir2;   // scale-free target
+                // scale-free target; a column that is zero over the pool (the scalar
+                // flux of a flow without temperature) has scale 0 and target 0
+                const float os = norms[16 + o];
+                const float y = os > 0.f ? labels[s * 16 + o] / os * ir2 : 0.f;
                 d = A.out[r] - y;
                 lsum += (double)d * d;
             }

@@ -15,14 +15,20 @@
 !   i_lapse 0 restart keeps lapse = 0 as SAM does (initial_field.f:37 is the
 !           only place lapse is set), 1 uses lapse0 after a restart
 !   pr_t    turbulent Prandtl number of the SGS scalar flux (default 1)
+!   mlp_labels   i_les 4: 1 (default) the network was trained on unit-
+!                coefficient dynamic-model labels and is scaled by C Delta^2
+!                from the dynamic procedure; 0 trained on box-filter labels
+!   mlp_refresh  i_les 4, mlp_labels 1: steps between C Delta^2 updates (20)
 ! i_les: 0 DNS, 1 Chollet-Lesieur (diagnostic-only in SAM; runs as DNS),
-!        2 Smagorinsky (sm.f), 3 dynamic Smagorinsky (dm.f), 4 learned closure.
+!        2 Smagorinsky (sm.f), 3 dynamic Smagorinsky (dm.f), 4 learned closure
+!        with its parameters read from mlp.params (SAM_MLP_NPARAM + 32 raw
+!        FP32 values as written by sam_mlp_get_params).
 program samhip_driver
   use iso_c_binding
   use samhip_mod
   implicit none
 
-  integer, parameter :: NIN = 72, NDYN = 6, NPAR = NIN + NDYN, NEXT = 3
+  integer, parameter :: NIN = 72, NDYN = 6, NPAR = NIN + NDYN, NEXT = 5
   character(len=12) :: labels(NPAR + NEXT)
   real(8) :: values(NPAR + NEXT), defaults(NPAR + NEXT)
   logical :: required(NPAR + NEXT), fixed(NPAR + NEXT), found(NPAR + NEXT)
@@ -36,7 +42,8 @@ program samhip_driver
   type(sam_params) :: p
   type(sam_dims) :: d
   type(sam_diag) :: dg
-  type(c_ptr) :: h
+  type(c_ptr) :: h, mlp
+  real(c_float), allocatable :: mlp_params(:)
   complex(c_double_complex), allocatable :: u(:, :, :, :)
   real(c_double), allocatable :: spec(:)
   real(8) :: time, dt, t_stat
@@ -81,8 +88,11 @@ program samhip_driver
   end if
   p%cfl0 = rv('cfl0'); p%dt0 = rv('dt0'); p%c_smag = rv('c_smag'); p%pr_t = rv('pr_t')
 
+  if (i_les == 4) call read_network()
   rc = sam_create(p, h)
   if (rc /= SAM_OK) call fail(int(rc), 'sam_create: '//trim(sam_error_message()))
+  mlp = c_null_ptr
+  if (i_les == 4) call attach_network()
   rc = sam_get_dims(h, d)
   lu = d%lu
   lm = d%lm
@@ -135,9 +145,36 @@ program samhip_driver
   end do
 
   rc = sam_destroy(h)
+  if (c_associated(mlp)) rc = sam_mlp_destroy(mlp)
   close (2); close (3); close (4); close (7); close (12)
 
 contains
+
+  ! learned closure for i_les 4: parameters from mlp.params
+  subroutine read_network()
+    integer :: un, ios
+    integer(8) :: fsize
+    allocate(mlp_params(SAM_MLP_NPARAM + 32))
+    inquire (file='mlp.params', size=fsize)
+    if (fsize < 0) call fail(6, 'i_les=4 needs mlp.params (the trained network)')
+    if (fsize /= 4_8 * size(mlp_params, kind=8)) &
+         call fail(10, 'mlp.params must hold SAM_MLP_NPARAM + 32 FP32 values')
+    open (newunit=un, file='mlp.params', access='stream', form='unformatted', status='old', &
+         action='read', iostat=ios)
+    if (ios == 0) read (un, iostat=ios) mlp_params
+    if (ios /= 0) call fail(6, 'cannot read mlp.params')
+    close (un)
+  end subroutine read_network
+
+  subroutine attach_network()
+    rc = sam_mlp_create(mlp, 0_c_int, 0_c_long_long)
+    if (rc /= SAM_OK) call fail(int(rc), 'sam_mlp_create: '//trim(sam_error_message()))
+    rc = sam_mlp_set_labels(mlp, int(iv('mlp_labels'), c_int), rv('pr_t'))
+    if (rc == SAM_OK) rc = sam_mlp_set_params(mlp, mlp_params)
+    if (rc == SAM_OK) rc = sam_attach_mlp(h, mlp)
+    if (rc == SAM_OK) rc = sam_set_mlp_refresh(h, int(iv('mlp_refresh'), c_int))
+    if (rc /= SAM_OK) call fail(int(rc), 'learned closure: '//trim(sam_error_message()))
+  end subroutine attach_network
 
   subroutine fail(code, what)
     integer, intent(in) :: code
@@ -215,8 +252,8 @@ contains
     end do
     labels(73:78) = [character(len=12) :: 'time', 'dt', 't_stat', 'nt', 'n_frame_p', 'n_hist']
     defaults(73:78) = 0d0; required(73:78) = .true.; fixed(73:78) = .false.
-    labels(79:81) = [character(len=12) :: 'i_prec', 'i_lapse', 'pr_t']
-    defaults(79:81) = [0d0, 0d0, 1d0]; required(79:81) = .false.; fixed(79:81) = .false.
+    labels(79:83) = [character(len=12) :: 'i_prec', 'i_lapse', 'pr_t', 'mlp_labels', 'mlp_refresh']
+    defaults(79:83) = [0d0, 0d0, 1d0, 1d0, 20d0]; required(79:83) = .false.; fixed(79:83) = .false.
   end subroutine set_labels
 
   ! read_line.f:24-89: one "label value [comment]" per line; '#' in column 1
@@ -334,6 +371,8 @@ contains
     call range_check('i_force', 0, 1)
     call range_check('k_force', 0, max(iv('nx') / 2, iv('ny') / 2, iv('nz') / 2))
     call range_check('i_prec', 0, 1)
+    call range_check('mlp_labels', 0, 1)
+    call range_check('mlp_refresh', 1, 1000000)
   contains
     subroutine range_check(name, lo, hi)
       character(len=*), intent(in) :: name

@@ -10,6 +10,7 @@ module samhip_mod
        SAM_ERR_SIZE = 10, SAM_ERR_UNSUPPORTED = 20
   integer, parameter :: SAM_PREC_FP64 = 0, SAM_PREC_SPLIT = 1
   integer, parameter :: SAM_SGS_NONE = 0, SAM_SGS_SMAG = 1, SAM_SGS_DYN = 2, SAM_SGS_MLP = 3
+  integer, parameter :: SAM_LABELS_FILTERED = 0, SAM_LABELS_DYNAMIC = 1, SAM_MLP_NPARAM = 10448
 
   type, bind(c) :: sam_params
      integer(c_int) :: nx, ny, nz, n_dealias, i_strat, nrk_max, i_cfl, iubc_z, precision, sgs
@@ -91,6 +92,43 @@ module samhip_mod
        type(c_ptr), value :: h
        real(c_double), intent(out) :: out(*)
      end function sam_get_spectra
+
+     ! learned closure (no counterpart in the reference)
+     integer(c_int) function sam_mlp_create(m, seed, cap) bind(c, name='sam_mlp_create')
+       import :: c_int, c_ptr, c_long_long
+       type(c_ptr), intent(out) :: m
+       integer(c_int), value :: seed
+       integer(c_long_long), value :: cap
+     end function sam_mlp_create
+
+     integer(c_int) function sam_mlp_destroy(m) bind(c, name='sam_mlp_destroy')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: m
+     end function sam_mlp_destroy
+
+     integer(c_int) function sam_mlp_set_params(m, params) bind(c, name='sam_mlp_set_params')
+       import :: c_int, c_ptr, c_float
+       type(c_ptr), value :: m
+       real(c_float), intent(in) :: params(*)
+     end function sam_mlp_set_params
+
+     integer(c_int) function sam_mlp_set_labels(m, mode, pr_t) bind(c, name='sam_mlp_set_labels')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: m
+       integer(c_int), value :: mode
+       real(c_double), value :: pr_t
+     end function sam_mlp_set_labels
+
+     integer(c_int) function sam_attach_mlp(h, m) bind(c, name='sam_attach_mlp')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: h, m
+     end function sam_attach_mlp
+
+     integer(c_int) function sam_set_mlp_refresh(h, nsteps) bind(c, name='sam_set_mlp_refresh')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: h
+       integer(c_int), value :: nsteps
+     end function sam_set_mlp_refresh
   end interface
 
 contains

@@ -81,3 +81,56 @@ def test_taylor_green_run_end_to_end(tmp_path, oracle_mod):
     e = sio.read_log(str(tmp_path / "energy.out"))
     assert list(e[:, 0]) == [0, 5, 10, 15, 20]
     assert abs(e[0, 2] - 0.125) < 1e-4          # TG energy at t=0 (log precision)
+
+
+@needs_driver
+def test_learned_closure_needs_its_network(tmp_path):
+    """i_les 4 reads the trained network from mlp.params before any GPU work"""
+    r = run(tmp_path, **dict(TG64, i_les=4))
+    assert r.returncode == 6 and "mlp.params" in r.stdout
+    (tmp_path / "mlp.params").write_bytes(b"\0" * 400)
+    r = run(tmp_path, **dict(TG64, i_les=4))
+    assert r.returncode == 10 and "mlp.params" in r.stdout
+    r = run(tmp_path, **dict(TG64, i_les=4, mlp_labels=2))
+    assert r.returncode == 9 and "mlp_labels" in r.stdout
+
+
+@pytest.mark.gpu
+@needs_driver
+def test_learned_closure_run_matches_library(tmp_path):
+    """i_les 4 from SAM's run files gives the state of the same network driven
+    through the library directly (unit-coefficient labels, C Delta^2 every 5
+    steps), 32^3 Taylor-Green, 12 steps"""
+    from sam_amd import les as L
+    from sam_amd import samhip as S
+    dns = S.Sam(S.make_params(64, vis=1e-3, dt0=0.01))
+    dns.set_state(ic.random_solenoidal(64, k0=4, seed=11))
+    dns.step(5)
+    mlp = S.Mlp(seed=5, pool_capacity=(64 // 2) ** 3)
+    mlp.set_labels(S.LABELS_DYNAMIC, 0.7)
+    mlp.add_samples(dns, 2)
+    L.train(mlp, 60, batch=4096, lr=2e-3, seed=1)
+    params = mlp.get_params()
+    params.tofile(str(tmp_path / "mlp.params"))
+
+    run_p = dict(TG64, Nx=32, Ny=32, Nz=32, vis=1e-3, n_steps=12, n_skip_h=4, i_les=4, pr_t=0.7,
+                 mlp_labels=1, mlp_refresh=5)
+    u0 = ic.taylor_green(32)
+    sio.write_vel(str(tmp_path / "vel.000000"), u0)
+    sio.write_header(str(tmp_path / "header.000000"), run_p, time=0.0, dt=0.01, nt=0)
+    r = run(tmp_path, **run_p)
+    assert r.returncode == 0, r.stdout + r.stderr
+
+    les = S.Sam(S.make_params(32, vis=1e-3, dt0=0.01, sgs=S.SGS_MLP, pr_t=0.7))
+    net = S.Mlp(seed=0)
+    net.set_labels(S.LABELS_DYNAMIC, 0.7)
+    net.set_params(params)
+    S.attach_mlp(les, net)
+    S.set_mlp_refresh(les, 5)
+    les.set_state(u0)
+    les.step(12)
+    ref = les.get_state()
+    d = les.dims
+    u = sio.read_vel(str(tmp_path / "vel.000012"), d.nz_min, d.ny_min, d.nx_min, d.lu)
+    assert np.linalg.norm(ref - ic.taylor_green(32)) > 1e-6 * np.linalg.norm(ref)
+    assert np.linalg.norm(u - ref) <= 1e-14 * np.linalg.norm(ref)

@@ -59,6 +59,28 @@ def test_gradient_matches_oracle():
     assert rel(g, g64) < 5e-2
 
 
+def test_gradient_with_zero_output_columns():
+    """a pool without temperature has zero scalar-flux columns: their output
+    scale is 0 and their target 0 (not 0/0), the loss and gradient stay finite
+    and match the oracle"""
+    p = random_params(8)
+    p[M.NPARAM + 16 + 6:M.NPARAM + 16 + 9] = 0.0
+    p[M.NPARAM + 9:M.NPARAM + 12] = 0.0
+    m = S.Mlp(seed=1)
+    m.set_params(p)
+    rng = np.random.default_rng(9)
+    n = 4096
+    f = rng.standard_normal((n, 12)).astype(np.float32)
+    f[:, 9:] = 0.0
+    y = rng.standard_normal((n, 9)).astype(np.float32)
+    y[:, 6:] = 0.0
+    loss, g = m.grad(f, y)
+    loss_ref, g_ref = M.Net(p, bf=True).loss_grad(f, y)
+    assert np.isfinite(loss) and np.all(np.isfinite(g))
+    assert abs(loss - loss_ref) < 1e-4 * loss_ref
+    assert rel(g, g_ref) < 5e-3, rel(g, g_ref)
+
+
 def test_training_reduces_loss():
     """learn a Smagorinsky-like map tau = -|A| (A + A^T)/2 from gradients"""
     rng = np.random.default_rng(7)

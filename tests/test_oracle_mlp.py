@@ -56,3 +56,20 @@ def test_gradient_finite_differences():
         pm[j] -= h
         fd = (M.Net(pp, bf=False).loss_grad(f, y)[0] - M.Net(pm, bf=False).loss_grad(f, y)[0]) / (2 * h)
         assert abs(fd - g[j]) < 1e-6 * max(1.0, abs(g[j])) + 1e-9, (j, fd, g[j])
+
+
+def test_zero_output_scale_gives_zero_target():
+    """an output column with scale 0 (all zero in the pool) is trained towards
+    0: the loss equals the one with that column's label set to anything"""
+    p = random_params(3)
+    p[M.NPARAM + 16 + 7] = 0.0
+    rng = np.random.default_rng(3)
+    f = rng.standard_normal((64, 12))
+    y = rng.standard_normal((64, 9))
+    net = M.Net(p, bf=False)
+    l0, g0 = net.loss_grad(f, y)
+    y[:, 7] = 0.0
+    l1, g1 = net.loss_grad(f, y)
+    assert np.isfinite(l0) and l0 > 0
+    assert l0 == l1
+    assert np.array_equal(g0, g1)
