@@ -95,18 +95,19 @@ template <int S, typename T> __device__ __forceinline__ void dft8(cpx<T> *v)
     dft4<S>(e0, e1, e2, e3);
     dft4<S>(o0, o1, o2, o3);
     const T h = (T)0.70710678118654752440084436210484903928483593768847;
-    // w^1 = h(1 + S i), w^2 = S i, w^3 = h(-1 + S i)
-    cpx<T> w1o1 = {h * (o1.x - (T)S * o1.y), h * (o1.y + (T)S * o1.x)};
+    // w^1 = h(1 + S i), w^2 = S i, w^3 = h(-1 + S i); the h scalings are
+    // folded into the output sums: e +- h a is one FMA per component
+    const cpx<T> a1 = {o1.x - (T)S * o1.y, o1.y + (T)S * o1.x};    // w^1 o1 / h
+    const cpx<T> a3 = {o3.x + (T)S * o3.y, o3.y - (T)S * o3.x};    // -w^3 o3 / h
     cpx<T> w2o2 = mul_si<S>(o2);
-    cpx<T> w3o3 = {h * (-o3.x - (T)S * o3.y), h * (-o3.y + (T)S * o3.x)};
     v[0] = e0 + o0;
     v[4] = e0 - o0;
-    v[1] = e1 + w1o1;
-    v[5] = e1 - w1o1;
+    v[1] = {fma(h, a1.x, e1.x), fma(h, a1.y, e1.y)};
+    v[5] = {fma(-h, a1.x, e1.x), fma(-h, a1.y, e1.y)};
     v[2] = e2 + w2o2;
     v[6] = e2 - w2o2;
-    v[3] = e3 + w3o3;
-    v[7] = e3 - w3o3;
+    v[3] = {fma(-h, a3.x, e3.x), fma(-h, a3.y, e3.y)};
+    v[7] = {fma(h, a3.x, e3.x), fma(h, a3.y, e3.y)};
 }
 
 // LDS ordering between passes.  When every column of the tile belongs to a
@@ -276,6 +277,68 @@ template <typename T, int N, int EPT> struct Tile {
         Tw cur;
         tw_fetch<p, S>(tw, j0, cur);
         run_passes<S, p, TWPF>(lds, tw, c, j0, v, cur);
+    }
+
+    // Twiddles of every pass p >= 1 held in registers across transforms.  A
+    // thread's twiddles depend on j0 only, so a kernel that runs many
+    // transforms per thread reads and expands them once; the inverse
+    // direction multiplies by their conjugates.  Slot s = m*R + r of pass p
+    // holds the forward twiddle of butterfly m, power r (r >= 1).
+    struct TwRegs {
+        cpx<T> w[SC::P][EPT];
+    };
+    __device__ static __forceinline__ void tw_regs(const cpx<T> *__restrict__ tw, int j0, TwRegs &t)
+    {
+        tw_regs_pass<1>(tw, j0, t);
+    }
+    template <int p>
+    __device__ static __forceinline__ void tw_regs_pass(const cpx<T> *__restrict__ tw, int j0, TwRegs &t)
+    {
+        if constexpr (p < SC::P) {
+            constexpr int R = SC::radix(p);
+            constexpr int Ns = SC::ns(p);
+#pragma unroll
+            for (int m = 0; m < EPT / R; m++) {
+                const int jl = (j0 + m * TPC) & (Ns - 1);
+                cpx<T> w[8];
+                twiddles<R, -1>(tw, jl * (N / (Ns * R)), w);
+#pragma unroll
+                for (int r = 1; r < R; r++) t.w[p][m * R + r] = w[r];
+            }
+            tw_regs_pass<p + 1>(tw, j0, t);
+        }
+    }
+    // a * w (S < 0) or a * conj(w) (S > 0)
+    template <int S> __device__ static __forceinline__ cpx<T> twmul(cpx<T> a, cpx<T> w)
+    {
+        if constexpr (S < 0) return cmul(a, w);
+        else return {a.x * w.x + a.y * w.y, a.y * w.x - a.x * w.y};
+    }
+    template <int p, int S> __device__ static __forceinline__ void compute_r(const TwRegs &t, cpx<T> *v)
+    {
+        constexpr int R = SC::radix(p);
+        constexpr int Ns = SC::ns(p);
+#pragma unroll
+        for (int m = 0; m < EPT / R; m++) {
+            if constexpr (Ns > 1) {
+#pragma unroll
+                for (int r = 1; r < R; r++) v[m * R + r] = twmul<S>(v[m * R + r], t.w[p][m * R + r]);
+            }
+            dftR<R, S>(v + m * R);
+        }
+    }
+    // run_regs with the register twiddles
+    template <int S, int p = 0>
+    __device__ static __forceinline__ void run_regs(cpx<T> *lds, const TwRegs &t, int c, int j0, cpx<T> *v)
+    {
+        compute_r<p, S>(t, v);
+        if constexpr (p + 1 < SC::P) {
+            lds_write<p>(lds, c, j0, v);
+            tile_sync<WAVE>();
+            lds_read<p + 1>(lds, c, j0, v);
+            tile_sync<WAVE>();
+            run_regs<S, p + 1>(lds, t, c, j0, v);
+        }
     }
 
     // LDS -> registers (pass-0 inputs) -> FFT -> registers (final outputs).

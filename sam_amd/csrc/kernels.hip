@@ -568,13 +568,25 @@ __device__ __forceinline__ void xrow_stats(const cpx<T> (&uf)[LU][EPT], double (
     }
 }
 
+// one x transform with the twiddle table (LDS) or with register twiddles
+template <int S, typename TL, typename T>
+__device__ __forceinline__ void xrow_fft(cpx<T> *lds, const cpx<T> *tw, int c, int j0, cpx<T> *v)
+{
+    TL::template run_regs<S, 0, false>(lds, tw, c, j0, v);
+}
+template <int S, typename TL, typename T>
+__device__ __forceinline__ void xrow_fft(cpx<T> *lds, const typename TL::TwRegs &tw, int c, int j0, cpx<T> *v)
+{
+    TL::template run_regs<S>(lds, tw, c, j0, v);
+}
+
 // The LR products of the transformed fields, forward x transforms and the
 // R2C stores.  DBUF: products alternate between the FFT tile and the (idle,
 // at least as large) staging buffer, so the store of product q-1 and the
 // transform of product q overlap and one barrier per product suffices.
 // KEPT: kx_max = N/3, outputs in the truncated band are not formed.
-template <typename T, int N, int CP, int LU, int LR, bool DBUF, bool KEPT, int EPT = 8>
-__device__ __forceinline__ void xrow_products(cpx<T> *lds, cpx<T> *stg, const cpx<T> *tw, const cpx<T> (&uf)[LU][EPT],
+template <typename T, int N, int CP, int LU, int LR, bool DBUF, bool KEPT, int EPT = 8, typename TWS = const cpx<T> *>
+__device__ __forceinline__ void xrow_products(cpx<T> *lds, cpx<T> *stg, const TWS &tw, const cpx<T> (&uf)[LU][EPT],
                                               cpx<T> *__restrict__ Yz, long long fstride, const Dims &d, int y0, int c,
                                               int j0)
 {
@@ -596,7 +608,7 @@ __device__ __forceinline__ void xrow_products(cpx<T> *lds, cpx<T> *stg, const cp
         } else {
             lds_barrier();
         }
-        TL::template run_regs<-1, 0, false>(tile, tw, c, j0, pi);
+        xrow_fft<-1, TL>(tile, tw, c, j0, pi);
         if constexpr (KEPT) xrow_store_kept<T, N, EPT>(tile, c, j0, pi);
         else TL::store_natural(tile, c, j0, pi);
         lds_barrier();
@@ -618,13 +630,14 @@ __global__ __launch_bounds__(CP * N / EPT, 2) void k_xrow_p(T *__restrict__ Yr, 
     constexpr int TILE = CP * N, SR = xrow_srows<N>();
     __shared__ cpx<T> smem[TILE + 2 * SR * 2 * CP];
     __shared__ double red[NW][3 * LU];
-    __shared__ cpx<T> tw[N / 2];
-    for (int e = threadIdx.x; e < N / 2; e += NT) tw[e] = twg[e];
     cpx<T> *const lds = smem;
     auto stage = [&](int i) { return smem + TILE + (i & 1) * (SR * 2 * CP); };
     const int nyt = d.ny / (2 * CP);
     const int c = threadIdx.x / TL::TPC, j0 = threadIdx.x % TL::TPC;
     const int mine = xrow_stage_count<N, NT>(threadIdx.x >> 6);
+    // the thread's twiddles, read once for all 13 transforms of every group
+    typename TL::TwRegs tw;
+    TL::tw_regs(twg, j0, tw);
     cpx<T> *Y = reinterpret_cast<cpx<T> *>(Yr);
     const long long fstride = d.yfield;
     auto zoff = [&](int b) { return (long long)(b / nyt) * d.nxm * d.ny; };
@@ -656,7 +669,7 @@ __global__ __launch_bounds__(CP * N / EPT, 2) void k_xrow_p(T *__restrict__ Yr, 
                 if (g < LU) xrow_issue_stage_ds<T, N, CP, NT>(sf, Yz + g * fstride, d.ny, y0);
                 else xrow_issue_stage_ds<T, N, CP, NT>(sf, Y + zoff(bn) + (g - LU) * fstride, d.ny, yof(bn));
             }
-            TL::template run_regs<+1, 0, false>(lds, tw, c, j0, uf[f]);
+            xrow_fft<+1, TL>(lds, tw, c, j0, uf[f]);
         }
         if (want_stats) xrow_stats<T, NT, LU, EPT>(uf, red, partials + (long long)b * 3 * LU);
         xrow_products<T, N, CP, LU, LR, false, true, EPT>(lds, nullptr, tw, uf, Yz, fstride, d, y0, c, j0);
