@@ -3,10 +3,10 @@
 //
 // Each thread owns EPT (8 or 16) elements of one column per pass and keeps
 // them in VGPRs; passes exchange data through an LDS tile of C*N complex
-// elements.  The LDS address of element p of column c is XOR-swizzled in
-// 128-byte groups so that the strided Stockham reads/writes and the
-// column-interleaved cooperative loads are free of bank conflicts without
-// padding (the tile stays exactly C*N elements).
+// elements.  The LDS address of element p of column c is XOR-swizzled within
+// 128-byte groups (lds_idx) so that the strided Stockham reads/writes and the
+// column-interleaved cooperative loads avoid bank conflicts without padding
+// (the tile stays exactly C*N elements).
 //
 // Sign convention (FFTPACK, fft.f:57-59,171-173): forward = exp(-i..),
 // backward = exp(+i..), both unnormalised.  Twiddles come from a table
@@ -59,16 +59,37 @@ template <int N> struct Sched {
     __host__ __device__ static constexpr int ns(int p) { return p == 0 ? 1 : ns(p - 1) * radix(p - 1); }
 };
 
-template <typename T> struct Swz;
-template <> struct Swz<double> { static constexpr int SW = 8, LOG = 3; };   // 16-byte elements
-template <> struct Swz<float> { static constexpr int SW = 16, LOG = 4; };   // 8-byte elements
+// rotate the low B bits of c left by one
+template <int B> __device__ __forceinline__ int rotl_bits(int c)
+{
+    c &= (1 << B) - 1;
+    return ((c << 1) & ((1 << B) - 1)) | (c >> (B - 1));
+}
 
 // LDS address of element p of column c in a tile with column length N.
+// Element p stays in its aligned group of SW elements (SW x element size =
+// 128 bytes, all 32 banks); its slot in the group is p XOR a key of the
+// higher bits of p and of the column.  The keys were chosen, per element
+// size and N, so that the Stockham exchanges of every pass, natural-order
+// and transposed tile accesses and the x-row two-row pairs (columns c and
+// k, k + 1 in one access group) hit distinct banks (LDS model: 32 banks of
+// 4 bytes; an access of W bytes per lane serves 128/W lanes per cycle).
+// The column term is rotated so that a group of columns at two adjacent k
+// (the x-row pairs) still covers every bank.
 template <typename T, int N>
 __device__ __forceinline__ int lds_idx(int c, int p)
 {
-    constexpr int SW = Swz<T>::SW, LOG = Swz<T>::LOG;
-    return c * N + (p ^ (((p >> LOG) ^ c) & (SW - 1)));
+    if constexpr (sizeof(T) == 8) {   // 16-byte elements, 8 per group
+        return c * N + (p ^ (((p >> 3) ^ rotl_bits<3>(c)) & 7));
+    } else {                          // 8-byte elements, 16 per group
+        int key;
+        if constexpr (N == 128) key = (p >> 3) ^ ((p >> 5) << 1) ^ rotl_bits<4>(rotl_bits<4>(rotl_bits<4>(c)));
+        else if constexpr (N == 256) key = (p >> 2) ^ (p >> 5) ^ rotl_bits<4>(c);
+        else if constexpr (N == 512) key = (p >> 3) ^ rotl_bits<4>(c);
+        else if constexpr (N == 1024) key = (p >> 4) ^ (p >> 5) ^ rotl_bits<4>(c);
+        else key = (p >> 4) ^ c;
+        return c * N + (p ^ (key & 15));
+    }
 }
 
 template <int S, typename T> __device__ __forceinline__ void dft2(cpx<T> &a, cpx<T> &b)
@@ -214,6 +235,55 @@ template <typename T, int N, int EPT> struct Tile {
         }
     }
 
+    // Twiddle table in LDS, laid out without bank conflicts.  The plain
+    // table tw[k] = exp(-2 pi i k/N) read at e0, 2 e0, 4 e0 puts the lanes of
+    // an access group on one bank in the early passes (e0 = jl N/(Ns R) is a
+    // multiple of a power of two).  Here pass p >= 1 stores the bases w^1,
+    // w^2 (R >= 4) and w^4 (R = 8) of its Ns twiddle rows as runs of Ns
+    // consecutive entries, so consecutive jl read consecutive entries.  At
+    // most N/2 entries.
+    __host__ __device__ static constexpr int twl_nb(int p) { return SC::radix(p) == 8 ? 3 : SC::radix(p) == 4 ? 2 : 1; }
+    __host__ __device__ static constexpr int twl_off(int p)
+    {
+        return p <= 1 ? 0 : twl_off(p - 1) + twl_nb(p - 1) * SC::ns(p - 1);
+    }
+    static constexpr int TWL_SIZE = twl_off(SC::P);
+    static_assert(TWL_SIZE <= N / 2, "LDS twiddle table fits N/2 entries");
+    struct TwLds {
+        const cpx<T> *t;
+    };
+    // fill the LDS table from tw[k] (k < N/2); the caller barriers before use
+    template <int p = 1>
+    __device__ static __forceinline__ void twl_fill(const cpx<T> *__restrict__ twg, cpx<T> *twl, int tid, int nthreads)
+    {
+        if constexpr (p < SC::P) {
+            constexpr int Ns = SC::ns(p), R = SC::radix(p), off = twl_off(p);
+            for (int e = tid; e < twl_nb(p) * Ns; e += nthreads)
+                twl[off + e] = twg[(1 << (e / Ns)) * (e % Ns) * (N / (Ns * R))];
+            twl_fill<p + 1>(twg, twl, tid, nthreads);
+        }
+    }
+    template <int p, int S> __device__ static __forceinline__ void tw_fetch(TwLds tl, int j0, Tw &t)
+    {
+        constexpr int R = SC::radix(p);
+        constexpr int Ns = SC::ns(p);
+        if constexpr (Ns > 1) {
+            constexpr int off = twl_off(p);
+            auto ld = [&](int e) {
+                cpx<T> w = tl.t[e];
+                if constexpr (S > 0) w.y = -w.y;
+                return w;
+            };
+#pragma unroll
+            for (int m = 0; m < EPT / R; m++) {
+                const int jl = (j0 + m * TPC) & (Ns - 1);
+                t.w[m][1] = ld(off + jl);
+                if constexpr (R >= 4) t.w[m][2] = ld(off + Ns + jl);
+                if constexpr (R == 8) t.w[m][4] = ld(off + 2 * Ns + jl);
+            }
+        }
+    }
+
     // twiddle + butterflies of pass p
     template <int p, int S> __device__ static __forceinline__ void compute(Tw &t, cpx<T> *v)
     {
@@ -248,9 +318,8 @@ template <typename T, int N, int EPT> struct Tile {
     // next pass's twiddles before this pass's LDS exchange (for a twiddle
     // table in global memory, whose latency the exchange then covers); with
     // an LDS table they are read at the pass itself, holding fewer VGPRs.
-    template <int S, int p, bool TWPF>
-    __device__ static __forceinline__ void run_passes(cpx<T> *lds, const cpx<T> *__restrict__ tw, int c, int j0,
-                                                      cpx<T> *v, Tw &cur)
+    template <int S, int p, bool TWPF, typename TWS>
+    __device__ static __forceinline__ void run_passes(cpx<T> *lds, TWS tw, int c, int j0, cpx<T> *v, Tw &cur)
     {
         if constexpr (p + 1 < SC::P) {
             Tw nxt;
@@ -271,8 +340,8 @@ template <typename T, int N, int EPT> struct Tile {
     // inputs; the final pass leaves its outputs in v (positions kout).
     // Requires a __syncthreads() before the call if other threads may still
     // access the LDS tile region this thread's column uses.
-    template <int S, int p = 0, bool TWPF = true>
-    __device__ static __forceinline__ void run_regs(cpx<T> *lds, const cpx<T> *__restrict__ tw, int c, int j0, cpx<T> *v)
+    template <int S, int p = 0, bool TWPF = true, typename TWS = const cpx<T> *>
+    __device__ static __forceinline__ void run_regs(cpx<T> *lds, TWS tw, int c, int j0, cpx<T> *v)
     {
         Tw cur;
         tw_fetch<p, S>(tw, j0, cur);
@@ -345,8 +414,8 @@ template <typename T, int N, int EPT> struct Tile {
     // Touches only column c of the tile; when WAVE, other waves may still be
     // working on their columns at return, so a caller that next writes other
     // columns must __syncthreads() first.
-    template <int S>
-    __device__ static __forceinline__ void run_from_lds(cpx<T> *lds, const cpx<T> *__restrict__ tw, int c, int j0, cpx<T> *v)
+    template <int S, typename TWS = const cpx<T> *>
+    __device__ static __forceinline__ void run_from_lds(cpx<T> *lds, TWS tw, int c, int j0, cpx<T> *v)
     {
         lds_read<0>(lds, c, j0, v);
         tile_sync<WAVE>();

@@ -385,7 +385,7 @@ __device__ __forceinline__ void xrow_unpack_stage(cpx<T> *lds, const cpx<T> *stg
 // not drain it before every tile read.
 template <typename T, int N, int CP, int LU>
 __device__ __forceinline__ void xrow_phase_a(cpx<T> *__restrict__ lds, cpx<T> *__restrict__ stg,
-                                             const cpx<T> *__restrict__ tw, const cpx<T> *__restrict__ Yz,
+                                             typename Tile<T, N, 8>::TwLds tw, const cpx<T> *__restrict__ Yz,
                                              long long fstride, int ny, int y0, int kxm, int c, int j0,
                                              cpx<T> (&uf)[LU][8])
 {
@@ -510,7 +510,7 @@ __device__ __forceinline__ void xrow_store_kept(cpx<T> *lds, int c, int j0, cons
 }
 
 template <typename T, int N, int CP, int LU>
-__device__ __forceinline__ void xrow_phase_a_ds(cpx<T> *lds, cpx<T> *stg, const cpx<T> *tw,
+__device__ __forceinline__ void xrow_phase_a_ds(cpx<T> *lds, cpx<T> *stg, typename Tile<T, N, 8>::TwLds tw,
                                                 const cpx<T> *__restrict__ Yz, long long fstride, int ny, int y0,
                                                 int kxm, int c, int j0, cpx<T> (&uf)[LU][8])
 {
@@ -568,9 +568,9 @@ __device__ __forceinline__ void xrow_stats(const cpx<T> (&uf)[LU][EPT], double (
     }
 }
 
-// one x transform with the twiddle table (LDS) or with register twiddles
+// one x transform with the LDS twiddle table or with register twiddles
 template <int S, typename TL, typename T>
-__device__ __forceinline__ void xrow_fft(cpx<T> *lds, const cpx<T> *tw, int c, int j0, cpx<T> *v)
+__device__ __forceinline__ void xrow_fft(cpx<T> *lds, typename TL::TwLds tw, int c, int j0, cpx<T> *v)
 {
     TL::template run_regs<S, 0, false>(lds, tw, c, j0, v);
 }
@@ -585,7 +585,7 @@ __device__ __forceinline__ void xrow_fft(cpx<T> *lds, const typename TL::TwRegs 
 // at least as large) staging buffer, so the store of product q-1 and the
 // transform of product q overlap and one barrier per product suffices.
 // KEPT: kx_max = N/3, outputs in the truncated band are not formed.
-template <typename T, int N, int CP, int LU, int LR, bool DBUF, bool KEPT, int EPT = 8, typename TWS = const cpx<T> *>
+template <typename T, int N, int CP, int LU, int LR, bool DBUF, bool KEPT, int EPT, typename TWS>
 __device__ __forceinline__ void xrow_products(cpx<T> *lds, cpx<T> *stg, const TWS &tw, const cpx<T> (&uf)[LU][EPT],
                                               cpx<T> *__restrict__ Yz, long long fstride, const Dims &d, int y0, int c,
                                               int j0)
@@ -697,11 +697,12 @@ __global__ __launch_bounds__(CP * N / 8, xrow_waves<T>()) void k_xrow(T *__restr
     static_assert(NT < 64 || STG >= TILE, "staging buffer holds a tile");
     __shared__ cpx<T> smem[TILE + STG];
     __shared__ double red[NW][3 * LU];
-    // the x twiddles (the passes read indices < N/2) in LDS: the transforms then
-    // issue no vector-memory reads, so the staged loads and the product stores
-    // stay in flight across them instead of being drained by a twiddle wait
-    __shared__ cpx<T> tw[N / 2];
-    for (int e = threadIdx.x; e < N / 2; e += NT) tw[e] = twg[e];
+    // the x twiddles in LDS: the transforms then issue no vector-memory reads,
+    // so the staged loads and the product stores stay in flight across them
+    // instead of being drained by a twiddle wait (bank-conflict-free layout)
+    __shared__ cpx<T> twt[TL::TWL_SIZE];
+    TL::twl_fill(twg, twt, threadIdx.x, NT);
+    const typename TL::TwLds tw{twt};
     cpx<T> *const lds = smem, *const stg = smem + TILE;
     const int nyt = d.ny / (2 * CP);
     const int yt = blockIdx.x % nyt, z = blockIdx.x / nyt;
