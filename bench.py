@@ -205,6 +205,36 @@ def les_comparison(S, ic, n_dns, n_les=128, les_steps=2000, refresh=20):
     }
 
 
+def les_dynamic(S, ic, n, steps, warmup):
+    """The stratified LES with the dynamic model (dm.f math, (1,4,1) test
+    filter) at n^3: seeded k0 = 8 IC, N = 8, nu = 1e-5, FP32 transforms and
+    products, FP64 state; `steps` timed steps after `warmup`.  Reports the
+    step time and the split over kernel classes (HIP events)."""
+    sam = S.Sam(S.make_params(n, sgs=S.SGS_DYN, i_strat=1, grav=64.0, to=1.0, lapse=1.0, vis=1e-5, pr=1.0,
+                              i_cfl=1, cfl0=0.5, dt0=0.01, precision=S.PREC_SPLIT))
+    sam.set_state(ic.random_solenoidal(n, k0=8.0, seed=20261203, i_strat=1))
+    sam.step(warmup)
+    sam.sync()
+    sam.prof_enable(True)
+    t0 = time.perf_counter()
+    d = sam.step(steps)
+    sam.sync()
+    wall = time.perf_counter() - t0
+    prof = sam.prof_read()
+    sam.close()
+    tot = sum(v[0] for v in prof.values()) or 1.0
+    return {
+        "workload": f"stratified LES {n}^3 with the dynamic model, k0=8, N=8, nu=1e-5, FP32 transforms / FP64 state",
+        "ms_per_step": round(wall / steps * 1e3, 3),
+        "value": round(n ** 3 * steps / wall / 1e6, 2),
+        "unit": "Mpoint-steps/s",
+        "steps": steps,
+        "warmup": warmup,
+        "c_delta_sq": float(d["c_delta_sq"]),
+        "kernel_share": {k: round(v[0] / tot, 3) for k, v in prof.items() if v[1]},
+    }
+
+
 def big_split(S, ic, u_coarse, n_big, steps, warmup):
     """One or more RK3 steps of the FP32-split DNS at n_big^3 (the state of
     the bench run, spectrally padded), timed like the main runs."""
@@ -252,6 +282,7 @@ def main():
     ap.add_argument("--no-les", action="store_true")
     ap.add_argument("--no-big", action="store_true", help="skip the 1024^3 FP32-split step")
     ap.add_argument("--big-n", type=int, default=1024)
+    ap.add_argument("--no-les512", action="store_true", help="skip the 512^3 dynamic-model LES step")
     ap.add_argument("--les-steps", type=int, default=2000,
                     help="steps of each 128^3 LES run (a short count for profiler runs)")
     ap.add_argument("--cpu-n", type=int, default=0, help="CPU baseline grid (default: same n if RAM allows)")
@@ -350,6 +381,11 @@ def main():
         lr = out["les_128"]
         log(f"[bench] LES dyn {lr['dynamic_ms_per_step']} ms, learned {lr['learned_ms_per_step']} ms, "
             f"E(k) err {lr['ek_rel_err_learned_vs_dynamic']}")
+
+    if not args.no_les512 and n >= 512:
+        log("[bench] LES 512^3 with the dynamic model")
+        out["les_512_dynamic"] = les_dynamic(S, ic, 512, 10, 2)
+        log(f"[bench] LES 512^3 dynamic: {out['les_512_dynamic']['ms_per_step']} ms/step")
 
     if not args.no_big and args.big_n > n:
         log(f"[bench] FP32-split {args.big_n}^3")

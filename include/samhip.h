@@ -119,6 +119,27 @@ int  sam_rhs(sam_handle *h, double *r, int ldz, int want_spectra, sam_diag *diag
  * div_rms div_max  (5*lm + 3 doubles). */
 int  sam_get_spectra(sam_handle *h, double *out);
 
+/* ---- per-step extras of sam.f ------------------------------------------ */
+/* force.f (i_force = 1, sam.f:320-322): after every step of sam_step, the
+ * u, v, w Fourier coefficients of shells 2..k_force+1 (1-based shells
+ * is = 1 + int(|k| + 0.5)) are scaled by sqrt(ek_0(is)/ek(is)), so the
+ * low-shell spectrum stays at ek_0, the ek of the last spectra taken by
+ * sam_step / sam_rhs (sam.f:250).  Before any spectrum the energies at the
+ * first forced step become ek_0.  k_force < 0 switches forcing off;
+ * 0 <= k_force <= max(Nx,Ny,Nz)/2 as check_inputs.f:215 requires. */
+int  sam_set_forcing(sam_handle *h, int k_force);
+/* Chollet-Lesieur model (i_les = 1): the energy of the band around
+ * k_cl = int(Nx/3) is taken at the first substep of every step
+ * (cl_energy.f, sam.f:128-132) and nu_t(k) = sqrt(2 E_cl/k_cl) (0.145 +
+ * 5.01 exp(-3.03 k_cl/|k|)) (spectra.f:39-43, init.f:88-93) enters
+ *   mode 1: the spectra only (dk, vt), as SAM does;
+ *   mode 2: also the integrating factor: exp(-(nu + nu_t(k)) k^2 dt) for
+ *           u, v, w and exp(-(nu/Pr + nu_t(k)/Pr_t) k^2 dt) for T' (an
+ *           extension: SAM never couples the model into the step).
+ * mode 0 switches it off.  SAM_ERR_UNSUPPORTED when k_cl exceeds the
+ * stored modes of a direction. */
+int  sam_set_cl(sam_handle *h, int mode);
+
 /* ---- stage entry points (parity with the reference's routines) -------- */
 /* z_trans_b.f (case 0): u (ldz,Ny_min,Nx_min,nvar) -> out (Nz,Ny_min,Nx_min,nvar).
  * The reference scatters to (Ny_min,Nx_min,Nz,nvar); this is the same data

@@ -33,6 +33,7 @@ class OrcCfg(ctypes.Structure):
         ("to", ctypes.c_double), ("lapse0", ctypes.c_double), ("lapse", ctypes.c_double),
         ("cfl0", ctypes.c_double), ("dt0", ctypes.c_double),
         ("c_smag", ctypes.c_double), ("pr_t", ctypes.c_double),
+        ("k_force", ctypes.c_int), ("i_cl", ctypes.c_int),
     ]
 
 
@@ -64,7 +65,13 @@ def lib():
         L.orc_to_physical.argtypes = [ctypes.c_void_p, _dp, _dp]
         L.orc_rhs.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_double, _dp]
         L.orc_step.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int, _dp, _dp]
-        L.orc_spectra.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
+        L.orc_step_ext.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int, _dp, _dp, _dp]
+        L.orc_spectra.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_double, _dp]
+        L.orc_cl_energy.restype = ctypes.c_double
+        L.orc_cl_energy.argtypes = [ctypes.c_void_p, _dp]
+        L.orc_cl_fac.restype = ctypes.c_double
+        L.orc_cl_fac.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        L.orc_force.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
         _lib = L
     return _lib
 
@@ -99,14 +106,15 @@ def _p(a):
 
 def make_cfg(nx, ny=None, nz=None, *, xl=2 * np.pi, yl=None, zl=None, n_dealias=1,
              i_strat=0, nrk_max=3, i_cfl=0, sgs=0, vis=0.0025, pr=1.0, grav=0.0,
-             to=1.0, lapse0=0.0, lapse=None, cfl0=0.5, dt0=0.01, c_smag=0.01, pr_t=1.0):
+             to=1.0, lapse0=0.0, lapse=None, cfl0=0.5, dt0=0.01, c_smag=0.01, pr_t=1.0,
+             k_force=-1, i_cl=0):
     ny = nx if ny is None else ny
     nz = nx if nz is None else nz
     yl = xl if yl is None else yl
     zl = xl if zl is None else zl
     lapse = lapse0 if lapse is None else lapse
     return OrcCfg(nx, ny, nz, n_dealias, i_strat, nrk_max, i_cfl, sgs, xl, yl, zl,
-                  vis, pr, grav, to, lapse0, lapse, cfl0, dt0, c_smag, pr_t)
+                  vis, pr, grav, to, lapse0, lapse, cfl0, dt0, c_smag, pr_t, k_force, i_cl)
 
 
 class Oracle:
@@ -223,18 +231,39 @@ class Oracle:
         self.L.orc_rhs(self.h, _p(u), _p(r), self.cfg.dt0 if dt_in is None else dt_in, _p(st))
         return r, st
 
-    def step(self, u, nsteps, time=0.0, dt=None):
+    def step(self, u, nsteps, time=0.0, dt=None, ek0=None):
+        """nsteps RK steps; with cfg.i_cl the Chollet-Lesieur energy of every
+        step (st[18] = fac_cl of the last one), with cfg.k_force >= 0 and ek0
+        given, force.f after every step toward ek0"""
         u = np.array(u, order="F", copy=True)
         state = np.array([time, self.cfg.dt0 if dt is None else dt])
-        st = np.zeros(18)
-        self.L.orc_step(self.h, _p(u), nsteps, _p(state), _p(st))
+        st = np.zeros(19)
+        if ek0 is None:
+            self.L.orc_step(self.h, _p(u), nsteps, _p(state), _p(st))
+        else:
+            ek0 = np.ascontiguousarray(ek0, dtype=np.float64)
+            self.L.orc_step_ext(self.h, _p(u), nsteps, _p(state), _p(st), _p(ek0))
         return u, state[0], state[1], st
 
-    def spectra(self, u, r):
+    def cl_energy(self, u):
+        """E_cl of cl_energy.f and fac_cl = sqrt(2 E_cl / k_cl)"""
+        u = np.asfortranarray(u)
+        e = self.L.orc_cl_energy(self.h, _p(u))
+        return e, self.L.orc_cl_fac(self.h, e)
+
+    def force(self, u, ek0):
+        """force.f on a copy of u; returns (u, shell energies before)"""
+        u = np.array(u, order="F", copy=True)
+        ek0 = np.ascontiguousarray(ek0, dtype=np.float64)
+        ekg = np.zeros(max(self.cfg.k_force, 0) + 2)
+        self.L.orc_force(self.h, _p(u), _p(ek0), _p(ekg))
+        return u, ekg[: self.cfg.k_force + 1]
+
+    def spectra(self, u, r, fac_cl=0.0):
         u = np.asfortranarray(u)
         r = np.asfortranarray(r)
         out = np.zeros(5 * self.lm + 3)
-        self.L.orc_spectra(self.h, _p(u), _p(r), _p(out))
+        self.L.orc_spectra(self.h, _p(u), _p(r), fac_cl, _p(out))
         lm = self.lm
         return dict(ek=out[:lm], tk=out[lm:2 * lm], dk=out[2 * lm:3 * lm],
                     vt=out[3 * lm:4 * lm], sample=out[4 * lm:5 * lm],

@@ -187,6 +187,8 @@ typedef struct {
     double cfl0, dt0;
     double c_smag;   /* sm.f: nu_t = c_smag * Delta^2 * |S| */
     double pr_t;     /* turbulent Prandtl number for the SGS scalar flux */
+    int k_force;     /* force.f after every step when >= 0 (i_force = 1) */
+    int i_cl;        /* Chollet-Lesieur: 0 off, 1 diagnostics (i_les = 1), 2 nu_t(k) in the IF */
 } orc_cfg;
 
 typedef struct {
@@ -197,6 +199,7 @@ typedef struct {
     double buoy_fac_x, buoy_fac_z, pr_inv;
     double delta_sq;
     int *ky, *kz;          /* integer wavenumber maps, fft_init.f:56-72 */
+    double a_cl, b_cl, c_cl; int k_cl;   /* init.f:88-93 */
     fftplan px, pxh, py, pz;
     int nthreads;
 } orc;
@@ -237,6 +240,8 @@ orc *orc_create(const orc_cfg *cfg)
         o->adv_cfl_limit = sqrt(3.0);
     }
     o->pr_inv = 1.0 / cfg->pr;
+    o->a_cl = 0.145; o->b_cl = 5.01; o->c_cl = 3.03;   /* init.f:88-93 */
+    o->k_cl = (int)(cfg->nx / 3.0);
     o->buoy_fac_x = 0.0;
     o->buoy_fac_z = cfg->i_strat ? cfg->grav / cfg->to : 0.0;
     double dx = cfg->xl / cfg->nx, dy = cfg->yl / cfg->ny, dz = cfg->zl / cfg->nz;
@@ -773,11 +778,118 @@ void orc_rhs(const orc *o, const cx *u, cx *r, double dt_in, double *stats)
     ws_free(&w);
 }
 
-/* nsteps full RK steps of sam.f:122-326 (no I/O, no forcing).
+/* ------------------------------------------------------------------ */
+/* Chollet-Lesieur energy and eddy viscosity, spectral forcing           */
+/* ------------------------------------------------------------------ */
+
+/* cl_energy.f:8-43: energy of the modes in the shell band around k_cl,
+ * walked as the reference does: per (kx, ky) column the kz index range
+ * k_start..k_stop (1-based) and its mirror row kk = Nz_min + 2 - k.  For
+ * k = 1 the mirror is row Nz_min + 1, past the stored modes: it adds the
+ * (zero) padding row of u(Nze, ...). */
+double orc_cl_energy(const orc *o, const cx *u)
+{
+    const int nxm = o->nxm, nym = o->nym, nzm = o->nzm;
+    const size_t ss = (size_t)nzm * nym * nxm;
+    const double start_sq = ((double)o->k_cl - 0.5) * ((double)o->k_cl - 0.5);
+    const double stop_sq = ((double)o->k_cl + 0.5) * ((double)o->k_cl + 0.5);
+    double sum_e = 0.0;
+    for (int ix = 0; ix < nxm; ix++) {
+        const double factor = (ix == 0 || ix == o->c.nx / 2) ? 1.0 : 2.0;
+        const int kx2 = ix * ix;
+        for (int iy = 0; iy < nym; iy++) {
+            const int k_sq2 = kx2 + o->ky[iy] * o->ky[iy];
+            const int k_start = (k_sq2 > start_sq) ? 1 : (int)sqrt(start_sq - k_sq2) + 2;
+            const int k_stop = (k_sq2 > stop_sq) ? 0 : (int)sqrt(stop_sq - k_sq2) + 1;
+            for (int k = k_start; k <= k_stop; k++) {
+                const int kk = nzm + 2 - k;
+                double usq1 = 0, usq2 = 0;
+                for (int n = 0; n < 3; n++) {
+                    const cx a = u[(k - 1) + (size_t)nzm * (iy + (size_t)nym * ix) + n * ss];
+                    usq1 += creal(a * conj(a));
+                    if (kk <= nzm) {
+                        const cx b = u[(kk - 1) + (size_t)nzm * (iy + (size_t)nym * ix) + n * ss];
+                        usq2 += creal(b * conj(b));
+                    }
+                }
+                sum_e += factor * (usq1 + usq2);
+            }
+        }
+    }
+    return sum_e;
+}
+
+/* sam.f:131 / spectra.f:21 */
+double orc_cl_fac(const orc *o, double e_cl) { return sqrt(2.0 * e_cl / (double)o->k_cl); }
+
+/* spectra.f:39-43: nu_t(k) of the Chollet-Lesieur model, k = |k_int| */
+static double cl_vist(const orc *o, double fac_cl, int ksq)
+{
+    if (ksq == 0) return 0.0;
+    return fac_cl * (o->a_cl + o->b_cl * exp(-(o->c_cl * o->k_cl) / sqrt((double)ksq)));
+}
+
+/* force.f:12-63: energy of u, v, w in the shells is <= k_force + 1
+ * (is = 1 + int(|k| + 0.5), factor 2 for kx > 0) scaled back to ek0 (the
+ * spectrum of the last spectra call, ek of spectra.f); shell 1 (the mean)
+ * is left alone.  ekg (>= k_force + 1 doubles) receives the energies before
+ * the scaling. */
+void orc_force(const orc *o, cx *u, const double *ek0, double *ekg)
+{
+    const int nxm = o->nxm, nym = o->nym, nzm = o->nzm, kf = o->c.k_force;
+    const size_t ss = (size_t)nzm * nym * nxm;
+    double *work = (double *)calloc((size_t)kf + 2, sizeof(double));
+    for (int pass = 0; pass < 2; pass++) {
+        for (int ix = 0; ix < nxm; ix++) {
+            const double factor = (ix == 0 || ix == o->c.nx / 2) ? 1.0 : 2.0;
+            for (int iy = 0; iy < nym; iy++) {
+                const int k_sq2 = ix * ix + o->ky[iy] * o->ky[iy];
+                for (int k = 0; k < nzm; k++) {
+                    const int k_sq = k_sq2 + o->kz[k] * o->kz[k];
+                    const int is = 1 + (int)(sqrt((double)k_sq) + 0.5);
+                    if (is > kf + 1) continue;
+                    const size_t i = k + (size_t)nzm * (iy + (size_t)nym * ix);
+                    if (pass == 0) {
+                        double usq = 0;
+                        for (int n = 0; n < 3; n++) usq += creal(u[i + n * ss] * conj(u[i + n * ss]));
+                        work[is - 1] += factor * usq;
+                    } else {
+                        for (int n = 0; n < 3; n++) u[i + n * ss] *= work[is - 1];
+                    }
+                }
+            }
+        }
+        if (pass == 0) {
+            for (int s = 0; s <= kf; s++) ekg[s] = work[s];
+            work[0] = 1.0;
+            for (int s = 1; s <= kf; s++) work[s] = sqrt(ek0[s] / ekg[s]);
+        }
+    }
+    free(work);
+}
+
+/* nsteps full RK steps of sam.f:122-326 (no I/O, no forcing, no
+ * Chollet-Lesieur energy).
  * u: packed state (Nz_min,Ny_min,Nx_min,Lu), updated in place.
  * state[0]=time, state[1]=dt (in/out).  stats: vel_max of the last
- * step's first substep (18 doubles, [17] = C Delta^2 when sgs=2). */
+ * step's first substep (19 doubles, [17] = C Delta^2 when sgs=2). */
+void orc_step(const orc *o, cx *u, int nsteps, double *state, double *stats);
+
+/* orc_step with the reference's per-step extras: cl_energy at the first
+ * substep of every step (sam.f:128-132) when i_cl != 0, the
+ * Chollet-Lesieur nu_t(k) added to the integrating factor when i_cl = 2
+ * (velocity nu + nu_t, scalar nu/Pr + nu_t/Pr_t; an extension: SAM uses
+ * the model in its diagnostics only), and force.f after every step
+ * (sam.f:320-322) when k_force >= 0, toward ek0.  stats[18] = fac_cl of
+ * the last step. */
+void orc_step_ext(const orc *o, cx *u, int nsteps, double *state, double *stats, const double *ek0);
+
 void orc_step(const orc *o, cx *u, int nsteps, double *state, double *stats)
+{
+    orc_step_ext(o, u, nsteps, state, stats, NULL);
+}
+
+void orc_step_ext(const orc *o, cx *u, int nsteps, double *state, double *stats, const double *ek0)
 {
     orc_ws w; ws_init(o, &w);
     const size_t ss = (size_t)o->nzm * o->nym * o->nxm;
@@ -785,9 +897,12 @@ void orc_step(const orc *o, cx *u, int nsteps, double *state, double *stats)
     cx *r = (cx *)calloc(ss * (lu + 1), sizeof(cx));
     cx *usav = (cx *)malloc(sizeof(cx) * ss * lu);
     double time = state[0], dt = state[1];
-    double st[18] = {0};
+    double st[19] = {0};
+    double fac_cl = 0.0;
+    double *ekg = (double *)calloc((size_t)(o->c.k_force > 0 ? o->c.k_force : 0) + 2, sizeof(double));
     for (int nt = 0; nt < nsteps; nt++) {
         for (int m = 0; m < o->c.nrk_max; m++) {
+            if (m == 0 && o->c.i_cl) fac_cl = orc_cl_fac(o, orc_cl_energy(o, u));
             orc_z_trans_b(o, u, o->nzm, w.zb, lu);
             memcpy(usav, u, sizeof(cx) * ss * lu);
             const double dt_fac2 = o->zet[m] * dt;
@@ -805,6 +920,7 @@ void orc_step(const orc *o, cx *u, int nsteps, double *state, double *stats)
             for (int n = 0; n < lu; n++) {
                 double vis_f = o->c.vis * dt_fac;
                 if (n == 3) vis_f *= o->pr_inv;
+                const double vt_f = (o->c.i_cl == 2 ? dt_fac : 0.0) * (n == 3 ? 1.0 / o->c.pr_t : 1.0);
                 cx *un = u + ss * n, *rn = r + ss * n;
 #pragma omp parallel for schedule(static)
                 for (int ix = 0; ix < o->nxm; ix++) {
@@ -813,10 +929,13 @@ void orc_step(const orc *o, cx *u, int nsteps, double *state, double *stats)
                     for (int iy = 0; iy < o->nym; iy++) {
                         const double rky = o->wave_y * (double)o->ky[iy];
                         const double rk_sq2 = rkx2 + rky * rky;
+                        const int ksq2 = ix * ix + o->ky[iy] * o->ky[iy];
                         for (int k = 0; k < o->nzm; k++) {
                             const double rkz = o->wave_z * (double)o->kz[k];
                             const double rk_sq = rk_sq2 + rkz * rkz;
-                            const double vf = exp(-vis_f * rk_sq);
+                            double ve = vis_f;
+                            if (vt_f != 0.0) ve += vt_f * cl_vist(o, fac_cl, ksq2 + o->kz[k] * o->kz[k]);
+                            const double vf = exp(-ve * rk_sq);
                             const size_t i = k + (size_t)o->nzm * (iy + (size_t)o->nym * ix);
                             un[i] = (un[i] + dt_fac1 * rn[i]) * vf;
                         }
@@ -825,19 +944,22 @@ void orc_step(const orc *o, cx *u, int nsteps, double *state, double *stats)
             }
         }
         time += dt;
+        if (o->c.k_force >= 0 && ek0) orc_force(o, u, ek0, ekg);
     }
     state[0] = time; state[1] = dt;
+    st[18] = fac_cl;
     if (stats) memcpy(stats, st, sizeof(st));
-    free(r); free(usav);
+    free(r); free(usav); free(ekg);
     ws_free(&w);
 }
 
 /* ------------------------------------------------------------------ */
 /* Diagnostics: spectra.f:13-111 and div_check.f:11-36                  */
+/* fac_cl: sqrt(2 E_cl / k_cl) of the step (used when i_cl != 0).        */
 /* out: ek[lm] tk[lm] dk[lm] vt[lm] sample[lm], then energy_w, div_rms,  */
 /* div_max (5*lm + 3 doubles).  u and r in S layout (ld = Nz_min).      */
 /* ------------------------------------------------------------------ */
-void orc_spectra(const orc *o, const cx *u, const cx *r, double *out)
+void orc_spectra(const orc *o, const cx *u, const cx *r, double fac_cl, double *out)
 {
     const int lm = o->lm, nxm = o->nxm, nym = o->nym, nzm = o->nzm;
     const size_t ss = (size_t)nzm * nym * nxm;
@@ -861,9 +983,11 @@ void orc_spectra(const orc *o, const cx *u, const cx *r, double *out)
                 const cx u1 = u[i], u2 = u[i + ss], u3 = u[i + 2 * ss];
                 const double usq = creal(u1 * conj(u1) + u2 * conj(u2) + u3 * conj(u3));
                 const double udu = creal(r[i] * conj(u1) + r[i + ss] * conj(u2) + r[i + 2 * ss] * conj(u3));
+                const double vist = o->c.i_cl ? cl_vist(o, fac_cl, ksq) : 0.0;   /* spectra.f:39-43 */
                 acc[is] += factor * usq;
                 acc[lm + is] += factor * udu;
-                acc[2 * lm + is] += factor * rk_sq * o->c.vis * usq;
+                acc[2 * lm + is] += factor * rk_sq * (o->c.vis + vist) * usq;
+                acc[3 * lm + is] += factor * vist;
                 acc[4 * lm + is] += factor;
                 const cx div = rkx * u1 + rky * u2 + rkz * u3;
                 const double dsq = creal(div * conj(div));

@@ -12,6 +12,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "internal.hpp"
@@ -69,6 +70,11 @@ struct sam_handle {
     double *partials = nullptr;
     double *specout = nullptr;
     DevDiag *diag = nullptr;     // [0] step, [1] stage calls
+    // force.f: k_force >= 0 scales the low shells back to ek0 after every
+    // step; ek0 = ek of the last spectra taken by sam_step / sam_rhs
+    int k_force = -1;
+    double *ek0 = nullptr, *ekg = nullptr;
+    int ek0_valid = 0;
     void *tw[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
     int have_spectra = 0;
     int sgs_active = 0;
@@ -202,6 +208,8 @@ extern "C" int sam_destroy(sam_handle *h)
     hipFree(h->sgsw.p);
     hipFree(h->partials);
     hipFree(h->specout);
+    hipFree(h->ek0);
+    hipFree(h->ekg);
     hipFree(h->diag);
     for (int i = 0; i < 3; i++)
         for (int j = 0; j < 2; j++) hipFree(h->tw[i][j]);
@@ -300,6 +308,12 @@ extern "C" int sam_create(const sam_params *p, sam_handle **out)
     k.dt0 = p->dt0;
     k.i_cfl = p->i_cfl;
     k.strat = p->i_strat;
+    k.cl = 0;                                   // sam_set_cl
+    k.cl_a = 0.145;                             // init.f:88-93
+    k.cl_b = 5.01;
+    k.cl_c = 3.03;
+    k.k_cl = (double)(int)(p->nx / 3.0);
+    k.pr_t_inv = 1.0 / p->pr_t;
 
     auto fail = [&](int code) { sam_destroy(h); return code; };
     if (hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking) != hipSuccess)
@@ -314,6 +328,8 @@ extern "C" int sam_create(const sam_params *p, sam_handle **out)
         {&h->Yb, ts * 2 * d.yfield * d.lr},
         {(void **)&h->partials, sizeof(double) * (size_t)partials_capacity(d)},
         {(void **)&h->specout, sizeof(double) * (5 * d.lm + 8)},
+        {(void **)&h->ek0, sizeof(double) * d.lm},
+        {(void **)&h->ekg, sizeof(double) * d.lm},
         {(void **)&h->diag, sizeof(DevDiag) * 2},
     };
     for (auto &a : allocs) {
@@ -569,9 +585,51 @@ template <typename T> static int fluxes_to_z(sam_handle *h, int stats)
     return SAM_OK;
 }
 
-template <typename T> static int substep(sam_handle *h, int m, int spectra)
+// cl_energy at the first substep of a step (sam.f:128-132)
+static int cl_energy(sam_handle *h)
+{
+    if (!h->k.cl) return SAM_OK;
+    PROF(SAM_K_MISC);
+    RCHK(launch_cl_energy(h->d, (int)h->k.k_cl, h->U[h->cur], h->partials, h->diag, h->st));
+    return SAM_OK;
+}
+
+// spectra of U[cur] and R (spectra.f); a spectrum taken at the reference's
+// call sites becomes the forcing target ek_0 (sam.f:250-251)
+static int spectra(sam_handle *h, bool target)
+{
+    PROF(SAM_K_SPEC);
+    RCHK(launch_spectra(h->d, h->k, h->U[h->cur], h->R, h->partials, h->specout, h->diag, h->st));
+    h->have_spectra = 1;
+    if (target) {
+        HIPCHK(hipMemcpyAsync(h->ek0, h->specout, sizeof(double) * h->d.lm, hipMemcpyDeviceToDevice, h->st));
+        h->ek0_valid = 1;
+    }
+    return SAM_OK;
+}
+
+// force.f after a step: shell energies, then the scaling toward ek0.  With
+// no spectrum taken yet the current energies become the target.
+static int force(sam_handle *h)
+{
+    if (h->k_force < 0) return SAM_OK;
+    PROF(SAM_K_MISC);
+    RCHK(launch_force_shells(h->d, h->k_force, h->U[h->cur], h->ekg, h->st));
+    if (!h->ek0_valid) {
+        HIPCHK(hipMemcpyAsync(h->ek0, h->ekg, sizeof(double) * (h->k_force + 1), hipMemcpyDeviceToDevice, h->st));
+        h->ek0_valid = 1;
+    }
+    RCHK(launch_force_scale(h->d, h->k_force, h->U[h->cur], h->ek0, h->ekg, h->st));
+    return SAM_OK;
+}
+
+template <typename T> static int substep(sam_handle *h, int m, int spectra_now)
 {
     const Dims &d = h->d;
+    if (m == 0) {
+        int rc = cl_energy(h);
+        if (rc) return rc;
+    }
     int rc = fluxes_to_z<T>(h, m == 0);
     if (rc) return rc;
     {
@@ -579,10 +637,9 @@ template <typename T> static int substep(sam_handle *h, int m, int spectra)
         RCHK(launch_zrhs<T>(d, h->k, (const T *)h->Zb, h->U[h->cur], h->R, h->U[1 - h->cur], h->diag, m, 0,
                             h->tw[2][h->prec], h->st));
     }
-    if (spectra) {
-        PROF(SAM_K_SPEC);
-        RCHK(launch_spectra(d, h->p.vis, h->U[h->cur], h->R, h->partials, h->specout, h->diag, h->st));
-        h->have_spectra = 1;
+    if (spectra_now) {
+        rc = spectra(h, true);
+        if (rc) return rc;
     }
     h->cur = 1 - h->cur;
     return SAM_OK;
@@ -598,17 +655,24 @@ extern "C" int sam_step(sam_handle *h, int nsteps, int want_spectra, sam_diag *d
             int rc = h->prec ? substep<float>(h, m, spec) : substep<double>(h, m, spec);
             if (rc) return rc;
         }
-        PROF(SAM_K_MISC);
-        RCHK(launch_advance_clock(h->diag, h->st));
+        {
+            PROF(SAM_K_MISC);
+            RCHK(launch_advance_clock(h->diag, h->st));
+        }
+        int rc = force(h);   // sam.f:320-322
+        if (rc) return rc;
     }
     return fetch_diag(h, diag);
 }
 
-extern "C" int sam_rhs(sam_handle *h, double *r, int ldz, int want_spectra, sam_diag *diag)
+// the RHS-only pass; `target`: a spectrum taken here is the forcing target
+static int rhs_pass(sam_handle *h, double *r, int ldz, int want_spectra, sam_diag *diag, bool target)
 {
     if (!h) return set_err(SAM_ERR_VALUE, "null handle");
     const Dims &d = h->d;
-    int rc = h->prec ? fluxes_to_z<float>(h, 1) : fluxes_to_z<double>(h, 1);
+    int rc = cl_energy(h);
+    if (rc) return rc;
+    rc = h->prec ? fluxes_to_z<float>(h, 1) : fluxes_to_z<double>(h, 1);
     if (rc) return rc;
     if (h->prec)
         RCHK(launch_zrhs<float>(d, h->k, (const float *)h->Zb, h->U[h->cur], h->R, nullptr, h->diag, 0, 1,
@@ -617,14 +681,40 @@ extern "C" int sam_rhs(sam_handle *h, double *r, int ldz, int want_spectra, sam_
         RCHK(launch_zrhs<double>(d, h->k, (const double *)h->Zb, h->U[h->cur], h->R, nullptr, h->diag, 0, 1,
                                  h->tw[2][0], h->st));
     if (want_spectra) {
-        RCHK(launch_spectra(d, h->p.vis, h->U[h->cur], h->R, h->partials, h->specout, h->diag, h->st));
-        h->have_spectra = 1;
+        rc = spectra(h, target);
+        if (rc) return rc;
     }
     if (r) {
         rc = d2h_spec(h, r, h->R, ldz, d.lu + 1);
         if (rc) return rc;
     }
     return fetch_diag(h, diag);
+}
+
+extern "C" int sam_rhs(sam_handle *h, double *r, int ldz, int want_spectra, sam_diag *diag)
+{
+    return rhs_pass(h, r, ldz, want_spectra, diag, true);
+}
+
+extern "C" int sam_set_forcing(sam_handle *h, int k_force)
+{
+    if (!h) return set_err(SAM_ERR_VALUE, "null handle");
+    const int kmax = std::max(h->p.nx / 2, std::max(h->p.ny / 2, h->p.nz / 2));
+    if (k_force > kmax) return set_err(SAM_ERR_VALUE, "k_force must be <= max(Nx,Ny,Nz)/2 (check_inputs.f:215)");
+    if (k_force >= h->d.lm) return set_err(SAM_ERR_VALUE, "k_force beyond the spectral shells");
+    h->k_force = k_force < 0 ? -1 : k_force;
+    return SAM_OK;
+}
+
+extern "C" int sam_set_cl(sam_handle *h, int mode)
+{
+    if (!h) return set_err(SAM_ERR_VALUE, "null handle");
+    if (mode < 0 || mode > 2) return set_err(SAM_ERR_VALUE, "Chollet-Lesieur mode must be 0, 1 or 2");
+    const int kcl = (int)h->k.k_cl;
+    if (mode && (kcl < 1 || kcl > h->d.kxm || kcl > h->d.kym || kcl > h->d.kzm))
+        return set_err(SAM_ERR_UNSUPPORTED, "k_cl = int(Nx/3) lies beyond the stored modes (cl_energy.f would read past u)");
+    h->k.cl = mode;
+    return SAM_OK;
 }
 
 extern "C" int sam_get_spectra(sam_handle *h, double *out)
@@ -1293,7 +1383,7 @@ extern "C" int sam_sgs_check(sam_handle *h, double tol, double *err, int *switch
     const int models[3] = {SAM_SGS_NONE, SAM_SGS_DYN, SAM_SGS_MLP};
     for (int i = 0; i < 3; i++) {
         h->p.sgs = models[i];
-        int rc = sam_rhs(h, nullptr, h->d.nzm, 1, nullptr);
+        int rc = rhs_pass(h, nullptr, h->d.nzm, 1, nullptr, false);
         if (!rc) rc = sam_get_spectra(h, spec.data());
         if (rc) {
             h->p.sgs = SAM_SGS_MLP;

@@ -31,7 +31,8 @@ struct DevDiag {
     double c_delta_sq;
     long long nt;
     unsigned long long divmax_bits;    // running max |div u| as IEEE bits
-    double spare[6];
+    double fac_cl;                     // sqrt(2 E_cl / k_cl) of the step (sam.f:131)
+    double spare[5];
 };
 
 // Scalars the RK epilogue needs (init.f:97-148, rhs.f, sam.f:293-314).
@@ -40,6 +41,11 @@ struct StepConsts {
     double vis, pr_inv, buoy_fac_x, buoy_fac_z, lapse;
     double cfl0, adv_cfl_limit, dt0;
     int i_cfl, strat;
+    // Chollet-Lesieur nu_t(k) = fac_cl (a + b exp(-c k_cl / |k|)) (init.f:88-93,
+    // spectra.f:39-43): cl = 0 off, 1 in the spectra only (SAM's i_les = 1),
+    // 2 also in the integrating factor (scalar: nu_t / Pr_t)
+    int cl;
+    double cl_a, cl_b, cl_c, k_cl, pr_t_inv;
 };
 
 /* Device buffer layouts (complex elements, first index fastest):
@@ -70,8 +76,13 @@ int launch_finalize(const Dims &d, const StepConsts &k, const double *partials, 
 int launch_advance_clock(DevDiag *diag, hipStream_t st);
 int launch_project(const Dims &d, const StepConsts &k, const double *F, const double *usav, double *r, hipStream_t st);
 int launch_products(const Dims &d, const void *P, int is_double, void *Q, hipStream_t st);
-int launch_spectra(const Dims &d, double vis, const double *U, const double *R, double *partials, double *out,
-                   DevDiag *diag, hipStream_t st);
+int launch_spectra(const Dims &d, const StepConsts &k, const double *U, const double *R, double *partials,
+                   double *out, DevDiag *diag, hipStream_t st);
+// cl_energy.f: E_cl of the band around k_cl -> diag->fac_cl
+int launch_cl_energy(const Dims &d, int k_cl, const double *U, double *partials, DevDiag *diag, hipStream_t st);
+// force.f: shell energies (shells 0..kf) into ekg, then u, v, w of shells 1..kf scaled by sqrt(ek0/ekg)
+int launch_force_shells(const Dims &d, int kf, const double *U, double *ekg, hipStream_t st);
+int launch_force_scale(const Dims &d, int kf, double *U, const double *ek0, const double *ekg, hipStream_t st);
 int launch_strided_copy(const double *src, double *dst, long long n0, long long n1, long long n2, long long n3,
                         long long s0, long long s1, long long s2, long long s3, long long d0, long long d1,
                         long long d2, long long d3, int cplx, hipStream_t st);

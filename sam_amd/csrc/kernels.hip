@@ -13,6 +13,7 @@
 // The unfused k_xb / k_xf / k_zf serve the stage entry points and the LES path.
 #include "internal.hpp"
 
+#include <algorithm>
 #include <map>
 #include <mutex>
 
@@ -955,6 +956,14 @@ __device__ __forceinline__ void rhs_mode(const cpx<double> *F, double rkx, doubl
 
 __device__ __forceinline__ int kwave(int i, int km, int nm) { return i <= km ? i : i - nm; }
 
+// Chollet-Lesieur eddy viscosity of integer wavenumber magnitude sqrt(ksq)
+// (spectra.f:39-43; constants init.f:88-93); 0 at k = 0
+__device__ __forceinline__ double cl_vist(const StepConsts &k, double fac_cl, int ksq)
+{
+    if (ksq == 0) return 0.0;
+    return fac_cl * (k.cl_a + k.cl_b * exp(-(k.cl_c * k.k_cl) / sqrt((double)ksq)));
+}
+
 // k_zrhs: LR z-columns of one (kx,ky) -> RHS and RK/IF update of Lu fields
 template <typename T, int N, int LU, int LR, int EPT>
 __global__ __launch_bounds__(LR * N / EPT) void k_zrhs(const T *__restrict__ Zr, const double *__restrict__ Ur,
@@ -1017,9 +1026,16 @@ __global__ __launch_bounds__(LR * N / EPT) void k_zrhs(const T *__restrict__ Zr,
         cpx<double> r[4], pr;
         rhs_mode<LU>(F, rkx, rky, rkz, rk_sq, wsav, tsav, k, r, pr);
         if (mode == 0) {
+            // Chollet-Lesieur nu_t(k) in the integrating factor (an extension:
+            // SAM applies the model to its diagnostics only)
+            double vt = 0.0;
+            if (k.cl == 2) {
+                const int kzw = kwave(kz, d.kzm, d.nzm), kyw = kwave(iy, d.kym, d.nym);
+                vt = cl_vist(k, diag->fac_cl, ix * ix + kyw * kyw + kzw * kzw) * (dtf1 + dtf2);
+            }
 #pragma unroll
             for (int n = 0; n < LU; n++) {
-                const double vfn = (n == 3) ? vf * k.pr_inv : vf;
+                const double vfn = (n == 3) ? vf * k.pr_inv + vt * k.pr_t_inv : vf + vt;
                 const double vis_fac = exp(-vfn * rk_sq);
                 cpx<double> u = us[n];
                 if (m > 0) u = {u.x + dtf2 * rp[n].x, u.y + dtf2 * rp[n].y};
@@ -1294,8 +1310,10 @@ int launch_products(const Dims &d, const void *P, int is_double, void *Q, hipStr
 constexpr int SPEC_YCH = 8;
 
 __global__ __launch_bounds__(256) void k_spectra(const double *__restrict__ Ur, const double *__restrict__ Rr, Dims d,
-                                                  double vis, double *__restrict__ partials, DevDiag *__restrict__ diag)
+                                                  StepConsts k, double *__restrict__ partials,
+                                                  DevDiag *__restrict__ diag)
 {
+    const double vis = k.vis, fac_cl = k.cl ? diag->fac_cl : 0.0;
     extern __shared__ double acc[];   // 5*lm + 1
     const int lm = d.lm;
     for (int i = threadIdx.x; i < 5 * lm + 1; i += 256) acc[i] = 0.0;
@@ -1321,9 +1339,11 @@ __global__ __launch_bounds__(256) void k_spectra(const double *__restrict__ Ur, 
         const cpx<double> r1 = R[si], r2 = R[d.ss + si], r3 = R[2 * d.ss + si];
         const double usq = u1.x * u1.x + u1.y * u1.y + u2.x * u2.x + u2.y * u2.y + u3.x * u3.x + u3.y * u3.y;
         const double udu = r1.x * u1.x + r1.y * u1.y + r2.x * u2.x + r2.y * u2.y + r3.x * u3.x + r3.y * u3.y;
+        const double vist = k.cl ? cl_vist(k, fac_cl, ksq) : 0.0;
         atomicAdd(&acc[is], factor * usq);
         atomicAdd(&acc[lm + is], factor * udu);
-        atomicAdd(&acc[2 * lm + is], factor * rk_sq * vis * usq);
+        atomicAdd(&acc[2 * lm + is], factor * rk_sq * (vis + vist) * usq);
+        if (k.cl) atomicAdd(&acc[3 * lm + is], factor * vist);
         atomicAdd(&acc[4 * lm + is], factor);
         const cpx<double> dv = lin3(rkx, u1, rky, u2, rkz, u3);
         const double dsq = dv.x * dv.x + dv.y * dv.y;
@@ -1370,14 +1390,170 @@ __global__ void k_spectra_final(int lm, double *__restrict__ out, DevDiag *__res
     }
 }
 
-int launch_spectra(const Dims &d, double vis, const double *U, const double *R, double *partials, double *out,
-                   DevDiag *diag, hipStream_t st)
+int launch_spectra(const Dims &d, const StepConsts &k, const double *U, const double *R, double *partials,
+                   double *out, DevDiag *diag, hipStream_t st)
 {
     const int nb = d.nxm * SPEC_YCH;
     const size_t sh = sizeof(double) * (5 * d.lm + 1);
-    hipLaunchKernelGGL(k_spectra, dim3(nb), dim3(256), sh, st, U, R, d, vis, partials, diag);
+    hipLaunchKernelGGL(k_spectra, dim3(nb), dim3(256), sh, st, U, R, d, k, partials, diag);
     hipLaunchKernelGGL(k_spectra_reduce, dim3((5 * d.lm + 1 + 255) / 256), dim3(256), 0, st, partials, nb, d.lm, out);
     hipLaunchKernelGGL(k_spectra_final, dim3(1), dim3(64), 0, st, d.lm, out, diag);
+    return launched();
+}
+
+// ---------------------------------------------------------------------
+// Chollet-Lesieur energy, cl_energy.f:8-43.  One thread per (kx, ky)
+// column walks the band's kz rows as the reference does: k = k_start..k_stop
+// (1-based) and the mirror row kk = Nz_min + 2 - k; for k = 1 the mirror
+// is the padding row past the stored modes, which holds zeros.  Per-block
+// sums, then one thread adds them in block order (deterministic).
+// ---------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_cl_energy(const double *__restrict__ Ur, Dims d, int k_cl,
+                                                    double *__restrict__ partials)
+{
+    __shared__ double red[4];
+    const int col = blockIdx.x * 256 + threadIdx.x;
+    double e = 0.0;
+    if (col < d.nxm * d.nym) {
+        const int ix = col / d.nym, iy = col % d.nym;
+        const int ky = kwave(iy, d.kym, d.nym);
+        const int ksq2 = ix * ix + ky * ky;
+        const double start_sq = (k_cl - 0.5) * (k_cl - 0.5), stop_sq = (k_cl + 0.5) * (k_cl + 0.5);
+        const int k_start = (ksq2 > start_sq) ? 1 : (int)sqrt(start_sq - ksq2) + 2;
+        const int k_stop = (ksq2 > stop_sq) ? 0 : (int)sqrt(stop_sq - ksq2) + 1;
+        const double factor = (ix == 0 || 2 * ix == d.nx) ? 1.0 : 2.0;
+        const cpx<double> *U = reinterpret_cast<const cpx<double> *>(Ur) + (long long)col * d.nzm;
+        for (int k = k_start; k <= k_stop; k++) {
+            const int kk = d.nzm + 2 - k;
+            double s1 = 0.0, s2 = 0.0;
+#pragma unroll
+            for (int n = 0; n < 3; n++) {
+                const cpx<double> a = U[n * d.ss + k - 1];
+                s1 += a.x * a.x + a.y * a.y;
+                if (kk <= d.nzm) {
+                    const cpx<double> b = U[n * d.ss + kk - 1];
+                    s2 += b.x * b.x + b.y * b.y;
+                }
+            }
+            e += factor * (s1 + s2);
+        }
+    }
+    e = wave_sum(e);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e;
+    __syncthreads();
+    if (threadIdx.x == 0) partials[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+__global__ void k_cl_final(const double *__restrict__ partials, int nb, int k_cl, DevDiag *__restrict__ diag)
+{
+    double e = 0.0;
+    for (int b = 0; b < nb; b++) e += partials[b];
+    diag->fac_cl = sqrt(2.0 * e / (double)k_cl);   // sam.f:131
+}
+
+int launch_cl_energy(const Dims &d, int k_cl, const double *U, double *partials, DevDiag *diag, hipStream_t st)
+{
+    const int nb = (d.nxm * d.nym + 255) / 256;
+    hipLaunchKernelGGL(k_cl_energy, dim3(nb), dim3(256), 0, st, U, d, k_cl, partials);
+    hipLaunchKernelGGL(k_cl_final, dim3(1), dim3(1), 0, st, partials, nb, k_cl, diag);
+    return launched();
+}
+
+// ---------------------------------------------------------------------
+// Spectral forcing, force.f:12-63.  Only modes with |k| < k_force + 1/2 can
+// fall in shells 1..k_force + 1 (1-based), so both kernels walk the box
+// kx <= kf, |ky| <= kf, |kz| <= kf (clipped to the stored modes).
+// ---------------------------------------------------------------------
+constexpr int FORCE_NT = 256, FORCE_CH = 32;
+
+struct ForceBox {
+    int bx, by, bz, ny, nz;
+    long long n;
+    __device__ ForceBox(const Dims &d, int kf)
+    {
+        bx = min(kf, d.kxm);
+        by = min(kf, d.kym);
+        bz = min(kf, d.kzm);
+        ny = 2 * by + 1;
+        nz = 2 * bz + 1;
+        n = (long long)(bx + 1) * ny * nz;
+    }
+    // mode e of the box -> S-layout index, integer |k|^2, kx
+    __device__ long long mode(const Dims &d, long long e, int &ksq, int &ix) const
+    {
+        ix = (int)(e / (ny * nz));
+        const int rem = (int)(e % (ny * nz));
+        const int ky = rem / nz - by, kz = rem % nz - bz;
+        ksq = ix * ix + ky * ky + kz * kz;
+        const int iy = ky >= 0 ? ky : ky + d.nym, iz = kz >= 0 ? kz : kz + d.nzm;
+        return ((long long)ix * d.nym + iy) * d.nzm + iz;
+    }
+};
+
+// shell energies (force.f:12-35) of shells 0..kf (0-based), one block: per
+// thread LDS accumulators for 32 shells at a time, summed in thread order
+__global__ __launch_bounds__(FORCE_NT) void k_force_shells(const double *__restrict__ Ur, Dims d, int kf,
+                                                            double *__restrict__ ekg)
+{
+    __shared__ double acc[FORCE_CH][FORCE_NT];
+    const ForceBox box(d, kf);
+    const cpx<double> *U = reinterpret_cast<const cpx<double> *>(Ur);
+    for (int s0 = 0; s0 <= kf; s0 += FORCE_CH) {
+        for (int c = 0; c < FORCE_CH; c++) acc[c][threadIdx.x] = 0.0;
+        for (long long e = threadIdx.x; e < box.n; e += FORCE_NT) {
+            int ksq, ix;
+            const long long si = box.mode(d, e, ksq, ix);
+            const int is = (int)(sqrt((double)ksq) + 0.5);
+            if (is < s0 || is > kf || is >= s0 + FORCE_CH) continue;
+            double usq = 0.0;
+#pragma unroll
+            for (int n = 0; n < 3; n++) {
+                const cpx<double> a = U[n * d.ss + si];
+                usq += a.x * a.x + a.y * a.y;
+            }
+            acc[is - s0][threadIdx.x] += ((ix == 0 || 2 * ix == d.nx) ? 1.0 : 2.0) * usq;
+        }
+        __syncthreads();
+        if (threadIdx.x < FORCE_CH && s0 + (int)threadIdx.x <= kf) {
+            double t = 0.0;
+            for (int i = 0; i < FORCE_NT; i++) t += acc[threadIdx.x][i];
+            ekg[s0 + threadIdx.x] = t;
+        }
+        __syncthreads();
+    }
+}
+
+// u, v, w of shells 1..kf (0-based; shell 0 is the mean) times sqrt(ek0/ekg) (force.f:37-63)
+__global__ __launch_bounds__(256) void k_force_scale(double *__restrict__ Ur, Dims d, int kf,
+                                                      const double *__restrict__ ek0, const double *__restrict__ ekg)
+{
+    const ForceBox box(d, kf);
+    cpx<double> *U = reinterpret_cast<cpx<double> *>(Ur);
+    for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < box.n; e += (long long)gridDim.x * 256) {
+        int ksq, ix;
+        const long long si = box.mode(d, e, ksq, ix);
+        const int is = (int)(sqrt((double)ksq) + 0.5);
+        if (is < 1 || is > kf) continue;
+        const double f = sqrt(ek0[is] / ekg[is]);
+#pragma unroll
+        for (int n = 0; n < 3; n++) {
+            const cpx<double> a = U[n * d.ss + si];
+            U[n * d.ss + si] = {a.x * f, a.y * f};
+        }
+    }
+}
+
+int launch_force_shells(const Dims &d, int kf, const double *U, double *ekg, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_force_shells, dim3(1), dim3(FORCE_NT), 0, st, U, d, kf, ekg);
+    return launched();
+}
+
+int launch_force_scale(const Dims &d, int kf, double *U, const double *ek0, const double *ekg, hipStream_t st)
+{
+    const long long n = (long long)(min(kf, d.kxm) + 1) * (2 * min(kf, d.kym) + 1) * (2 * min(kf, d.kzm) + 1);
+    const unsigned nb = (unsigned)std::min<long long>((n + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_force_scale, dim3(nb), dim3(256), 0, st, U, d, kf, ek0, ekg);
     return launched();
 }
 

@@ -19,7 +19,11 @@
 !                coefficient dynamic-model labels and is scaled by C Delta^2
 !                from the dynamic procedure; 0 trained on box-filter labels
 !   mlp_refresh  i_les 4, mlp_labels 1: steps between C Delta^2 updates (20)
-! i_les: 0 DNS, 1 Chollet-Lesieur (diagnostic-only in SAM; runs as DNS),
+!   cl_couple    i_les 1: 0 (default) the Chollet-Lesieur nu_t(k) enters the
+!                spectra only, as in SAM; 1 also the integrating factor
+! i_force 1: force.f after every step (shells 2..k_force+1 held at the last
+!            spectrum written to spectra.out)
+! i_les: 0 DNS, 1 Chollet-Lesieur (diagnostics of spectra.f; the step is DNS),
 !        2 Smagorinsky (sm.f), 3 dynamic Smagorinsky (dm.f), 4 learned closure
 !        with its parameters read from mlp.params (SAM_MLP_NPARAM + 32 raw
 !        FP32 values as written by sam_mlp_get_params).
@@ -28,7 +32,7 @@ program samhip_driver
   use samhip_mod
   implicit none
 
-  integer, parameter :: NIN = 72, NDYN = 6, NPAR = NIN + NDYN, NEXT = 5
+  integer, parameter :: NIN = 72, NDYN = 6, NPAR = NIN + NDYN, NEXT = 6
   character(len=12) :: labels(NPAR + NEXT)
   real(8) :: values(NPAR + NEXT), defaults(NPAR + NEXT)
   logical :: required(NPAR + NEXT), fixed(NPAR + NEXT), found(NPAR + NEXT)
@@ -78,7 +82,6 @@ program samhip_driver
   case (4); p%sgs = SAM_SGS_MLP
   case default; p%sgs = SAM_SGS_NONE
   end select
-  if (i_les == 1) print *, 'WARNING: i_les=1 (Chollet-Lesieur) only changes diagnostics in SAM; running DNS'
   p%xl = rv('xl'); p%yl = rv('yl'); p%zl = rv('zl')
   p%vis = rv('vis'); p%pr = rv('pr'); p%grav = rv('grav'); p%to = rv('to')
   p%lapse = rv('lapse0')
@@ -93,6 +96,14 @@ program samhip_driver
   if (rc /= SAM_OK) call fail(int(rc), 'sam_create: '//trim(sam_error_message()))
   mlp = c_null_ptr
   if (i_les == 4) call attach_network()
+  if (i_les == 1) then
+     rc = sam_set_cl(h, int(1 + iv('cl_couple'), c_int))
+     if (rc /= SAM_OK) call fail(int(rc), 'sam_set_cl: '//trim(sam_error_message()))
+  end if
+  if (iv('i_force') == 1) then
+     rc = sam_set_forcing(h, int(iv('k_force'), c_int))
+     if (rc /= SAM_OK) call fail(int(rc), 'sam_set_forcing: '//trim(sam_error_message()))
+  end if
   rc = sam_get_dims(h, d)
   lu = d%lu
   lm = d%lm
@@ -252,8 +263,8 @@ contains
     end do
     labels(73:78) = [character(len=12) :: 'time', 'dt', 't_stat', 'nt', 'n_frame_p', 'n_hist']
     defaults(73:78) = 0d0; required(73:78) = .true.; fixed(73:78) = .false.
-    labels(79:83) = [character(len=12) :: 'i_prec', 'i_lapse', 'pr_t', 'mlp_labels', 'mlp_refresh']
-    defaults(79:83) = [0d0, 0d0, 1d0, 1d0, 20d0]; required(79:83) = .false.; fixed(79:83) = .false.
+    labels(79:84) = [character(len=12) :: 'i_prec', 'i_lapse', 'pr_t', 'mlp_labels', 'mlp_refresh', 'cl_couple']
+    defaults(79:84) = [0d0, 0d0, 1d0, 1d0, 20d0, 0d0]; required(79:84) = .false.; fixed(79:84) = .false.
   end subroutine set_labels
 
   ! read_line.f:24-89: one "label value [comment]" per line; '#' in column 1
@@ -373,6 +384,7 @@ contains
     call range_check('i_prec', 0, 1)
     call range_check('mlp_labels', 0, 1)
     call range_check('mlp_refresh', 1, 1000000)
+    call range_check('cl_couple', 0, 1)
   contains
     subroutine range_check(name, lo, hi)
       character(len=*), intent(in) :: name

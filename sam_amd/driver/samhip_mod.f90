@@ -124,6 +124,19 @@ module samhip_mod
        type(c_ptr), value :: h, m
      end function sam_attach_mlp
 
+     ! per-step extras: force.f and the Chollet-Lesieur model
+     integer(c_int) function sam_set_forcing(h, k_force) bind(c, name='sam_set_forcing')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: h
+       integer(c_int), value :: k_force
+     end function sam_set_forcing
+
+     integer(c_int) function sam_set_cl(h, mode) bind(c, name='sam_set_cl')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: h
+       integer(c_int), value :: mode
+     end function sam_set_cl
+
      integer(c_int) function sam_set_mlp_refresh(h, nsteps) bind(c, name='sam_set_mlp_refresh')
        import :: c_int, c_ptr
        type(c_ptr), value :: h

@@ -42,6 +42,7 @@ EXPORTS = (
     "sam_mlp_clear_pool", "sam_mlp_set_labels",
     "sam_mlp_fit_norms", "sam_mlp_train", "sam_mlp_grad", "sam_mlp_infer",
     "sam_attach_mlp", "sam_set_sgs", "sam_sgs_check", "sam_set_mlp_refresh",
+    "sam_set_forcing", "sam_set_cl",
 )
 
 MLP_NPARAM = 10448
@@ -151,6 +152,8 @@ def load():
     L.sam_attach_mlp.argtypes = [_hp, _hp]
     L.sam_set_sgs.argtypes = [_hp, ctypes.c_int]
     L.sam_set_mlp_refresh.argtypes = [_hp, ctypes.c_int]
+    L.sam_set_forcing.argtypes = [_hp, ctypes.c_int]
+    L.sam_set_cl.argtypes = [_hp, ctypes.c_int]
     L.sam_sgs_check.argtypes = [_hp, ctypes.c_double, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]
     for name in EXPORTS:
         if name in ("sam_last_error", "sam_mlp_pool_size"):
@@ -429,6 +432,21 @@ def sgs_check(sam, tol):
 
 def set_sgs(sam, model):
     _chk(load().sam_set_sgs(sam.h, int(model)))
+
+
+def set_forcing(sam, k_force):
+    """force.f after every step: shells 2..k_force+1 held at the last
+    spectrum (k_force < 0: off)"""
+    _chk(load().sam_set_forcing(sam.h, int(k_force)))
+
+
+CL_OFF, CL_DIAG, CL_COUPLED = 0, 1, 2
+
+
+def set_cl(sam, mode):
+    """Chollet-Lesieur model: CL_DIAG in the spectra (SAM's i_les = 1),
+    CL_COUPLED also in the integrating factor"""
+    _chk(load().sam_set_cl(sam.h, int(mode)))
 
 
 def set_mlp_refresh(sam, nsteps):

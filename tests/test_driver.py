@@ -134,3 +134,33 @@ def test_learned_closure_run_matches_library(tmp_path):
     u = sio.read_vel(str(tmp_path / "vel.000012"), d.nz_min, d.ny_min, d.nx_min, d.lu)
     assert np.linalg.norm(ref - ic.taylor_green(32)) > 1e-6 * np.linalg.norm(ref)
     assert np.linalg.norm(u - ref) <= 1e-14 * np.linalg.norm(ref)
+
+
+@pytest.mark.gpu
+@needs_driver
+def test_forcing_and_chollet_lesieur_run_matches_library(tmp_path):
+    """i_force 1 / k_force 2 and i_les 1 / cl_couple 1 from SAM's run files:
+    the state equals the library's with the same spectra cadence (the
+    forcing target is the spectrum of every n_skip_h-th step)"""
+    from sam_amd import samhip as S
+    run_p = dict(TG64, Nx=32, Ny=32, Nz=32, vis=1e-3, n_steps=10, n_skip_h=5, i_les=1, c_smag=0.1,
+                 cl_couple=1, i_force=1, k_force=2)
+    u0 = ic.taylor_green(32)
+    sio.write_vel(str(tmp_path / "vel.000000"), u0)
+    sio.write_header(str(tmp_path / "header.000000"), run_p, time=0.0, dt=0.01, nt=0)
+    r = run(tmp_path, **run_p)
+    assert r.returncode == 0, r.stdout + r.stderr
+    sam = S.Sam(S.make_params(32, vis=1e-3, dt0=0.01))
+    S.set_cl(sam, S.CL_COUPLED)
+    S.set_forcing(sam, 2)
+    sam.set_state(u0)
+    for nt in range(10):
+        sam.step(1, want_spectra=(nt % 5 == 0))
+    ref = sam.get_state()
+    d = sam.dims
+    u = sio.read_vel(str(tmp_path / "vel.000010"), d.nz_min, d.ny_min, d.nx_min, d.lu)
+    assert np.linalg.norm(u - ref) <= 1e-14 * np.linalg.norm(ref)
+    plain = S.Sam(S.make_params(32, vis=1e-3, dt0=0.01))
+    plain.set_state(u0)
+    plain.step(10)
+    assert np.linalg.norm(plain.get_state() - ref) > 1e-8 * np.linalg.norm(ref)
