@@ -140,6 +140,23 @@ int  sam_set_forcing(sam_handle *h, int k_force);
  * stored modes of a direction. */
 int  sam_set_cl(sam_handle *h, int mode);
 
+/* ---- visualization output (write_planes.f, get_mean.f, x/y_deriv.f) --- */
+/* keep_pressure != 0: the last substep of every step keeps its pressure,
+ * the r(.,.,.,Lu+1) that write_planes.f outputs at the next planes step
+ * (sam.f:136-150).  Costs one field write per step; off by default. */
+int  sam_set_planes(sam_handle *h, int keep_pressure);
+/* Planes and horizontal means of the current state, as sam.f:145-167 and
+ * write_planes.f form them at the first substep of an n_skip_p step
+ * (positions 1-based, at most 9 of each kind; nf = 2 Lu fields each):
+ *   xy (Nx, Ny, nf, n_xy): u v w [T] p du/dz dv/dz [dT/dz] at z = k_xy(m)
+ *   xz (Nx, Nz, nf, n_xz): u v w [T] p du/dy dw/dy [dT/dy] at y = j_xz(m)
+ *   yz (Ny, Nz, nf, n_yz): u v w [T] p dv/dx dw/dx [dT/dx] at x = i_yz(m)
+ *   mean (Nz, Lu): horizontal mean profiles (get_mean.f, periodic z)
+ * p is the kept pressure (zero before any).  Computed in the handle's
+ * precision.  mean may be NULL; a kind with n = 0 may pass NULLs. */
+int  sam_planes(sam_handle *h, int n_xy, const int *k_xy, int n_xz, const int *j_xz, int n_yz, const int *i_yz,
+                double *mean, double *xy, double *xz, double *yz);
+
 /* ---- stage entry points (parity with the reference's routines) -------- */
 /* z_trans_b.f (case 0): u (ldz,Ny_min,Nx_min,nvar) -> out (Nz,Ny_min,Nx_min,nvar).
  * The reference scatters to (Ny_min,Nx_min,Nz,nvar); this is the same data

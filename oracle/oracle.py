@@ -65,7 +65,9 @@ def lib():
         L.orc_to_physical.argtypes = [ctypes.c_void_p, _dp, _dp]
         L.orc_rhs.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_double, _dp]
         L.orc_step.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int, _dp, _dp]
-        L.orc_step_ext.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int, _dp, _dp, _dp]
+        L.orc_step_ext.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int, _dp, _dp, _dp, _dp]
+        L.orc_planes.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int, _ip, ctypes.c_int, _ip, ctypes.c_int, _ip,
+                                 _dp, _dp, _dp, _dp]
         L.orc_spectra.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_double, _dp]
         L.orc_cl_energy.restype = ctypes.c_double
         L.orc_cl_energy.argtypes = [ctypes.c_void_p, _dp]
@@ -231,19 +233,40 @@ class Oracle:
         self.L.orc_rhs(self.h, _p(u), _p(r), self.cfg.dt0 if dt_in is None else dt_in, _p(st))
         return r, st
 
-    def step(self, u, nsteps, time=0.0, dt=None, ek0=None):
+    def step(self, u, nsteps, time=0.0, dt=None, ek0=None, want_p=False):
         """nsteps RK steps; with cfg.i_cl the Chollet-Lesieur energy of every
         step (st[18] = fac_cl of the last one), with cfg.k_force >= 0 and ek0
-        given, force.f after every step toward ek0"""
+        given, force.f after every step toward ek0.  want_p: also return the
+        pressure of the last substep (S layout)."""
         u = np.array(u, order="F", copy=True)
         state = np.array([time, self.cfg.dt0 if dt is None else dt])
         st = np.zeros(19)
-        if ek0 is None:
+        if ek0 is None and not want_p:
             self.L.orc_step(self.h, _p(u), nsteps, _p(state), _p(st))
-        else:
-            ek0 = np.ascontiguousarray(ek0, dtype=np.float64)
-            self.L.orc_step_ext(self.h, _p(u), nsteps, _p(state), _p(st), _p(ek0))
+            return u, state[0], state[1], st
+        e = np.ascontiguousarray(ek0 if ek0 is not None else np.zeros(self.lm), dtype=np.float64)
+        p = self.zeros_spec(1)
+        force = ek0 is not None and self.cfg.k_force >= 0
+        self.L.orc_step_ext(self.h, _p(u), nsteps, _p(state), _p(st), _p(e) if force else None, _p(p))
+        if want_p:
+            return u, state[0], state[1], st, p[..., 0]
         return u, state[0], state[1], st
+
+    def planes(self, u, p, k_xy=(), j_xz=(), i_yz=()):
+        """write_planes.f / get_mean.f of state u with pressure p (S layout):
+        mean (Nz, Lu); xy (Nx, Ny, 2Lu, n_xy); xz (Nx, Nz, 2Lu, n_xz);
+        yz (Ny, Nz, 2Lu, n_yz); positions 1-based"""
+        u = np.asfortranarray(u)
+        p = np.asfortranarray(p.reshape(p.shape[:3]) if p.ndim == 4 else p)
+        nf = 2 * self.lu
+        ia = lambda v: (ctypes.c_int * max(len(v), 1))(*v)
+        mean = np.zeros((self.nz, self.lu), order="F")
+        xy = np.zeros((self.nx, self.ny, nf, max(len(k_xy), 1)), order="F")
+        xz = np.zeros((self.nx, self.nz, nf, max(len(j_xz), 1)), order="F")
+        yz = np.zeros((self.ny, self.nz, nf, max(len(i_yz), 1)), order="F")
+        self.L.orc_planes(self.h, _p(u), _p(p), len(k_xy), ia(k_xy), len(j_xz), ia(j_xz), len(i_yz), ia(i_yz),
+                          _p(mean), _p(xy), _p(xz), _p(yz))
+        return mean, xy[..., :len(k_xy)], xz[..., :len(j_xz)], yz[..., :len(i_yz)]
 
     def cl_energy(self, u):
         """E_cl of cl_energy.f and fac_cl = sqrt(2 E_cl / k_cl)"""

@@ -882,14 +882,16 @@ void orc_step(const orc *o, cx *u, int nsteps, double *state, double *stats);
  * the model in its diagnostics only), and force.f after every step
  * (sam.f:320-322) when k_force >= 0, toward ek0.  stats[18] = fac_cl of
  * the last step. */
-void orc_step_ext(const orc *o, cx *u, int nsteps, double *state, double *stats, const double *ek0);
+void orc_step_ext(const orc *o, cx *u, int nsteps, double *state, double *stats, const double *ek0, cx *p_last);
 
 void orc_step(const orc *o, cx *u, int nsteps, double *state, double *stats)
 {
-    orc_step_ext(o, u, nsteps, state, stats, NULL);
+    orc_step_ext(o, u, nsteps, state, stats, NULL, NULL);
 }
 
-void orc_step_ext(const orc *o, cx *u, int nsteps, double *state, double *stats, const double *ek0)
+/* p_last (may be NULL): the pressure r(.,.,.,Lu+1) of the last substep,
+ * which write_planes.f outputs at the next step (sam.f:136-150). */
+void orc_step_ext(const orc *o, cx *u, int nsteps, double *state, double *stats, const double *ek0, cx *p_last)
 {
     orc_ws w; ws_init(o, &w);
     const size_t ss = (size_t)o->nzm * o->nym * o->nxm;
@@ -949,6 +951,7 @@ void orc_step_ext(const orc *o, cx *u, int nsteps, double *state, double *stats,
     state[0] = time; state[1] = dt;
     st[18] = fac_cl;
     if (stats) memcpy(stats, st, sizeof(st));
+    if (p_last) memcpy(p_last, r + ss * lu, sizeof(cx) * ss);
     free(r); free(usav); free(ekg);
     ws_free(&w);
 }
@@ -1003,4 +1006,140 @@ void orc_spectra(const orc *o, const cx *u, const cx *r, double fac_cl, double *
     out[5 * lm + 1] = sqrt(divs);
     out[5 * lm + 2] = sqrt(divmax);
     free(acc);
+}
+
+/* ------------------------------------------------------------------ */
+/* Visualization output: write_planes.f:18-144 with get_mean.f:8-23    */
+/* (periodic case, no jump removal) and x_deriv.f / y_deriv.f           */
+/* ------------------------------------------------------------------ */
+
+/* x_deriv.f:7-25: d/dx of an (Nx, Ny) plane, one real FFT per row */
+static void plane_x_deriv(const orc *o, const double *u, double *du)
+{
+    const int nx = o->c.nx, ny = o->c.ny, kxm = o->kxm;
+    for (int j = 0; j < ny; j++) {
+        double *r = du + (size_t)j * nx;
+        for (int i = 0; i < nx; i++) r[i] = u[i + (size_t)j * nx] / nx;
+        orc_rfft(nx, r, -1);
+        r[0] = 0.0;
+        for (int k = 1; k <= kxm; k++) {
+            const double rk = k * o->wave_x, re = r[2 * k - 1];
+            r[2 * k - 1] = -rk * r[2 * k];
+            r[2 * k] = rk * re;
+        }
+        for (int i = 2 * kxm + 1; i < nx; i++) r[i] = 0.0;
+        orc_rfft(nx, r, 1);
+    }
+}
+
+/* y_deriv.f:35-56: d/dy of an (Nx, Ny) plane, one real FFT per column */
+static void plane_y_deriv(const orc *o, const double *u, double *du)
+{
+    const int nx = o->c.nx, ny = o->c.ny, kym = o->kym;
+    double *w = (double *)malloc(sizeof(double) * ny);
+    for (int i = 0; i < nx; i++) {
+        for (int j = 0; j < ny; j++) w[j] = u[i + (size_t)j * nx] / ny;
+        orc_rfft(ny, w, -1);
+        w[0] = 0.0;
+        for (int k = 1; k <= kym; k++) {
+            const double rk = k * o->wave_y, re = w[2 * k - 1];
+            w[2 * k - 1] = -rk * w[2 * k];
+            w[2 * k] = rk * re;
+        }
+        for (int j = 2 * kym + 1; j < ny; j++) w[j] = 0.0;
+        orc_rfft(ny, w, 1);
+        for (int j = 0; j < ny; j++) du[i + (size_t)j * nx] = w[j];
+    }
+    free(w);
+}
+
+/* Planes and means of the state u (S layout) with pressure p (S layout,
+ * the r(.,.,.,Lu+1) of the last RHS), as sam.f:145-167 builds them and
+ * write_planes.f writes them.  Fields per plane (nf = 2 Lu):
+ *   xy: u v w [T] p du/dz dv/dz [dT/dz]   (Nx, Ny, nf) per plane
+ *   xz: u v w [T] p du/dy dw/dy [dT/dy]   (Nx, Nz, nf)
+ *   yz: u v w [T] p dv/dx dw/dx [dT/dx]   (Ny, Nz, nf)
+ * mean: (Nz, Lu), the kx = ky = 0 column through rfftb (get_mean.f case 0).
+ * Plane positions are 1-based (k_xy_plane, j_xz_plane, i_yz_plane). */
+void orc_planes(const orc *o, const cx *u, const cx *p, int n_xy, const int *k_xy, int n_xz, const int *j_xz,
+                int n_yz, const int *i_yz, double *mean, double *xy, double *xz, double *yz)
+{
+    const int nx = o->c.nx, ny = o->c.ny, nz = o->c.nz, lu = o->lu, nf = 2 * lu;
+    const int nxm = o->nxm, nym = o->nym, nzm = o->nzm, kzm = o->kzm;
+    const size_t ss = (size_t)nzm * nym * nxm, np = (size_t)nx * ny * nz;
+    cx *spec = (cx *)malloc(sizeof(cx) * ss * nf);
+    memcpy(spec, u, sizeof(cx) * ss * lu);
+    memcpy(spec + ss * lu, p, sizeof(cx) * ss);
+    const int dz_src[3] = {0, 1, 3};
+    for (int q = 0; q < lu - 1; q++) {      /* sam.f:147-156 */
+        const cx *a = u + ss * dz_src[q];
+        cx *b = spec + ss * (lu + 1 + q);
+        for (int ix = 0; ix < nxm; ix++)
+            for (int iy = 0; iy < nym; iy++)
+                for (int k = 0; k < nzm; k++) {
+                    const size_t i = k + (size_t)nzm * (iy + (size_t)nym * ix);
+                    b[i] = I * o->wave_z * o->kz[k] * a[i];
+                }
+    }
+    cx *zb = (cx *)malloc(sizeof(cx) * (size_t)nym * nxm * nz * nf);
+    cx *yb = (cx *)malloc(sizeof(cx) * (size_t)ny * o->nxh * nz * nf);
+    double *phys = (double *)malloc(sizeof(double) * np * nf);
+    orc_z_trans_b(o, spec, nzm, zb, nf);
+    orc_x2z(o, zb, yb, nf);
+    orc_xy_trans_b(o, yb, phys, nf);
+
+    /* get_mean.f:10-20 */
+    double *um = (double *)malloc(sizeof(double) * nz);
+    for (int n = 0; n < lu; n++) {
+        const cx *a = u + ss * n;
+        um[0] = creal(a[0]);
+        for (int k = 1; k <= kzm; k++) { um[2 * k - 1] = creal(a[k]); um[2 * k] = cimag(a[k]); }
+        for (int k = 2 * kzm + 1; k < nz; k++) um[k] = 0.0;
+        orc_rfft(nz, um, 1);
+        memcpy(mean + (size_t)nz * n, um, sizeof(double) * nz);
+    }
+    free(um);
+
+    const size_t pl = (size_t)nx * ny;
+    double *du = (double *)malloc(sizeof(double) * pl);
+    for (int m = 0; m < n_xy; m++) {          /* write_planes.f:38-48 */
+        const int k = k_xy[m] - 1;
+        for (int n = 0; n < nf; n++)
+            memcpy(xy + pl * (n + (size_t)nf * m), phys + pl * (k + (size_t)nz * n), sizeof(double) * pl);
+    }
+    for (int m = 0; m < n_xz; m++) {          /* write_planes.f:58-100 */
+        const int j = j_xz[m] - 1;
+        double *out = xz + (size_t)nx * nz * nf * m;
+        for (int n = 0; n <= lu; n++)
+            for (int k = 0; k < nz; k++)
+                for (int i = 0; i < nx; i++)
+                    out[i + (size_t)nx * (k + (size_t)nz * n)] = phys[i + nx * (j + (size_t)ny * (k + (size_t)nz * n))];
+        for (int k = 0; k < nz; k++) {
+            int L = 1;
+            for (int n = 0; n < lu; n++) {
+                if (n == 1) continue;
+                L++;
+                plane_y_deriv(o, phys + pl * (k + (size_t)nz * n), du);
+                for (int i = 0; i < nx; i++) out[i + (size_t)nx * (k + (size_t)nz * (lu + L - 1))] = du[i + (size_t)nx * j];
+            }
+        }
+    }
+    for (int m = 0; m < n_yz; m++) {          /* write_planes.f:110-144 */
+        const int i = i_yz[m] - 1;
+        double *out = yz + (size_t)ny * nz * nf * m;
+        for (int n = 0; n <= lu; n++)
+            for (int k = 0; k < nz; k++)
+                for (int j = 0; j < ny; j++)
+                    out[j + (size_t)ny * (k + (size_t)nz * n)] = phys[i + nx * (j + (size_t)ny * (k + (size_t)nz * n))];
+        for (int k = 0; k < nz; k++) {
+            int L = 1;
+            for (int n = 0; n < lu; n++) {
+                if (n == 0) continue;
+                L++;
+                plane_x_deriv(o, phys + pl * (k + (size_t)nz * n), du);
+                for (int j = 0; j < ny; j++) out[j + (size_t)ny * (k + (size_t)nz * (lu + L - 1))] = du[i + (size_t)nx * j];
+            }
+        }
+    }
+    free(du); free(spec); free(zb); free(yb); free(phys);
 }

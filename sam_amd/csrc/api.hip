@@ -75,6 +75,9 @@ struct sam_handle {
     int k_force = -1;
     double *ek0 = nullptr, *ekg = nullptr;
     int ek0_valid = 0;
+    // planes output: the last substep keeps its pressure (sam.f:148) when on
+    int keep_p = 0;
+    DevBuf planes_spec, planes_out;
     void *tw[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
     int have_spectra = 0;
     int sgs_active = 0;
@@ -203,6 +206,8 @@ extern "C" int sam_destroy(sam_handle *h)
     hipFree(h->Zb);
     hipFree(h->Yb);
     hipFree(h->Pb.p);
+    hipFree(h->planes_spec.p);
+    hipFree(h->planes_out.p);
     hipFree(h->stage0.p);
     hipFree(h->stage1.p);
     hipFree(h->sgsw.p);
@@ -634,7 +639,8 @@ template <typename T> static int substep(sam_handle *h, int m, int spectra_now)
     if (rc) return rc;
     {
         PROF(SAM_K_ZRHS);
-        RCHK(launch_zrhs<T>(d, h->k, (const T *)h->Zb, h->U[h->cur], h->R, h->U[1 - h->cur], h->diag, m, 0,
+        const int mode = (h->keep_p && m == h->p.nrk_max - 1) ? 2 : 0;
+        RCHK(launch_zrhs<T>(d, h->k, (const T *)h->Zb, h->U[h->cur], h->R, h->U[1 - h->cur], h->diag, m, mode,
                             h->tw[2][h->prec], h->st));
     }
     if (spectra_now) {
@@ -715,6 +721,91 @@ extern "C" int sam_set_cl(sam_handle *h, int mode)
         return set_err(SAM_ERR_UNSUPPORTED, "k_cl = int(Nx/3) lies beyond the stored modes (cl_energy.f would read past u)");
     h->k.cl = mode;
     return SAM_OK;
+}
+
+extern "C" int sam_set_planes(sam_handle *h, int keep_pressure)
+{
+    if (!h) return set_err(SAM_ERR_VALUE, "null handle");
+    h->keep_p = keep_pressure ? 1 : 0;
+    return SAM_OK;
+}
+
+// one batch of planes fields: spectral fields -> z, y, x inverse transforms
+// -> physical buffer Pb (nf fields); the Z column (ky = kx = 0) of the
+// first Lu fields is the horizontal mean (get_mean.f)
+template <typename T> static int planes_batch(sam_handle *h, int batch, int nf, std::vector<cpx<T>> *meancol)
+{
+    const Dims &d = h->d;
+    const int pi = h->prec;
+    RCHK(launch_planes_spec(d, batch, h->U[h->cur], h->R, (double *)h->planes_spec.p, h->st));
+    RCHK(launch_zb<T>(d, (const double *)h->planes_spec.p, d.nzm, nf, (T *)h->Zb, h->tw[2][pi], h->st));
+    if (meancol) {
+        meancol->resize((size_t)d.nz * d.lu);
+        for (int f = 0; f < d.lu; f++)
+            HIPCHK(hipMemcpyAsync(meancol->data() + (size_t)d.nz * f, (const cpx<T> *)h->Zb + (size_t)f * d.zfield,
+                                  sizeof(cpx<T>) * d.nz, hipMemcpyDeviceToHost, h->st));
+    }
+    RCHK(launch_yb<T>(d, (const T *)h->Zb, nf, (T *)h->Yb, h->tw[1][pi], h->st));
+    RCHK(launch_xb<T>(d, (const T *)h->Yb, nf, (T *)h->Pb.p, h->tw[0][pi], h->st));
+    return SAM_OK;
+}
+
+template <typename T>
+static int planes_impl(sam_handle *h, int n_xy, const int *k_xy, int n_xz, const int *j_xz, int n_yz, const int *i_yz,
+                       double *mean, double *xy, double *xz, double *yz)
+{
+    const Dims &d = h->d;
+    const int lu = d.lu, nf = 2 * lu;
+    const size_t pxy = (size_t)d.nx * d.ny, pxz = (size_t)d.nx * d.nz, pyz = (size_t)d.ny * d.nz;
+    const size_t oxz = pxy * nf * n_xy, oyz = oxz + pxz * nf * n_xz, ntot = oyz + pyz * nf * n_yz;
+    int rc = ensure(h->planes_spec, sizeof(double) * 2 * d.ss * nf);
+    if (!rc) rc = ensure(h->Pb, sizeof(T) * d.pfield * nf);
+    if (!rc) rc = ensure(h->planes_out, sizeof(double) * (ntot ? ntot : 1));
+    if (rc) return rc;
+    double *out = (double *)h->planes_out.p;
+    std::vector<cpx<T>> col;
+    rc = planes_batch<T>(h, 0, nf, &col);
+    if (rc) return rc;
+    const T *P = (const T *)h->Pb.p;
+    for (int m = 0; m < n_xy; m++) RCHK(launch_plane_gather<T>(d, P, 0, k_xy[m] - 1, 0, nf, out + pxy * nf * m, 0, h->st));
+    for (int m = 0; m < n_xz; m++)
+        RCHK(launch_plane_gather<T>(d, P, 1, j_xz[m] - 1, 0, lu + 1, out + oxz + pxz * nf * m, 0, h->st));
+    for (int m = 0; m < n_yz; m++)
+        RCHK(launch_plane_gather<T>(d, P, 2, i_yz[m] - 1, 0, lu + 1, out + oyz + pyz * nf * m, 0, h->st));
+    if (n_xz || n_yz) {
+        rc = planes_batch<T>(h, 1, 2 * (lu - 1), nullptr);
+        if (rc) return rc;
+        for (int m = 0; m < n_xz; m++)     // d/dy of u, w, [T]
+            RCHK(launch_plane_gather<T>(d, P, 1, j_xz[m] - 1, 0, lu - 1, out + oxz + pxz * nf * m, lu + 1, h->st));
+        for (int m = 0; m < n_yz; m++)     // d/dx of v, w, [T]
+            RCHK(launch_plane_gather<T>(d, P, 2, i_yz[m] - 1, lu - 1, lu - 1, out + oyz + pyz * nf * m, lu + 1, h->st));
+    }
+    HIPCHK(hipStreamSynchronize(h->st));
+    if (n_xy) HIPCHK(hipMemcpy(xy, out, sizeof(double) * pxy * nf * n_xy, hipMemcpyDeviceToHost));
+    if (n_xz) HIPCHK(hipMemcpy(xz, out + oxz, sizeof(double) * pxz * nf * n_xz, hipMemcpyDeviceToHost));
+    if (n_yz) HIPCHK(hipMemcpy(yz, out + oyz, sizeof(double) * pyz * nf * n_yz, hipMemcpyDeviceToHost));
+    if (mean)
+        for (int f = 0; f < lu; f++)
+            for (int z = 0; z < d.nz; z++) mean[(size_t)d.nz * f + z] = (double)col[(size_t)d.nz * f + z].x;
+    return SAM_OK;
+}
+
+extern "C" int sam_planes(sam_handle *h, int n_xy, const int *k_xy, int n_xz, const int *j_xz, int n_yz,
+                          const int *i_yz, double *mean, double *xy, double *xz, double *yz)
+{
+    if (!h) return set_err(SAM_ERR_VALUE, "null handle");
+    if (n_xy < 0 || n_xz < 0 || n_yz < 0 || n_xy > 9 || n_xz > 9 || n_yz > 9)
+        return set_err(SAM_ERR_VALUE, "0..9 planes of each kind (set_labels.f k_xy_plane1..9)");
+    if ((n_xy && (!k_xy || !xy)) || (n_xz && (!j_xz || !xz)) || (n_yz && (!i_yz || !yz)))
+        return set_err(SAM_ERR_VALUE, "null plane list or output");
+    for (int m = 0; m < n_xy; m++)
+        if (k_xy[m] < 1 || k_xy[m] > h->d.nz) return set_err(SAM_ERR_VALUE, "k_xy_plane out of 1..Nz");
+    for (int m = 0; m < n_xz; m++)
+        if (j_xz[m] < 1 || j_xz[m] > h->d.ny) return set_err(SAM_ERR_VALUE, "j_xz_plane out of 1..Ny");
+    for (int m = 0; m < n_yz; m++)
+        if (i_yz[m] < 1 || i_yz[m] > h->d.nx) return set_err(SAM_ERR_VALUE, "i_yz_plane out of 1..Nx");
+    return h->prec ? planes_impl<float>(h, n_xy, k_xy, n_xz, j_xz, n_yz, i_yz, mean, xy, xz, yz)
+                   : planes_impl<double>(h, n_xy, k_xy, n_xz, j_xz, n_yz, i_yz, mean, xy, xz, yz);
 }
 
 extern "C" int sam_get_spectra(sam_handle *h, double *out)

@@ -65,7 +65,8 @@ template <typename T> int launch_zf(const Dims &d, const T *Z, int nvar, double 
 // fused x-row kernel: inverse x, products, forward x (+ vel_max partials)
 template <typename T> int launch_xrow(const Dims &d, T *Y, int want_stats, double *partials, DevDiag *diag,
                                       const void *twx, hipStream_t st, int *nblocks_out);
-// fused z-forward + projection + RK/IF update (mode 0) or RHS only (mode 1)
+// fused z-forward + projection + RK/IF update (mode 0; mode 2 also keeps the
+// pressure in R's field Lu) or RHS only with the pressure (mode 1)
 template <typename T> int launch_zrhs(const Dims &d, const StepConsts &k, const T *Z, const double *U, double *R,
                                       double *Unew, const DevDiag *diag, int m, int mode, const void *twz,
                                       hipStream_t st);
@@ -139,6 +140,14 @@ template <typename T> int launch_box_filter(const Dims &d, const T *U, int r, fl
 int launch_coarse_features(const float *C, int nx, int ny, int nz, int lu, float dxi, float dyi, float dzi,
                            float *feats, hipStream_t st);
 int launch_pool_norms(const float *feats, const float *labels, long long n, double *part, float *norms, hipStream_t st);
+
+// planes output (write_planes.f): spectral fields of one batch (S layout,
+// ld Nz_min) -- batch 0: u v w [T] p d/dz(u v [T]); batch 1: d/dy(u w [T])
+// d/dx(v w [T]) -- and the gather of one plane from a physical buffer
+int launch_planes_spec(const Dims &d, int batch, const double *U, const double *R, double *out, hipStream_t st);
+template <typename T>
+int launch_plane_gather(const Dims &d, const T *P, int kind, int pos, int src_f0, int nfld, double *out, int dst_f0,
+                        hipStream_t st);
 
 // plain 16-byte-per-thread copy (counter calibration)
 int launch_copy16(const void *src, void *dst, long long n16, hipStream_t st);

@@ -994,7 +994,7 @@ __global__ __launch_bounds__(LR * N / EPT) void k_zrhs(const T *__restrict__ Zr,
     const double inv = 1.0 / (double)N;
     const long long ss = d.ss;
     double dt = 0, dtf1 = 0, dtf2 = 0, vf = 0;
-    if (mode == 0) {
+    if (mode != 1) {
         dt = diag->dt;
         dtf2 = k.zeta[m] * dt;
         dtf1 = k.gamma[m] * dt;
@@ -1007,8 +1007,8 @@ __global__ __launch_bounds__(LR * N / EPT) void k_zrhs(const T *__restrict__ Zr,
         cpx<double> us[LU], rp[LU];
 #pragma unroll
         for (int n = 0; n < LU; n++) {
-            us[n] = (mode == 0 || n >= 2) ? U[n * ss + si] : cpx<double>{0, 0};
-            rp[n] = (mode == 0 && m > 0) ? R[n * ss + si] : cpx<double>{0, 0};
+            us[n] = (mode != 1 || n >= 2) ? U[n * ss + si] : cpx<double>{0, 0};
+            rp[n] = (mode != 1 && m > 0) ? R[n * ss + si] : cpx<double>{0, 0};
         }
         cpx<double> F[LR];
 #pragma unroll
@@ -1025,7 +1025,7 @@ __global__ __launch_bounds__(LR * N / EPT) void k_zrhs(const T *__restrict__ Zr,
         }
         cpx<double> r[4], pr;
         rhs_mode<LU>(F, rkx, rky, rkz, rk_sq, wsav, tsav, k, r, pr);
-        if (mode == 0) {
+        if (mode != 1) {
             // Chollet-Lesieur nu_t(k) in the integrating factor (an extension:
             // SAM applies the model to its diagnostics only)
             double vt = 0.0;
@@ -1043,6 +1043,7 @@ __global__ __launch_bounds__(LR * N / EPT) void k_zrhs(const T *__restrict__ Zr,
                 Un[n * ss + si] = {u.x * vis_fac, u.y * vis_fac};
                 R[n * ss + si] = r[n];
             }
+            if (mode == 2) R[LU * ss + si] = pr;   // kept for the planes output
         } else {
 #pragma unroll
             for (int n = 0; n < LU; n++) R[n * ss + si] = r[n];
@@ -1556,6 +1557,87 @@ int launch_force_scale(const Dims &d, int kf, double *U, const double *ek0, cons
     hipLaunchKernelGGL(k_force_scale, dim3(nb), dim3(256), 0, st, U, d, kf, ek0, ekg);
     return launched();
 }
+
+// ---------------------------------------------------------------------
+// Planes output (write_planes.f:18-144, sam.f:145-167).  The spectral
+// fields of a batch are formed from the state and the kept pressure; the
+// derivatives are i k f in spectral space, which for these band-limited
+// fields equals x_deriv.f / y_deriv.f's per-line transforms.
+// ---------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_planes_spec(const double *__restrict__ Ur, const double *__restrict__ Rr,
+                                                      Dims d, int batch, double *__restrict__ outr)
+{
+    const int lu = d.lu, nf = batch == 0 ? 2 * lu : 2 * (lu - 1);
+    const cpx<double> *U = reinterpret_cast<const cpx<double> *>(Ur);
+    const cpx<double> *R = reinterpret_cast<const cpx<double> *>(Rr);
+    cpx<double> *out = reinterpret_cast<cpx<double> *>(outr);
+    const long long n = (long long)nf * d.ss;
+    for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < n; e += (long long)gridDim.x * 256) {
+        const int f = (int)(e / d.ss);
+        const long long si = e % d.ss;
+        const int kz = (int)(si % d.nzm), iy = (int)((si / d.nzm) % d.nym), ix = (int)(si / ((long long)d.nzm * d.nym));
+        cpx<double> v;
+        if (batch == 0) {
+            if (f < lu) {
+                v = U[f * d.ss + si];
+            } else if (f == lu) {
+                v = R[lu * d.ss + si];
+            } else {
+                const int q = f - lu - 1, src = q < 2 ? q : 3;    // u, v, [T]
+                const cpx<double> a = U[src * d.ss + si];
+                const double k = d.wave_z * kwave(kz, d.kzm, d.nzm);
+                v = {-k * a.y, k * a.x};
+            }
+        } else {
+            const int q = f < lu - 1 ? f : f - (lu - 1);
+            const bool dy = f < lu - 1;
+            const int src = dy ? (q == 0 ? 0 : q + 1) : q + 1;  // y: u, w, [T]; x: v, w, [T]
+            const cpx<double> a = U[src * d.ss + si];
+            const double k = dy ? d.wave_y * kwave(iy, d.kym, d.nym) : d.wave_x * ix;
+            v = {-k * a.y, k * a.x};
+        }
+        out[e] = v;
+    }
+}
+
+int launch_planes_spec(const Dims &d, int batch, const double *U, const double *R, double *out, hipStream_t st)
+{
+    const long long n = (long long)(batch == 0 ? 2 * d.lu : 2 * (d.lu - 1)) * d.ss;
+    const unsigned nb = (unsigned)std::min<long long>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_planes_spec, dim3(nb), dim3(256), 0, st, U, R, d, batch, out);
+    return launched();
+}
+
+// one plane of fields src_f0 .. src_f0 + nfld - 1 of the physical buffer
+// P (Nx, Ny, Nz, F) into out (fields dst_f0 ..): kind 0 xy at z = pos,
+// (Nx, Ny) per field; 1 xz at y = pos, (Nx, Nz); 2 yz at x = pos, (Ny, Nz)
+template <typename T>
+__global__ __launch_bounds__(256) void k_plane_gather(const T *__restrict__ P, Dims d, int kind, int pos, int src_f0,
+                                                       int nfld, double *__restrict__ out, int dst_f0)
+{
+    const long long nx = d.nx, ny = d.ny, nz = d.nz;
+    const long long per = kind == 0 ? nx * ny : kind == 1 ? nx * nz : ny * nz;
+    for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < per * nfld; e += (long long)gridDim.x * 256) {
+        const long long n = e / per, r = e % per;
+        long long i, j, k;
+        if (kind == 0) { i = r % nx; j = r / nx; k = pos; }
+        else if (kind == 1) { i = r % nx; k = r / nx; j = pos; }
+        else { j = r % ny; k = r / ny; i = pos; }
+        out[(dst_f0 + n) * per + r] = (double)P[i + nx * (j + ny * (k + nz * (src_f0 + n)))];
+    }
+}
+
+template <typename T>
+int launch_plane_gather(const Dims &d, const T *P, int kind, int pos, int src_f0, int nfld, double *out, int dst_f0,
+                        hipStream_t st)
+{
+    const long long per = kind == 0 ? (long long)d.nx * d.ny : kind == 1 ? (long long)d.nx * d.nz : (long long)d.ny * d.nz;
+    const unsigned nb = (unsigned)std::min<long long>((per * nfld + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_plane_gather<T>, dim3(nb), dim3(256), 0, st, P, d, kind, pos, src_f0, nfld, out, dst_f0);
+    return launched();
+}
+template int launch_plane_gather<double>(const Dims &, const double *, int, int, int, int, double *, int, hipStream_t);
+template int launch_plane_gather<float>(const Dims &, const float *, int, int, int, int, double *, int, hipStream_t);
 
 // ---------------------------------------------------------------------
 // precision conversion helpers

@@ -21,6 +21,9 @@
 !   mlp_refresh  i_les 4, mlp_labels 1: steps between C Delta^2 updates (20)
 !   cl_couple    i_les 1: 0 (default) the Chollet-Lesieur nu_t(k) enters the
 !                spectra only, as in SAM; 1 also the integrating factor
+! n_skip_p: planes every n_skip_p steps (write_planes.f): time.out, mean.out,
+!           xy[m].dat, xz[m].dat, yz[m].dat at k_xy_plane[m], j_xz_plane[m],
+!           i_yz_plane[m] (first plane of each kind N/2 when not given)
 ! i_force 1: force.f after every step (shells 2..k_force+1 held at the last
 !            spectrum written to spectra.out)
 ! i_les: 0 DNS, 1 Chollet-Lesieur (diagnostics of spectra.f; the step is DNS),
@@ -53,6 +56,11 @@ program samhip_driver
   real(8) :: time, dt, t_stat
   integer :: n_skip_h, n_skip_v, n_steps, i_restart, nt_restart, i_les, i_strat
   logical :: appendf
+  ! planes output (write_planes.f)
+  logical :: planes_on
+  integer :: n_skip_p, n_frame_p, n_xy, n_xz, n_yz
+  integer(c_int) :: k_xy(9), j_xz(9), i_yz(9)
+  real(c_double), allocatable :: pmean(:, :), pxy(:, :, :, :), pxz(:, :, :, :), pyz(:, :, :, :)
 
   fin = 'input.dat'
   if (command_argument_count() >= 1) call get_command_argument(1, fin)
@@ -104,6 +112,15 @@ program samhip_driver
      rc = sam_set_forcing(h, int(iv('k_force'), c_int))
      if (rc /= SAM_OK) call fail(int(rc), 'sam_set_forcing: '//trim(sam_error_message()))
   end if
+  ! planes frames at every nt with mod(nt, n_skip_p) = 0 (n_skip_p defaults
+  ! to 1e9: a fresh start still writes time.out and mean.out at nt = 0, as
+  ! SAM does); the pressure is kept only when n_skip_p is given
+  n_skip_p = iv('n_skip_p')
+  planes_on = .true.
+  if (found(idx('n_skip_p'))) then
+     rc = sam_set_planes(h, 1_c_int)
+     if (rc /= SAM_OK) call fail(int(rc), 'sam_set_planes: '//trim(sam_error_message()))
+  end if
   rc = sam_get_dims(h, d)
   lu = d%lu
   lm = d%lm
@@ -121,6 +138,7 @@ program samhip_driver
   if (ierr /= 0) call fail(ierr, 'vel.'//ext)
   time = hvalues(idx('time'))
   t_stat = hvalues(idx('t_stat'))
+  n_frame_p = nint(hvalues(idx('n_frame_p')))
   nt_start = nint(hvalues(idx('nt')))
   dt = rv('dt0')            ! assign_run_time.f:7-14 keeps the input dt
   rc = sam_set_state(h, u, int(d%nz_min, c_int))
@@ -129,10 +147,12 @@ program samhip_driver
 
   appendf = (i_restart == 1)
   call open_files(appendf)
+  if (planes_on) call open_planes()
 
   ! ---- time loop (sam.f:122-326) ---------------------------------------
   nt_end = nt_start + n_steps
   do nt = nt_start, nt_end
+     if (planes_on .and. mod(nt, n_skip_p) == 0) call write_planes_out()   ! sam.f:236-239
      want = 0
      if (mod(nt, n_skip_h) == 0 .or. nt == nt_end) want = 1
      if ((nt > 0 .and. mod(nt, n_skip_v) == 0) .or. nt == nt_end) then
@@ -158,6 +178,7 @@ program samhip_driver
   rc = sam_destroy(h)
   if (c_associated(mlp)) rc = sam_mlp_destroy(mlp)
   close (2); close (3); close (4); close (7); close (12)
+  if (planes_on) call close_planes()
 
 contains
 
@@ -433,7 +454,7 @@ contains
     integer :: un, k
     real(8) :: v(NPAR)
     v(1:NIN) = values(1:NIN)
-    v(73) = t; v(74) = dtv; v(75) = ts; v(76) = ntv; v(77) = 0; v(78) = 0
+    v(73) = t; v(74) = dtv; v(75) = ts; v(76) = ntv; v(77) = n_frame_p; v(78) = 0
     open (newunit=un, file=fname, status='replace', action='write')
     do k = 1, NPAR
        write (un, *) labels(k), v(k)
@@ -474,6 +495,101 @@ contains
     write (un) u
     close (un)
   end subroutine write_field
+
+  ! check_inputs.f:27-62 (default first planes at N/2, counts) and
+  ! open_files.f:24-58: xy[m].dat direct access (records of Nx*Ny doubles),
+  ! xz/yz[m].dat written at byte offsets (mpi_file_write_at there, stream
+  ! access here), mean.out direct access (records of Nz*Lu doubles),
+  ! time.out sequential from its start
+  subroutine open_planes()
+    integer :: m
+    character(len=1) :: e1
+    k_xy = [(int(iv('k_xy_plane'//achar(48 + m)), c_int), m = 1, 9)]
+    j_xz = [(int(iv('j_xz_plane'//achar(48 + m)), c_int), m = 1, 9)]
+    i_yz = [(int(iv('i_yz_plane'//achar(48 + m)), c_int), m = 1, 9)]
+    if (found(idx('n_skip_p'))) then
+       if (k_xy(1) == 0) k_xy(1) = p%nz / 2
+       if (j_xz(1) == 0) j_xz(1) = p%ny / 2
+       if (i_yz(1) == 0) i_yz(1) = p%nx / 2
+    end if
+    n_xy = count_planes(k_xy)
+    n_xz = count_planes(j_xz)
+    n_yz = count_planes(i_yz)
+    allocate(pmean(p%nz, lu), pxy(p%nx, p%ny, 2 * lu, max(n_xy, 1)), pxz(p%nx, p%nz, 2 * lu, max(n_xz, 1)), &
+         pyz(p%ny, p%nz, 2 * lu, max(n_yz, 1)))
+    open (unit=10, file='time.out', position='rewind')
+    open (unit=11, file='mean.out', form='unformatted', access='direct', recl=p%nz * lu * 8)
+    do m = 1, n_xy
+       write (e1, '(i1.1)') m
+       open (unit=20 + m, file='xy'//e1//'.dat', form='unformatted', access='direct', recl=p%nx * p%ny * 8)
+    end do
+    do m = 1, n_xz
+       write (e1, '(i1.1)') m
+       open (unit=30 + m, file='xz'//e1//'.dat', form='unformatted', access='stream')
+    end do
+    do m = 1, n_yz
+       write (e1, '(i1.1)') m
+       open (unit=40 + m, file='yz'//e1//'.dat', form='unformatted', access='stream')
+    end do
+  end subroutine open_planes
+
+  integer function count_planes(a)
+    integer(c_int), intent(in) :: a(9)
+    integer :: m
+    count_planes = 9
+    do m = 1, 9
+       if (a(m) == 0) then
+          count_planes = m - 1
+          return
+       end if
+    end do
+  end function count_planes
+
+  ! write_planes.f:18-144 for one frame of the current state
+  subroutine write_planes_out()
+    integer :: m, n, nf
+    integer(8) :: blk, off
+    nf = 2 * lu
+    rc = sam_planes(h, int(n_xy, c_int), k_xy, int(n_xz, c_int), j_xz, int(n_yz, c_int), i_yz, pmean, pxy, pxz, pyz)
+    if (rc /= SAM_OK) call fail(int(rc), 'sam_planes: '//trim(sam_error_message()))
+    n_frame_p = n_frame_p + 1
+    write (10, '(i6,1p,e12.4)') nt, time
+    flush (10)
+    write (11, rec=n_frame_p) pmean
+    do m = 1, n_xy
+       do n = 1, nf
+          write (20 + m, rec=(n_frame_p - 1) * nf + n) pxy(:, :, n, m)
+       end do
+    end do
+    blk = int(p%nx, 8) * int(p%nz, 8) * int(nf * 8, 8)
+    do m = 1, n_xz
+       do n = 1, nf
+          off = int(n_frame_p - 1, 8) * blk + int(n - 1, 8) * int(p%nx, 8) * int(p%nz, 8) * 8
+          write (30 + m, pos=off + 1) pxz(:, :, n, m)
+       end do
+    end do
+    blk = int(p%ny, 8) * int(p%nz, 8) * int(nf * 8, 8)
+    do m = 1, n_yz
+       do n = 1, nf
+          off = int(n_frame_p - 1, 8) * blk + int(n - 1, 8) * int(p%ny, 8) * int(p%nz, 8) * 8
+          write (40 + m, pos=off + 1) pyz(:, :, n, m)
+       end do
+    end do
+  end subroutine write_planes_out
+
+  subroutine close_planes()
+    integer :: m
+    close (10); close (11)
+    do m = 1, n_xy
+       close (20 + m)
+    end do
+    do m = 1, n_xz
+       close (30 + m)
+    end do
+    do m = 1, n_yz
+       close (40 + m)
+    end do
+  end subroutine close_planes
 
   ! open_files.f:90-130 (ascii logs, appended on restart)
   subroutine open_files(app)

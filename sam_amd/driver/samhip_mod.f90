@@ -137,6 +137,22 @@ module samhip_mod
        integer(c_int), value :: mode
      end function sam_set_cl
 
+     ! planes output (write_planes.f)
+     integer(c_int) function sam_set_planes(h, keep) bind(c, name='sam_set_planes')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: h
+       integer(c_int), value :: keep
+     end function sam_set_planes
+
+     integer(c_int) function sam_planes(h, n_xy, k_xy, n_xz, j_xz, n_yz, i_yz, mean, xy, xz, yz) &
+          bind(c, name='sam_planes')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       integer(c_int), value :: n_xy, n_xz, n_yz
+       integer(c_int), intent(in) :: k_xy(*), j_xz(*), i_yz(*)
+       real(c_double), intent(out) :: mean(*), xy(*), xz(*), yz(*)
+     end function sam_planes
+
      integer(c_int) function sam_set_mlp_refresh(h, nsteps) bind(c, name='sam_set_mlp_refresh')
        import :: c_int, c_ptr
        type(c_ptr), value :: h

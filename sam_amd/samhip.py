@@ -42,7 +42,7 @@ EXPORTS = (
     "sam_mlp_clear_pool", "sam_mlp_set_labels",
     "sam_mlp_fit_norms", "sam_mlp_train", "sam_mlp_grad", "sam_mlp_infer",
     "sam_attach_mlp", "sam_set_sgs", "sam_sgs_check", "sam_set_mlp_refresh",
-    "sam_set_forcing", "sam_set_cl",
+    "sam_set_forcing", "sam_set_cl", "sam_set_planes", "sam_planes",
 )
 
 MLP_NPARAM = 10448
@@ -154,6 +154,9 @@ def load():
     L.sam_set_mlp_refresh.argtypes = [_hp, ctypes.c_int]
     L.sam_set_forcing.argtypes = [_hp, ctypes.c_int]
     L.sam_set_cl.argtypes = [_hp, ctypes.c_int]
+    L.sam_set_planes.argtypes = [_hp, ctypes.c_int]
+    _ipt = ctypes.POINTER(ctypes.c_int)
+    L.sam_planes.argtypes = [_hp, ctypes.c_int, _ipt, ctypes.c_int, _ipt, ctypes.c_int, _ipt, _dp, _dp, _dp, _dp]
     L.sam_sgs_check.argtypes = [_hp, ctypes.c_double, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]
     for name in EXPORTS:
         if name in ("sam_last_error", "sam_mlp_pool_size"):
@@ -447,6 +450,28 @@ def set_cl(sam, mode):
     """Chollet-Lesieur model: CL_DIAG in the spectra (SAM's i_les = 1),
     CL_COUPLED also in the integrating factor"""
     _chk(load().sam_set_cl(sam.h, int(mode)))
+
+
+def set_planes(sam, keep_pressure=True):
+    """keep the last substep's pressure for the planes output"""
+    _chk(load().sam_set_planes(sam.h, int(bool(keep_pressure))))
+
+
+def planes(sam, k_xy=(), j_xz=(), i_yz=()):
+    """write_planes.f / get_mean.f of the current state: (mean (Nz, Lu),
+    xy (Nx, Ny, 2Lu, n_xy), xz (Nx, Nz, 2Lu, n_xz), yz (Ny, Nz, 2Lu, n_yz));
+    positions 1-based"""
+    p = sam.params
+    nf = 2 * sam.lu
+    ia = lambda v: (ctypes.c_int * max(len(v), 1))(*v)
+    mean = np.zeros((p.nz, sam.lu), order="F")
+    xy = np.zeros((p.nx, p.ny, nf, max(len(k_xy), 1)), order="F")
+    xz = np.zeros((p.nx, p.nz, nf, max(len(j_xz), 1)), order="F")
+    yz = np.zeros((p.ny, p.nz, nf, max(len(i_yz), 1)), order="F")
+    _chk(load().sam_planes(sam.h, len(k_xy), ia(k_xy), len(j_xz), ia(j_xz), len(i_yz), ia(i_yz),
+                           mean.ctypes.data_as(_dp), xy.ctypes.data_as(_dp), xz.ctypes.data_as(_dp),
+                           yz.ctypes.data_as(_dp)))
+    return mean, xy[..., :len(k_xy)], xz[..., :len(j_xz)], yz[..., :len(i_yz)]
 
 
 def set_mlp_refresh(sam, nsteps):

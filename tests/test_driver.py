@@ -164,3 +164,40 @@ def test_forcing_and_chollet_lesieur_run_matches_library(tmp_path):
     plain.set_state(u0)
     plain.step(10)
     assert np.linalg.norm(plain.get_state() - ref) > 1e-8 * np.linalg.norm(ref)
+
+
+@pytest.mark.gpu
+@needs_driver
+def test_planes_files(tmp_path):
+    """n_skip_p 2 over 4 steps: frames at nt = 0, 2, 4 in time.out, mean.out
+    and the xy/xz/yz files (write_planes.f layouts); the last frame equals
+    the library's planes of the same state"""
+    from sam_amd import samhip as S
+    n = 32
+    run_p = dict(TG64, Nx=n, Ny=n, Nz=n, vis=1e-3, n_steps=4, n_skip_h=2, n_skip_p=2, k_xy_plane1=5,
+                 j_xz_plane1=3, i_yz_plane1=7, i_yz_plane2=20)
+    u0 = ic.taylor_green(n)
+    sio.write_vel(str(tmp_path / "vel.000000"), u0)
+    sio.write_header(str(tmp_path / "header.000000"), run_p, time=0.0, dt=0.01, nt=0)
+    r = run(tmp_path, **run_p)
+    assert r.returncode == 0, r.stdout + r.stderr
+    nf, nfr = 6, 3
+    t = np.loadtxt(str(tmp_path / "time.out"))
+    assert list(t[:, 0]) == [0, 2, 4] and abs(t[2, 1] - 0.04) < 1e-4
+    mean = np.fromfile(str(tmp_path / "mean.out")).reshape(nfr, 3, n)
+    xy = np.fromfile(str(tmp_path / "xy1.dat")).reshape(nfr, nf, n, n)
+    xz = np.fromfile(str(tmp_path / "xz1.dat")).reshape(nfr, nf, n, n)
+    yz2 = np.fromfile(str(tmp_path / "yz2.dat")).reshape(nfr, nf, n, n)
+    assert not (tmp_path / "xy2.dat").exists()
+    sam = S.Sam(S.make_params(n, vis=1e-3, dt0=0.01))
+    S.set_planes(sam, True)
+    sam.set_state(u0)
+    sam.step(4)
+    lm, lxy, lxz, lyz = S.planes(sam, (5,), (3,), (7, 20))
+    # Fortran records are column-major: (Nx, Ny) per field -> reshape [.., y, x]
+    assert np.allclose(xy[2].transpose(0, 2, 1), np.moveaxis(lxy[..., 0], 2, 0), rtol=0, atol=1e-14)
+    assert np.allclose(xz[2].transpose(0, 2, 1), np.moveaxis(lxz[..., 0], 2, 0), rtol=0, atol=1e-14)
+    assert np.allclose(yz2[2].transpose(0, 2, 1), np.moveaxis(lyz[..., 1], 2, 0), rtol=0, atol=1e-14)
+    assert np.allclose(mean[2], lm.T, rtol=0, atol=1e-14)
+    hdr = sio.parse_labels(str(tmp_path / "header.000004"))
+    assert hdr["n_frame_p"] == 3
