@@ -201,3 +201,11 @@ def test_planes_files(tmp_path):
     assert np.allclose(mean[2], lm.T, rtol=0, atol=1e-14)
     hdr = sio.parse_labels(str(tmp_path / "header.000004"))
     assert hdr["n_frame_p"] == 3
+    # the VTK converter reads the driver's files
+    from sam_amd import planes2vtk as P2V
+    out = P2V.convert(str(tmp_path))
+    assert len(out) == 4 * 3           # xy1, xz1, yz1, yz2 x 3 frames
+    raw = open(str(tmp_path / "xy1_0002.vtk"), "rb").read()
+    k = raw.index(b"Scalars u' float\nLOOKUP_TABLE default\n") + len(b"Scalars u' float\nLOOKUP_TABLE default\n")
+    vals = np.frombuffer(raw[k:k + 4 * 33 * 33], dtype=">f4").reshape(33, 33)     # [y, x]
+    assert np.allclose(vals[:32, :32], xy[2, 0], atol=1e-6)
