@@ -1209,9 +1209,52 @@ __global__ __launch_bounds__(FIN_THREADS) void k_finalize(const double *__restri
     }
 }
 
+// first level of the statistics reduction for many partial rows: block g
+// reduces the rows of its contiguous chunk in a fixed order to one row
+constexpr int FIN_G = 256;
+__global__ __launch_bounds__(256) void k_finalize_partial(const double *__restrict__ partials, int nblocks, int lu,
+                                                          double *__restrict__ out)
+{
+    __shared__ double red[4][12];
+    const int width = 3 * lu, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int chunk = (nblocks + gridDim.x - 1) / gridDim.x;
+    const int b0 = blockIdx.x * chunk, b1 = min(nblocks, b0 + chunk);
+    double acc[12];
+#pragma unroll
+    for (int f = 0; f < 12; f++) acc[f] = 0.0;
+    for (int b = b0 + threadIdx.x; b < b1; b += 256) {
+        const double *r = partials + (long long)b * width;
+#pragma unroll
+        for (int f = 0; f < 12; f++)
+            if (f < width) acc[f] = f < 2 * lu ? acc[f] + r[f] : fmax(acc[f], r[f]);
+    }
+#pragma unroll
+    for (int f = 0; f < 12; f++) {
+        if (f < width) {
+            const double v = f < 2 * lu ? wave_sum(acc[f]) : wave_max(acc[f]);
+            if (lane == 0) red[w][f] = v;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < width) {
+        const bool is_max = threadIdx.x >= 2 * lu;
+        double t = red[0][threadIdx.x];
+        for (int i = 1; i < 4; i++) t = is_max ? fmax(t, red[i][threadIdx.x]) : t + red[i][threadIdx.x];
+        out[(long long)blockIdx.x * width + threadIdx.x] = t;
+    }
+}
+
 int launch_finalize(const Dims &d, const StepConsts &k, const double *partials, int nblocks, DevDiag *diag,
                     int update_dt, hipStream_t st)
 {
+    if (nblocks >= 8 * FIN_THREADS) {
+        // two levels: FIN_G blocks reduce chunks into rows kept past every
+        // other use of the partials buffer, then one block finishes
+        double *rows = const_cast<double *>(partials) + (partials_capacity(d) - (long long)FIN_G * 12);
+        hipLaunchKernelGGL(k_finalize_partial, dim3(FIN_G), dim3(256), 0, st, partials, nblocks, d.lu, rows);
+        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(FIN_THREADS), 0, st, rows, FIN_G, d.lu, diag, k, d, update_dt);
+        return launched();
+    }
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(FIN_THREADS), 0, st, partials, nblocks, d.lu, diag, k, d, update_dt);
     return launched();
 }
@@ -1685,7 +1728,7 @@ long long partials_capacity(const Dims &d)
     long long m = a > b ? a : b;
     m = m > c ? m : c;
     m = m > e ? m : e;
-    return m + 1024;
+    return m + 1024 + 256 * 12;   // + the rows of the two-level statistics reduction
 }
 
 #define SAM_INST(T)                                                                                                   \
