@@ -184,6 +184,21 @@ def test_long_x_rows(oracle_mod, prec, tol):
     assert err < tol, err
 
 
+@pytest.mark.parametrize("ny", [256, 512])
+def test_long_y_columns(oracle_mod, ny):
+    """long y columns (Ny = 256, 512; 8 z per y-transform tile): full
+    stratified FP64 steps on a 32 x Ny x 32 box against the oracle"""
+    nsteps = 2
+    o, s = pair(oracle_mod, 32, ny, 32, xl=2.0, yl=ny / 16.0, zl=2.0, i_strat=1, grav=16.0, lapse0=1.0, vis=5e-3,
+                i_cfl=1, cfl0=0.5, prec=S.PREC_FP64)
+    u0 = ic.random_solenoidal(32, ny, 32, k0=4, seed=10, i_strat=1, xl=2.0, yl=ny / 16.0, zl=2.0)
+    s.set_state(u0)
+    s.step(nsteps)
+    ref, t, dt, st = o.step(u0, nsteps)
+    err = rel_l2(s.get_state(), ref)
+    assert err < 1e-12, err
+
+
 def test_split_precision_bounded(oracle_mod):
     """FP32-split: transforms/products in FP32, state/RK in FP64"""
     o, s = pair(oracle_mod, 32, i_strat=1, grav=16.0, lapse0=1.0, vis=5e-3, i_cfl=1, cfl0=0.5,
