@@ -185,7 +185,7 @@ int  sam_eddy_visc(sam_handle *h, const double *phys, double *vt, double *c_delt
 int  sam_test_filter(sam_handle *h, const double *in, double *out, int nvar);
 
 /* ---- learned SGS closure (no counterpart in the reference) ------------ */
-/* Per-point MLP 12 -> 64 -> 64 -> 64 -> 9 (GELU), BF16 MFMA / FP32 accumulate.
+/* Per-point MLP 12 -> 64 -> 64 -> 64 -> 9 (ReLU), BF16 MFMA / FP32 accumulate.
  * Inputs: du_i/dx_j (9, row-major) and dT'/dx_j (3) by centred differences;
  * outputs: deviatoric tau_11,22,33,12,23,13 and q_1..3; the step then uses
  * F_ij = u_i u_j + tau_ij and F_Tj = u_j T' + q_j. */

@@ -3,10 +3,10 @@
 TEST INFRASTRUCTURE ONLY (tests/, never the product path).  The closure has
 no reference counterpart; this restates the GPU network exactly:
 
-  12 -> 64 -> 64 -> 64 -> 9, GELU (tanh form), inputs padded to 16 and
+  12 -> 64 -> 64 -> 64 -> 9, ReLU, inputs padded to 16 and
   outputs to 16; every GEMM operand (weights, activations, output and hidden
   gradients) rounded to BF16 (round to nearest even), FP32 accumulation, FP32
-  bias, GELU and Adam.
+  bias, activation and Adam.
 
   Scale-free form: the scaled features x = f * fs of a point enter as the unit
   vector x / |x|, and the network's output is multiplied by |x|^2 (so the
@@ -50,11 +50,9 @@ def pack_grads(g):
     return out
 
 
-def gelu(x):
-    c, a = 0.7978845608028654, 0.044715
-    t = np.tanh(c * (x + a * x ** 3))
-    dg = 0.5 * (1 + t) + 0.5 * x * (1 - t * t) * c * (1 + 3 * a * x * x)
-    return 0.5 * x * (1 + t), dg
+def activation(x):
+    """hidden activation (ReLU) and its derivative (1 for x > 0, else 0)"""
+    return np.maximum(x, 0.0), (x > 0).astype(np.float64)
 
 
 class Net:
@@ -85,7 +83,7 @@ class Net:
         a = act["a0"]
         for L, (W, b) in enumerate((("W1", "b1"), ("W2", "b2"), ("W3", "b3")), start=1):
             z = (a.astype(dt) @ self.r(self.w[W].astype(dt)).T + self.w[b].astype(dt)).astype(dt)
-            g, dg = gelu(z.astype(np.float64))
+            g, dg = activation(z.astype(np.float64))
             act[f"z{L}"] = z
             act[f"dg{L}"] = dg.astype(dt)
             a = self.r(g.astype(dt))

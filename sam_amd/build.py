@@ -29,7 +29,7 @@ CFLAGS = [
 
 
 # per-file extras: the learned-closure kernels keep MFMA results in VGPRs
-# (no v_accvgpr_read per result element before the GELU)
+# (no v_accvgpr_read per result element before the activation)
 EXTRA = {"mlp.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 
 

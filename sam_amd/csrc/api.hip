@@ -456,7 +456,7 @@ template <typename T> static int les_bufs(sam_handle *h, LesBufs &b, bool dynami
     const size_t f = sizeof(T) * d.pfield;
     int rc = ensure(h->Pb, f * (d.lu + d.lr));
     if (rc) return rc;
-    rc = ensure(h->sgsw, f * (dynamic ? 41 : 1));
+    rc = ensure(h->sgsw, f * (dynamic ? 46 : 1));   // tmp: 6 fields (one filter call)
     if (rc) return rc;
     char *w = (char *)h->sgsw.p;
     b.smag = w;
@@ -1030,9 +1030,9 @@ extern "C" int sam_eddy_visc(sam_handle *h, const double *phys, double *vt, doub
 template <typename T> static int filter_stage(sam_handle *h, const double *in, double *out, int nvar)
 {
     const Dims &d = h->d;
-    int rc = ensure(h->Pb, sizeof(T) * d.pfield * (2 * nvar + 1));
+    int rc = ensure(h->Pb, sizeof(T) * d.pfield * (3 * nvar));
     if (rc) return rc;
-    T *A = (T *)h->Pb.p, *B = A + d.pfield * nvar, *tmp = B + d.pfield * nvar;
+    T *A = (T *)h->Pb.p, *B = A + d.pfield * nvar, *tmp = B + d.pfield * nvar;   // tmp: nvar fields
     rc = upload_T<T>(h, A, in, d.pfield * nvar);
     if (rc) return rc;
     RCHK(launch_filter<T>(d, A, B, tmp, nvar, h->st));

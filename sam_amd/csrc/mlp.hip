@@ -1,4 +1,4 @@
-// mlp.hip -- learned per-point SGS closure: 12 -> 64 -> 64 -> 64 -> 9, GELU.
+// mlp.hip -- learned per-point SGS closure: 12 -> 64 -> 64 -> 64 -> 9, ReLU.
 //
 // Inputs (12): resolved velocity gradient du_i/dx_j (9, row-major i,j) and
 // dT'/dx_j (3), centred differences on the collocated grid (the same
@@ -67,23 +67,14 @@ __device__ __forceinline__ float point_r2(v4f x)
 }
 __device__ __forceinline__ float inv_radius(float r2) { return r2 > 0.f ? 1.0f / sqrtf(r2) : 0.f; }
 
-// GELU, tanh form: 0.5 x (1 + tanh u) with u = c (x + a x^3), evaluated as
-// x * sigmoid(2u) = x / (1 + 2^(-2u log2 e)) (one exp2, one reciprocal);
-// e -> inf for large negative x gives exactly -0, e -> 0 gives x
-constexpr float GELU_C = 0.7978845608028654f, GELU_A = 0.044715f;
-constexpr float GELU_K1 = -2.0f * GELU_C * 1.4426950408889634f, GELU_K2 = GELU_K1 * GELU_A;
-__device__ __forceinline__ float gelu_sig(float x)
-{
-    const float e = __builtin_amdgcn_exp2f(x * __builtin_fmaf(GELU_K2, x * x, GELU_K1));
-    return __builtin_amdgcn_rcpf(1.0f + e);
-}
-__device__ __forceinline__ float gelu(float x) { return x * gelu_sig(x); }
-// derivative: s + g (1 - s) 2c (1 + 3 a x^2), s = sigmoid(2u), g = x s
-__device__ __forceinline__ float gelu_deriv(float x)
-{
-    const float s = gelu_sig(x), g = x * s;
-    return __builtin_fmaf(g * (1.0f - s), __builtin_fmaf(6.0f * GELU_C * GELU_A, x * x, 2.0f * GELU_C), s);
-}
+// Hidden activation: ReLU, one instruction per value.  (A GELU needs an exp and a
+// reciprocal, quarter-rate instructions: with 48 activations per lane per
+// 16-point tile it took over half of the inference kernel's cycles.)
+// (as a signed max on the bits: a negative float, sign bit set, is a
+// negative integer; one v_max_i32 where fmaxf also quiets NaNs first)
+__device__ __forceinline__ float act(float x) { return __int_as_float(max(__float_as_int(x), 0)); }
+// derivative: 1 for x > 0, else 0
+__device__ __forceinline__ float act_deriv(float x) { return x > 0.0f ? 1.0f : 0.0f; }
 
 // ---- weight tiles in LDS ---------------------------------------------
 // tile t (64 lanes x 4 bf16): forward tiles  0..3  W1 (mb),  4..19 W2 (mb*4+kb),
@@ -149,9 +140,9 @@ struct Acts {
     v4f out;         // outputs 4g..4g+3
 };
 
-__device__ __forceinline__ v4s gelu_bf4(v4f z)
+__device__ __forceinline__ v4s act_bf4(v4f z)
 {
-    return bf4(v4f{gelu(z[0]), gelu(z[1]), gelu(z[2]), gelu(z[3])});
+    return bf4(v4f{act(z[0]), act(z[1]), act(z[2]), act(z[3])});
 }
 
 // one hidden layer 64 -> 64: z = W a + b, tiles tb..tb+15 (mb*4+kb); the
@@ -170,7 +161,7 @@ __device__ __forceinline__ void fw_hidden(const float *bias, const v4s *tiles, i
 #pragma unroll
     for (int mb = 0; mb < 4; mb++) {
         z[mb] = acc[mb];
-        aout[mb] = gelu_bf4(acc[mb]);
+        aout[mb] = act_bf4(acc[mb]);
     }
 }
 
@@ -186,7 +177,7 @@ __device__ __forceinline__ void mlp_forward(const float *bias, const v4s *tiles,
 #pragma unroll
     for (int mb = 0; mb < 4; mb++) {
         A.z[0][mb] = z[mb];
-        A.a[0][mb] = gelu_bf4(z[mb]);
+        A.a[0][mb] = act_bf4(z[mb]);
     }
     fw_hidden(bias, tiles, T_W2, BO_2, lane, A.a[0], A.z[1], A.a[1]);
     fw_hidden(bias, tiles, T_W3, BO_3, lane, A.a[1], A.z[2], A.a[2]);
@@ -196,16 +187,16 @@ __device__ __forceinline__ void mlp_forward(const float *bias, const v4s *tiles,
     A.out = o;
 }
 
-__device__ __forceinline__ v4f gelu_grad4(v4f z)
+__device__ __forceinline__ v4f act_grad4(v4f z)
 {
-    return v4f{gelu_deriv(z[0]), gelu_deriv(z[1]), gelu_deriv(z[2]), gelu_deriv(z[3])};
+    return v4f{act_deriv(z[0]), act_deriv(z[1]), act_deriv(z[2]), act_deriv(z[3])};
 }
 
-__device__ __forceinline__ void bw_gelu(const v4f *z, const v4f *da, v4s *dz)
+__device__ __forceinline__ void bw_act(const v4f *z, const v4f *da, v4s *dz)
 {
 #pragma unroll
     for (int mb = 0; mb < 4; mb++) {
-        const v4f gd = gelu_grad4(z[mb]);
+        const v4f gd = act_grad4(z[mb]);
         dz[mb] = bf4(v4f{da[mb][0] * gd[0], da[mb][1] * gd[1], da[mb][2] * gd[2], da[mb][3] * gd[3]});
     }
 }
@@ -229,11 +220,11 @@ __device__ __forceinline__ void mlp_backward(const v4s *tiles, int lane, const A
     v4f da[4];
 #pragma unroll
     for (int mb = 0; mb < 4; mb++) da[mb] = mfma(tiles[(T_W4T + mb) * 64 + lane], dout_b, v4f{0, 0, 0, 0});
-    bw_gelu(A.z[2], da, dz[2]);
+    bw_act(A.z[2], da, dz[2]);
     bw_matvec(tiles, T_W3T, lane, dz[2], da);
-    bw_gelu(A.z[1], da, dz[1]);
+    bw_act(A.z[1], da, dz[1]);
     bw_matvec(tiles, T_W2T, lane, dz[1], da);
-    bw_gelu(A.z[0], da, dz[0]);
+    bw_act(A.z[0], da, dz[0]);
 }
 
 // ---------------------------------------------------------------------

@@ -112,12 +112,15 @@ int launch_dm_fields(const Dims &d, const T *P, const T *smag, const T *S6, T *M
     return launched_sgs();
 }
 
-// (1,4,1)/6 in x and y (within a z plane), then in z
+// (1,4,1)/6 in x and y (within a z plane), then in z; blockIdx.y = field,
+// so one launch per direction filters every field
 template <typename T>
 __global__ __launch_bounds__(256) void k_filter_xy(const T *__restrict__ in, T *__restrict__ out, Grid3 g)
 {
     const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
     if (i >= g.np) return;
+    in += blockIdx.y * g.np;
+    out += blockIdx.y * g.np;
     const int x = (int)(i % g.nx), y = (int)((i / g.nx) % g.ny), z = (int)(i / ((long long)g.nx * g.ny));
     double acc = 0.0;
 #pragma unroll
@@ -135,20 +138,20 @@ __global__ __launch_bounds__(256) void k_filter_z(const T *__restrict__ in, T *_
 {
     const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
     if (i >= g.np) return;
+    in += blockIdx.y * g.np;
+    out += blockIdx.y * g.np;
     const int x = (int)(i % g.nx), y = (int)((i / g.nx) % g.ny), z = (int)(i / ((long long)g.nx * g.ny));
     const double r = (double)in[g.at(x, y, z - 1)] + 4.0 * (double)in[g.at(x, y, z)] + (double)in[g.at(x, y, z + 1)];
     out[i] = (T)(r * (1.0 / 6.0));
 }
 
-// filters nvar fields in -> out using tmp (one field) as scratch
+// filters nvar fields in -> out using tmp (nvar fields) as scratch
 template <typename T> int launch_filter(const Dims &d, const T *in, T *out, T *tmp, int nvar, hipStream_t st)
 {
     Grid3 g{d.nx, d.ny, d.nz, d.pfield};
     const unsigned nb = (unsigned)((g.np + 255) / 256);
-    for (int f = 0; f < nvar; f++) {
-        hipLaunchKernelGGL((k_filter_xy<T>), dim3(nb), dim3(256), 0, st, in + f * g.np, tmp, g);
-        hipLaunchKernelGGL((k_filter_z<T>), dim3(nb), dim3(256), 0, st, tmp, out + f * g.np, g);
-    }
+    hipLaunchKernelGGL((k_filter_xy<T>), dim3(nb, nvar), dim3(256), 0, st, in, tmp, g);
+    hipLaunchKernelGGL((k_filter_z<T>), dim3(nb, nvar), dim3(256), 0, st, tmp, out, g);
     return launched_sgs();
 }
 

@@ -1,7 +1,7 @@
 """Pin the learned closure's CPU oracle (oracle/mlp_oracle.py) -- CPU only.
 The closure has no reference counterpart; its oracle is pinned by finite
-differences of the unrounded network, the BF16 rounding rule and GELU's
-derivative."""
+differences of the unrounded network, the BF16 rounding rule and the
+activation's derivative."""
 import numpy as np
 
 from oracle import mlp_oracle as M
@@ -30,13 +30,14 @@ def test_bf16_rounding_rule():
     assert b[4] == 65536.0
 
 
-def test_gelu_derivative():
-    x = np.linspace(-4, 4, 101)
-    g, dg = M.gelu(x)
+def test_activation_and_derivative():
+    x = np.linspace(-4, 4, 100)          # no point at the kink
+    g, dg = M.activation(x)
     h = 1e-6
-    fd = (M.gelu(x + h)[0] - M.gelu(x - h)[0]) / (2 * h)
+    fd = (M.activation(x + h)[0] - M.activation(x - h)[0]) / (2 * h)
     assert np.abs(dg - fd).max() < 1e-8
-    assert abs(M.gelu(np.array([0.0]))[0][0]) == 0.0
+    assert np.array_equal(g, np.where(x > 0, x, 0.0))
+    assert M.activation(np.array([0.0]))[0][0] == 0.0 and M.activation(np.array([0.0]))[1][0] == 0.0
 
 
 def test_gradient_finite_differences():
