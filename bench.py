@@ -125,10 +125,12 @@ def les_comparison(S, ic, n_dns, n_les=128, les_steps=2000, refresh=20):
     filtered fields (SAM_LABELS_DYNAMIC), 1200 Adam steps of 2^20 points.
     LES (nu = 1e-5, the DNS IC truncated to 128^3): `les_steps` steps with
     the dynamic model, with the learned closure (C Delta^2 from the dynamic
-    procedure every `refresh` steps), and with the dynamic model from an IC
+    procedure every `refresh` steps), with the dynamic model from an IC
     perturbed by 1e-6 (the trajectories' chaotic spread, the floor of any
-    comparison).  Whole-run wall time per step; E(k) compared at the end and
-    time-averaged over the second half (every 50 steps)."""
+    comparison), and with the dynamic model whose C Delta^2 is itself
+    refreshed every `refresh` steps (the control for that amortisation).
+    Whole-run wall time per step; E(k) compared at the end and time-averaged
+    over the second half (every 50 steps)."""
     from sam_amd import les as L
     r = n_dns // n_les
     phys_dns = dict(i_strat=1, grav=64.0, to=1.0, lapse=1.0, vis=5e-4, pr=1.0, i_cfl=1, cfl0=0.5, dt0=0.01,
@@ -152,7 +154,8 @@ def les_comparison(S, ic, n_dns, n_les=128, les_steps=2000, refresh=20):
     rng = np.random.default_rng(5)
     res = {}
     runs = (("dynamic", S.SGS_DYN, u_les), ("learned", S.SGS_MLP, u_les),
-            ("dynamic_perturbed", S.SGS_DYN, u_les * (1 + 1e-6 * rng.standard_normal(u_les.shape))))
+            ("dynamic_perturbed", S.SGS_DYN, u_les * (1 + 1e-6 * rng.standard_normal(u_les.shape))),
+            ("dynamic_lagged", S.SGS_DYN, u_les))
     half = max(1, les_steps // 2)
     chunk = min(50, max(1, half // 2))    # steps between E(k) samples of the second half
     for name, model, u in runs:
@@ -160,6 +163,8 @@ def les_comparison(S, ic, n_dns, n_les=128, les_steps=2000, refresh=20):
         if model == S.SGS_MLP:
             S.attach_mlp(les, mlp)
             S.set_mlp_refresh(les, refresh)
+        if name == "dynamic_lagged":
+            S.set_dyn_refresh(les, refresh)
         les.set_state(u)
         les.sync()
         t0 = time.perf_counter()
@@ -183,7 +188,7 @@ def les_comparison(S, ic, n_dns, n_les=128, les_steps=2000, refresh=20):
         k1 = len(b) - 1
         return round(float(np.linalg.norm(a[lo:k1] - b[lo:k1]) / np.linalg.norm(b[lo:k1])), 4)
 
-    d, m, p = res["dynamic"], res["learned"], res["dynamic_perturbed"]
+    d, m, p, g = res["dynamic"], res["learned"], res["dynamic_perturbed"], res["dynamic_lagged"]
     return {
         "workload": f"stratified LES {n_les}^3, N=8, nu=1e-5, IC: the seeded k0=6 DNS IC truncated, "
                     f"FP32 transforms / FP64 state, {les_steps} steps",
@@ -198,6 +203,12 @@ def les_comparison(S, ic, n_dns, n_les=128, les_steps=2000, refresh=20):
             "time_avg_second_half": err(p["avg"], d["avg"]), "time_avg_second_half_k_ge_8": err(p["avg"], d["avg"], 8),
             "final": err(p["final"], d["final"]), "final_k_ge_8": err(p["final"], d["final"], 8)},
         "a_posteriori_check_at_end": m["check"],
+        # control: the dynamic model itself with C Delta^2 refreshed every
+        # `refresh` steps (the amortisation the learned closure uses)
+        "dynamic_lagged_ms_per_step": round(g["ms"], 3),
+        "ek_rel_err_dynamic_lagged_vs_dynamic": {
+            "time_avg_second_half": err(g["avg"], d["avg"]), "time_avg_second_half_k_ge_8": err(g["avg"], d["avg"], 8),
+            "final": err(g["final"], d["final"]), "final_k_ge_8": err(g["final"], d["final"], 8)},
         "training": {"samples": mlp.pool_size(), "labels": "dynamic model at unit coefficient",
                      "source": f"{n_dns}^3 DNS box-filtered by {r}, 4 snapshots 50 steps apart",
                      "dns_seconds": round(t_dns, 2), "adam_steps": 1200, "batch": 1 << 20,

@@ -238,6 +238,11 @@ int  sam_set_sgs(sam_handle *h, int sgs);
  * trained on SAM_LABELS_DYNAMIC labels (default 20; the first step after
  * sam_attach_mlp always updates) */
 int  sam_set_mlp_refresh(sam_handle *h, int nsteps);
+/* SAM_SGS_DYN with a lagged coefficient (no reference counterpart): the
+ * dynamic procedure runs at the first substep of every nsteps-th step and
+ * C Delta^2 is held in between; nsteps = 1 (default) runs it every substep
+ * as dm.f does.  The control for the learned closure's CDelta^2 refresh. */
+int  sam_set_dyn_refresh(sam_handle *h, int nsteps);
 /* a-posteriori check of the learned closure on the current state, with the
  * dynamic model as judge: the shell spectra of the SGS energy transfer
  * T_m(k) = tk_m(k) - tk_none(k) (spectra.f's tk of the RHS with model m

@@ -84,6 +84,8 @@ struct sam_handle {
     sam_mlp *mlp = nullptr;      // network used when p.sgs == SAM_SGS_MLP
     int mlp_refresh = 20;        // steps between dynamic-coefficient updates (unit-coefficient networks)
     int mlp_since = 1 << 30;     // steps since the last update (large: update at the next step)
+    int dyn_refresh = 1;         // SAM_SGS_DYN: 1 = dynamic procedure every substep (dm.f), N = every N steps
+    int dyn_since = 1 << 30;
     // HIP-event timing of the step's kernels (sam_prof_*)
     bool prof = false;
     std::vector<hipEvent_t> ev_free;
@@ -541,8 +543,21 @@ template <typename T> static int les_fluxes(sam_handle *h, int stats)
             rc = mlp_flux<T>(h, P, Q);
             if (rc) return rc;
         } else {
-            rc = sgs_coefficient<T>(h, P, b, h->diag, stats, h->p.sgs == SAM_SGS_DYN);
-            if (rc) return rc;
+            const bool dyn = h->p.sgs == SAM_SGS_DYN;
+            bool procedure = dyn;
+            if (dyn && h->dyn_refresh > 1) {
+                // lagged coefficient: the procedure at the first substep of every
+                // dyn_refresh-th step, C Delta^2 held in between
+                procedure = stats && ++h->dyn_since >= h->dyn_refresh;
+                if (procedure) h->dyn_since = 0;
+            }
+            if (dyn && !procedure) {
+                RCHK(launch_strain_smag<T>(d, P, (T *)b.smag, nullptr, d.nx / h->p.xl, d.ny / h->p.yl, d.nz / h->p.zl,
+                                           h->st));
+            } else {
+                rc = sgs_coefficient<T>(h, P, b, h->diag, stats, dyn);
+                if (rc) return rc;
+            }
             RCHK(launch_sgs_flux<T>(d, P, (const T *)b.smag, h->diag, Q, d.nx / h->p.xl, d.ny / h->p.yl,
                                     d.nz / h->p.zl, 1.0 / h->p.pr_t, h->st));
         }
@@ -1303,6 +1318,15 @@ extern "C" int sam_mlp_set_labels(sam_mlp *m, int mode, double pr_t)
 }
 
 static bool mlp_unit_coefficient(const sam_mlp *m) { return m->label_mode == SAM_LABELS_DYNAMIC; }
+
+extern "C" int sam_set_dyn_refresh(sam_handle *h, int nsteps)
+{
+    if (!h) return set_err(SAM_ERR_VALUE, "null handle");
+    if (nsteps < 1) return set_err(SAM_ERR_VALUE, "refresh interval must be >= 1 step");
+    h->dyn_refresh = nsteps;
+    h->dyn_since = 1 << 30;
+    return SAM_OK;
+}
 
 extern "C" int sam_set_mlp_refresh(sam_handle *h, int nsteps)
 {

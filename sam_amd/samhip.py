@@ -42,7 +42,7 @@ EXPORTS = (
     "sam_mlp_clear_pool", "sam_mlp_set_labels",
     "sam_mlp_fit_norms", "sam_mlp_train", "sam_mlp_grad", "sam_mlp_infer",
     "sam_attach_mlp", "sam_set_sgs", "sam_sgs_check", "sam_set_mlp_refresh",
-    "sam_set_forcing", "sam_set_cl", "sam_set_planes", "sam_planes",
+    "sam_set_forcing", "sam_set_cl", "sam_set_planes", "sam_planes", "sam_set_dyn_refresh",
 )
 
 MLP_NPARAM = 10448
@@ -152,6 +152,7 @@ def load():
     L.sam_attach_mlp.argtypes = [_hp, _hp]
     L.sam_set_sgs.argtypes = [_hp, ctypes.c_int]
     L.sam_set_mlp_refresh.argtypes = [_hp, ctypes.c_int]
+    L.sam_set_dyn_refresh.argtypes = [_hp, ctypes.c_int]
     L.sam_set_forcing.argtypes = [_hp, ctypes.c_int]
     L.sam_set_cl.argtypes = [_hp, ctypes.c_int]
     L.sam_set_planes.argtypes = [_hp, ctypes.c_int]
@@ -472,6 +473,12 @@ def planes(sam, k_xy=(), j_xz=(), i_yz=()):
                            mean.ctypes.data_as(_dp), xy.ctypes.data_as(_dp), xz.ctypes.data_as(_dp),
                            yz.ctypes.data_as(_dp)))
     return mean, xy[..., :len(k_xy)], xz[..., :len(j_xz)], yz[..., :len(i_yz)]
+
+
+def set_dyn_refresh(sam, nsteps):
+    """dynamic model with C Delta^2 recomputed every nsteps steps (1: every
+    substep, as dm.f)"""
+    _chk(load().sam_set_dyn_refresh(sam.h, int(nsteps)))
 
 
 def set_mlp_refresh(sam, nsteps):

@@ -138,3 +138,28 @@ def test_unit_coefficient_network_tracks_dynamic_model():
     err = shells_err(avg[S.SGS_MLP], avg[S.SGS_DYN], 1, k1)
     print("time-averaged E(k) rel err, unit-coefficient network vs dynamic: %.3f" % err)
     assert err < 0.10, err
+
+
+def test_lagged_dynamic_coefficient_holds_between_refreshes():
+    """sam_set_dyn_refresh(n): the dynamic procedure runs at the first substep
+    of every n-th step and C Delta^2 is held in between, so n steps equal the
+    Smagorinsky step with c_smag = C Delta^2 / Delta^2"""
+    n = 32
+    kw = dict(KW, precision=S.PREC_FP64)
+    u0 = ic.random_solenoidal(n, k0=6, seed=7, i_strat=1)
+    lag = S.Sam(S.make_params(n, sgs=S.SGS_DYN, **kw))
+    S.set_dyn_refresh(lag, 4)
+    lag.set_state(u0)
+    d = lag.step(4)
+    cds = d["c_delta_sq"]          # the volume-averaged coefficient may be negative (dm.f does not clip)
+    assert cds != 0
+    delta_sq = (2 * np.pi / n) ** 2
+    smag = S.Sam(S.make_params(n, sgs=S.SGS_SMAG, c_smag=cds / delta_sq, **kw))
+    smag.set_state(u0)
+    smag.step(4)
+    a, b = lag.get_state(), smag.get_state()
+    assert np.linalg.norm(a - b) <= 1e-12 * np.linalg.norm(b)
+    every = S.Sam(S.make_params(n, sgs=S.SGS_DYN, **kw))
+    every.set_state(u0)
+    every.step(4)
+    assert np.linalg.norm(every.get_state() - b) > 1e-10 * np.linalg.norm(b)
