@@ -89,7 +89,7 @@ __global__ __launch_bounds__(C * N / 8) void k_zb(const double *__restrict__ S, 
         const int src = (p <= kzm) ? p : ((p >= N - kzm) ? p - N + nzm : -1);
         cpx<T> x = {(T)0, (T)0};
         if (ok && src >= 0) {
-            const cpx<double> dd = Sc[col * ldz + src];
+            const cpx<double> dd = ld_nt(Sc + col * ldz + src);
             x = {(T)dd.x, (T)dd.y};
         }
         v[s] = x;
@@ -97,7 +97,7 @@ __global__ __launch_bounds__(C * N / 8) void k_zb(const double *__restrict__ S, 
     TL::template run_regs<+1>(lds, tw, c, j0, v);
     if (ok) {
 #pragma unroll
-        for (int s = 0; s < 8; s++) Z[col * N + j0 + TL::kout(s) * TL::TPC] = v[s];
+        for (int s = 0; s < 8; s++) st_nt(Z + col * N + j0 + TL::kout(s) * TL::TPC, v[s]);
     }
 }
 
@@ -139,7 +139,7 @@ __global__ __launch_bounds__(C * N / EPT) void k_yb(const T *__restrict__ Zr, T 
         const int e = threadIdx.x + s * NT, p = e / C, c = e % C;
         const int iy = (p <= d.kym) ? p : ((p >= N - d.kym) ? p - N + d.nym : -1);
         x[s] = {(T)0, (T)0};
-        if (iy >= 0) x[s] = Z[(long long)iy * d.nz + c];
+        if (iy >= 0) x[s] = ld_nt(Z + (long long)iy * d.nz + c);
     }
 #pragma unroll
     for (int s = 0; s < EPT; s++) {
@@ -152,7 +152,7 @@ __global__ __launch_bounds__(C * N / EPT) void k_yb(const T *__restrict__ Zr, T 
     TL::template run_from_lds<+1>(lds, tw, c, j0, v);
     cpx<T> *Y = reinterpret_cast<cpx<T> *>(Yr) + (((long long)f * d.nz + z0 + c) * d.nxm + ix) * N;
 #pragma unroll
-    for (int s = 0; s < EPT; s++) Y[j0 + TL::kout(s) * TL::TPC] = v[s];
+    for (int s = 0; s < EPT; s++) st_nt(Y + j0 + TL::kout(s) * TL::TPC, v[s]);
 }
 
 template <typename T>
@@ -189,7 +189,7 @@ __global__ __launch_bounds__(C * N / EPT) void k_yf(const T *__restrict__ Yr, T 
     const cpx<T> *Y = reinterpret_cast<const cpx<T> *>(Yr) + (((long long)q * d.nz + z0 + c) * d.nxm + ix) * N;
     cpx<T> v[EPT];
 #pragma unroll
-    for (int s = 0; s < EPT; s++) v[s] = Y[j0 + TL::kin(0, s) * TL::TPC];
+    for (int s = 0; s < EPT; s++) v[s] = ld_nt(Y + j0 + TL::kin(0, s) * TL::TPC);
     TL::template run_regs<-1>(lds, tw, c, j0, v);
     TL::store_natural(lds, c, j0, v);
     __syncthreads();
@@ -206,7 +206,7 @@ __global__ __launch_bounds__(C * N / EPT) void k_yf(const T *__restrict__ Yr, T 
 #pragma unroll
     for (int s = 0; s < EPT; s++) {
         const int e = threadIdx.x + s * NT;
-        if (e < C * d.nym) Z[(long long)(e / C) * d.nz + e % C] = o[s];
+        if (e < C * d.nym) st_nt(Z + (long long)(e / C) * d.nz + e % C, o[s]);
     }
 }
 
@@ -262,7 +262,9 @@ __device__ __forceinline__ void xrow_store_pairs(const cpx<T> *lds, cpx<T> *__re
         Pair<T> pr;
         pr.a = {(zk.x + zm.x) * h, (zk.y - zm.y) * h};
         pr.b = {(zk.y + zm.y) * h, (zm.x - zk.x) * h};
-        *reinterpret_cast<Pair<T> *>(Yq + (long long)(e / CP) * ny + y0 + 2 * (e % CP)) = pr;
+        typedef T v4 __attribute__((ext_vector_type(4)));
+        __builtin_nontemporal_store(v4{pr.a.x, pr.a.y, pr.b.x, pr.b.y},
+                                    reinterpret_cast<v4 *>(Yq + (long long)(e / CP) * ny + y0 + 2 * (e % CP)));
     };
     cpx<T> zk[MI], zm[MI];
 #pragma unroll
@@ -314,7 +316,7 @@ using lds_void_t = __attribute__((address_space(3))) void;
 __device__ __forceinline__ void lds_dma16(const void *gsrc, const void *lds_dst)
 {
     const unsigned m0 = __builtin_amdgcn_readfirstlane((unsigned)(uintptr_t)(const lds_void_t *)lds_dst);
-    asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(gsrc), "s"(m0)
+    asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off nt" ::"v"(gsrc), "s"(m0)
                  : "memory", "m0");
 }
 
@@ -980,7 +982,7 @@ __global__ __launch_bounds__(LR * N / EPT) void k_zrhs(const T *__restrict__ Zr,
     const cpx<T> *Zc = reinterpret_cast<const cpx<T> *>(Zr) + (q * d.zfield + col * N);
     cpx<T> v[EPT];
 #pragma unroll
-    for (int s = 0; s < EPT; s++) v[s] = Zc[j0 + TL::kin(0, s) * TL::TPC];
+    for (int s = 0; s < EPT; s++) v[s] = ld_nt(Zc + j0 + TL::kin(0, s) * TL::TPC);
     TL::template run_regs<-1>(lds, tw, q, j0, v);
     TL::store_natural(lds, q, j0, v);
     __syncthreads();
@@ -1007,8 +1009,8 @@ __global__ __launch_bounds__(LR * N / EPT) void k_zrhs(const T *__restrict__ Zr,
         cpx<double> us[LU], rp[LU];
 #pragma unroll
         for (int n = 0; n < LU; n++) {
-            us[n] = (mode != 1 || n >= 2) ? U[n * ss + si] : cpx<double>{0, 0};
-            rp[n] = (mode != 1 && m > 0) ? R[n * ss + si] : cpx<double>{0, 0};
+            us[n] = (mode != 1 || n >= 2) ? ld_nt(U + n * ss + si) : cpx<double>{0, 0};
+            rp[n] = (mode != 1 && m > 0) ? ld_nt(R + n * ss + si) : cpx<double>{0, 0};
         }
         cpx<double> F[LR];
 #pragma unroll
@@ -1040,10 +1042,10 @@ __global__ __launch_bounds__(LR * N / EPT) void k_zrhs(const T *__restrict__ Zr,
                 cpx<double> u = us[n];
                 if (m > 0) u = {u.x + dtf2 * rp[n].x, u.y + dtf2 * rp[n].y};
                 u = {u.x + dtf1 * r[n].x, u.y + dtf1 * r[n].y};
-                Un[n * ss + si] = {u.x * vis_fac, u.y * vis_fac};
-                R[n * ss + si] = r[n];
+                st_nt(Un + n * ss + si, cpx<double>{u.x * vis_fac, u.y * vis_fac});
+                st_nt(R + n * ss + si, r[n]);
             }
-            if (mode == 2) R[LU * ss + si] = pr;   // kept for the planes output
+            if (mode == 2) st_nt(R + LU * ss + si, pr);   // kept for the planes output
         } else {
 #pragma unroll
             for (int n = 0; n < LU; n++) R[n * ss + si] = r[n];
