@@ -1724,13 +1724,29 @@ template <typename T> int launch_convert_to(const double *src, T *dst, long long
 
 __global__ __launch_bounds__(256) void k_copy16(const uint4 *__restrict__ s, uint4 *__restrict__ d, long long n)
 {
-    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) d[i] = s[i];
+    // 4 independent 16-byte elements per thread and iteration, nt policy on
+    // both sides (as in the step kernels): the practical HBM ceiling
+    typedef unsigned v4u __attribute__((ext_vector_type(4)));
+    const v4u *a = reinterpret_cast<const v4u *>(s);
+    v4u *b = reinterpret_cast<v4u *>(d);
+    const long long stride = (long long)gridDim.x * 1024;
+    long long i = (long long)blockIdx.x * 1024 + threadIdx.x;
+    for (; i + 768 < n; i += stride) {
+        v4u v[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) v[j] = __builtin_nontemporal_load(a + i + 256 * j);
+#pragma unroll
+        for (int j = 0; j < 4; j++) __builtin_nontemporal_store(v[j], b + i + 256 * j);
+    }
+    // the one partial chunk (every later chunk starts past n)
+    for (int j = 0; j < 4; j++)
+        if (i + 256 * j < n) __builtin_nontemporal_store(__builtin_nontemporal_load(a + i + 256 * j), b + i + 256 * j);
 }
 
 int launch_copy16(const void *src, void *dst, long long n16, hipStream_t st)
 {
-    long long nb = (n16 + 255) / 256;
-    if (nb > 65536) nb = 65536;
+    long long nb = (n16 + 1023) / 1024;
+    if (nb > 32768) nb = 32768;
     hipLaunchKernelGGL(k_copy16, dim3((unsigned)nb), dim3(256), 0, st, (const uint4 *)src, (uint4 *)dst, n16);
     return launched();
 }
