@@ -129,6 +129,8 @@ def les_comparison(S, ic, n_dns, n_les=128, les_steps=2000, refresh=20):
     perturbed by 1e-6 (the trajectories' chaotic spread, the floor of any
     comparison), and with the dynamic model whose C Delta^2 is itself
     refreshed every `refresh` steps (the control for that amortisation).
+    The learned closure runs twice: with ReLU hidden units (the default) and
+    with GELU (tanh form), each network trained on the same samples.
     Whole-run wall time per step; E(k) compared at the end and time-averaged
     over the second half (every 50 steps)."""
     from sam_amd import les as L
@@ -141,27 +143,34 @@ def les_comparison(S, ic, n_dns, n_les=128, les_steps=2000, refresh=20):
     dns.set_state(u0)
     mlp = S.Mlp(seed=1, pool_capacity=4 * n_les ** 3)
     mlp.set_labels(S.LABELS_DYNAMIC, 1.0)
+    mlp_gelu = S.Mlp(seed=1, pool_capacity=4 * n_les ** 3)
+    mlp_gelu.set_activation(S.ACT_GELU)
+    mlp_gelu.set_labels(S.LABELS_DYNAMIC, 1.0)
     t0 = time.perf_counter()
     for _ in range(4):
         dns.step(50)
         mlp.add_samples(dns, r)
+        mlp_gelu.add_samples(dns, r)
     dns.close()
     t_dns = time.perf_counter() - t0
     t0 = time.perf_counter()
     losses = L.train(mlp, 1200, batch=1 << 20, lr=1e-3, seed=7)
     t_train = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    losses_gelu = L.train(mlp_gelu, 1200, batch=1 << 20, lr=1e-3, seed=7)
+    t_train_gelu = time.perf_counter() - t0
     u_les = L.spectral_truncate(u0, n_les)
     rng = np.random.default_rng(5)
     res = {}
     runs = (("dynamic", S.SGS_DYN, u_les), ("learned", S.SGS_MLP, u_les),
             ("dynamic_perturbed", S.SGS_DYN, u_les * (1 + 1e-6 * rng.standard_normal(u_les.shape))),
-            ("dynamic_lagged", S.SGS_DYN, u_les))
+            ("dynamic_lagged", S.SGS_DYN, u_les), ("learned_gelu", S.SGS_MLP, u_les))
     half = max(1, les_steps // 2)
     chunk = min(50, max(1, half // 2))    # steps between E(k) samples of the second half
     for name, model, u in runs:
         les = S.Sam(S.make_params(n_les, sgs=model, **phys))
         if model == S.SGS_MLP:
-            S.attach_mlp(les, mlp)
+            S.attach_mlp(les, mlp_gelu if name == "learned_gelu" else mlp)
             S.set_mlp_refresh(les, refresh)
         if name == "dynamic_lagged":
             S.set_dyn_refresh(les, refresh)
@@ -189,6 +198,7 @@ def les_comparison(S, ic, n_dns, n_les=128, les_steps=2000, refresh=20):
         return round(float(np.linalg.norm(a[lo:k1] - b[lo:k1]) / np.linalg.norm(b[lo:k1])), 4)
 
     d, m, p, g = res["dynamic"], res["learned"], res["dynamic_perturbed"], res["dynamic_lagged"]
+    mg = res["learned_gelu"]
     return {
         "workload": f"stratified LES {n_les}^3, N=8, nu=1e-5, IC: the seeded k0=6 DNS IC truncated, "
                     f"FP32 transforms / FP64 state, {les_steps} steps",
@@ -209,10 +219,18 @@ def les_comparison(S, ic, n_dns, n_les=128, les_steps=2000, refresh=20):
         "ek_rel_err_dynamic_lagged_vs_dynamic": {
             "time_avg_second_half": err(g["avg"], d["avg"]), "time_avg_second_half_k_ge_8": err(g["avg"], d["avg"], 8),
             "final": err(g["final"], d["final"]), "final_k_ge_8": err(g["final"], d["final"], 8)},
+        # the same closure with GELU hidden units (tanh form)
+        "learned_gelu_ms_per_step": round(mg["ms"], 3),
+        "speedup_learned_gelu_vs_dynamic": round(d["ms"] / mg["ms"], 3),
+        "ek_rel_err_learned_gelu_vs_dynamic": {
+            "time_avg_second_half": err(mg["avg"], d["avg"]), "time_avg_second_half_k_ge_8": err(mg["avg"], d["avg"], 8),
+            "final": err(mg["final"], d["final"]), "final_k_ge_8": err(mg["final"], d["final"], 8)},
+        "a_posteriori_check_at_end_gelu": mg["check"],
         "training": {"samples": mlp.pool_size(), "labels": "dynamic model at unit coefficient",
                      "source": f"{n_dns}^3 DNS box-filtered by {r}, 4 snapshots 50 steps apart",
                      "dns_seconds": round(t_dns, 2), "adam_steps": 1200, "batch": 1 << 20,
-                     "seconds": round(t_train, 2), "loss_first": float(losses[0]), "loss_last": float(losses[-1])},
+                     "seconds": round(t_train, 2), "loss_first": float(losses[0]), "loss_last": float(losses[-1]),
+                     "gelu_seconds": round(t_train_gelu, 2), "gelu_loss_last": float(losses_gelu[-1])},
     }
 
 

@@ -185,7 +185,8 @@ int  sam_eddy_visc(sam_handle *h, const double *phys, double *vt, double *c_delt
 int  sam_test_filter(sam_handle *h, const double *in, double *out, int nvar);
 
 /* ---- learned SGS closure (no counterpart in the reference) ------------ */
-/* Per-point MLP 12 -> 64 -> 64 -> 64 -> 9 (ReLU), BF16 MFMA / FP32 accumulate.
+/* Per-point MLP 12 -> 64 -> 64 -> 64 -> 9 (ReLU or GELU hidden activation),
+ * BF16 MFMA / FP32 accumulate.
  * Inputs: du_i/dx_j (9, row-major) and dT'/dx_j (3) by centred differences;
  * outputs: deviatoric tau_11,22,33,12,23,13 and q_1..3; the step then uses
  * F_ij = u_i u_j + tau_ij and F_Tj = u_j T' + q_j. */
@@ -193,6 +194,14 @@ typedef struct sam_mlp sam_mlp;
 #define SAM_MLP_NPARAM 10448     /* FP32 parameters; + 32 normalisation scales */
 int  sam_mlp_create(sam_mlp **m, unsigned seed, long long pool_capacity);
 int  sam_mlp_destroy(sam_mlp *m);
+/* hidden activation: part of the network.  Set it before training, and
+ * before sam_mlp_set_params with parameters trained under it.
+ *   SAM_MLP_ACT_RELU  max(x, 0) (default);
+ *   SAM_MLP_ACT_GELU  tanh form 0.5 x (1 + tanh(sqrt(2/pi) (x + 0.044715 x^3))). */
+#define SAM_MLP_ACT_RELU 0
+#define SAM_MLP_ACT_GELU 1
+int  sam_mlp_set_activation(sam_mlp *m, int act);
+int  sam_mlp_get_activation(const sam_mlp *m, int *act);
 /* params: SAM_MLP_NPARAM weights/biases followed by 32 scales */
 int  sam_mlp_get_params(sam_mlp *m, float *params);
 int  sam_mlp_set_params(sam_mlp *m, const float *params);

@@ -3,7 +3,7 @@
 TEST INFRASTRUCTURE ONLY (tests/, never the product path).  The closure has
 no reference counterpart; this restates the GPU network exactly:
 
-  12 -> 64 -> 64 -> 64 -> 9, ReLU, inputs padded to 16 and
+  12 -> 64 -> 64 -> 64 -> 9, ReLU or GELU (tanh form), inputs padded to 16 and
   outputs to 16; every GEMM operand (weights, activations, output and hidden
   gradients) rounded to BF16 (round to nearest even), FP32 accumulation, FP32
   bias, activation and Adam.
@@ -50,15 +50,26 @@ def pack_grads(g):
     return out
 
 
-def activation(x):
-    """hidden activation (ReLU) and its derivative (1 for x > 0, else 0)"""
+RELU, GELU = 0, 1
+
+
+def activation(x, kind=RELU):
+    """hidden activation and its derivative.
+    RELU: max(x, 0), derivative 1 for x > 0, else 0.
+    GELU, tanh form: 0.5 x (1 + tanh(c (x + a x^3))), c = sqrt(2/pi), a = 0.044715."""
+    if kind == GELU:
+        c, a = 0.7978845608028654, 0.044715
+        t = np.tanh(c * (x + a * x ** 3))
+        dg = 0.5 * (1 + t) + 0.5 * x * (1 - t * t) * c * (1 + 3 * a * x * x)
+        return 0.5 * x * (1 + t), dg
     return np.maximum(x, 0.0), (x > 0).astype(np.float64)
 
 
 class Net:
-    def __init__(self, params, bf=True):
+    def __init__(self, params, bf=True, act=RELU):
         self.w, self.fs, self.os = unpack(params)
         self.bf = bf
+        self.act = act
         self.dt = np.float32 if bf else np.float64
 
     def r(self, x):
@@ -83,7 +94,7 @@ class Net:
         a = act["a0"]
         for L, (W, b) in enumerate((("W1", "b1"), ("W2", "b2"), ("W3", "b3")), start=1):
             z = (a.astype(dt) @ self.r(self.w[W].astype(dt)).T + self.w[b].astype(dt)).astype(dt)
-            g, dg = activation(z.astype(np.float64))
+            g, dg = activation(z.astype(np.float64), self.act)
             act[f"z{L}"] = z
             act[f"dg{L}"] = dg.astype(dt)
             a = self.r(g.astype(dt))

@@ -1119,6 +1119,7 @@ struct sam_mlp {
     float *feats = nullptr, *labels = nullptr;
     long long cap = 0, n = 0;
     int label_mode = SAM_LABELS_FILTERED;   // what the pool's labels are (see sam_mlp_set_labels)
+    int activation = SAM_MLP_ACT_RELU;      // hidden activation (sam_mlp_set_activation)
     float pr_t = 1.0f;
     DevBuf act, gpart, lpart, npart, gout, tmp, lossv;
 };
@@ -1306,6 +1307,21 @@ extern "C" int sam_mlp_add_samples(sam_mlp *m, sam_handle *dns, int r)
     return SAM_OK;
 }
 
+extern "C" int sam_mlp_set_activation(sam_mlp *m, int act)
+{
+    if (!m) return set_err(SAM_ERR_VALUE, "null argument");
+    if (act != SAM_MLP_ACT_RELU && act != SAM_MLP_ACT_GELU) return set_err(SAM_ERR_VALUE, "unknown activation %d", act);
+    m->activation = act;
+    return SAM_OK;
+}
+
+extern "C" int sam_mlp_get_activation(const sam_mlp *m, int *act)
+{
+    if (!m || !act) return set_err(SAM_ERR_VALUE, "null argument");
+    *act = m->activation;
+    return SAM_OK;
+}
+
 extern "C" int sam_mlp_set_labels(sam_mlp *m, int mode, double pr_t)
 {
     if (!m) return set_err(SAM_ERR_VALUE, "null argument");
@@ -1383,7 +1399,7 @@ static int mlp_pass(sam_mlp *m, const float *feats, const float *labels, long lo
     if (nblk > 1024) nblk = 1024;
     if (nblk < 1) nblk = 1;
     RCHK(launch_mlp_fb(m->P, m->img, m->norms, feats, labels, npool, base, pa, pb, batch, tb, (double *)m->lpart.p,
-                       nblk, m->st));
+                       nblk, m->activation, m->st));
     RCHK(launch_sum_loss((const double *)m->lpart.p, nblk, loss_dst, m->st));
     RCHK(launch_mlp_wgrad(tb, batch, (float *)m->gpart.p, nch, m->st));
     if (apply) m->step++;
@@ -1449,7 +1465,7 @@ extern "C" int sam_mlp_infer(sam_mlp *m, const float *feats, float *out, long lo
     if (rc) return rc;
     float *f = (float *)m->tmp.p, *o = f + 16 * (size_t)(n + 16);
     if ((rc = upload_padded(m, f, feats, n, MLP_IN))) return rc;
-    RCHK(launch_mlp_infer(m->P, m->img, m->norms, f, o, n, m->st));
+    RCHK(launch_mlp_infer(m->P, m->img, m->norms, f, o, n, m->activation, m->st));
     std::vector<float> tmp((size_t)n * 16);
     HIPCHK(hipMemcpyAsync(tmp.data(), o, sizeof(float) * 16 * n, hipMemcpyDeviceToHost, m->st));
     HIPCHK(hipStreamSynchronize(m->st));
@@ -1483,7 +1499,7 @@ template <typename T> static int mlp_flux(sam_handle *h, const T *P, T *Q)
     HIPCHK(hipStreamSynchronize(h->mlp->st));   // parameters written on the network's stream
     const double *cs = mlp_unit_coefficient(h->mlp) ? &h->diag->c_delta_sq : nullptr;
     RCHK(launch_les_mlp_flux<T>(d, h->mlp->img, h->mlp->norms, P, Q, (float)(d.nx / h->p.xl), (float)(d.ny / h->p.yl),
-                                (float)(d.nz / h->p.zl), cs, h->st));
+                                (float)(d.nz / h->p.zl), cs, h->mlp->activation, h->st));
     return SAM_OK;
 }
 

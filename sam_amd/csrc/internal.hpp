@@ -124,17 +124,17 @@ constexpr int MLP_TILE_BYTES = 76 * 64 * 8 + 208 * 4;   // packed BF16 weight ti
 int launch_mlp_pack(const float *P, void *img, hipStream_t st);
 int launch_mlp_fb(const float *P, const void *img, const float *norms, const float *feats, const float *labels,
                   long long npool, long long base, long long pa, long long pb, int batch, const MlpTrainBufs &b,
-                  double *loss_part, int nblk, hipStream_t st);
+                  double *loss_part, int nblk, int act, hipStream_t st);
 int launch_mlp_wgrad(const MlpTrainBufs &b, int batch, float *part, int nchunks, hipStream_t st);
 int launch_mlp_adam(const float *part, int nch, float *P, float *m1, float *m2, float *gout, float lr, int step,
                     int apply, hipStream_t st);
 int launch_sum_loss(const double *part, int n, double *out, hipStream_t st);
 int launch_mlp_infer(const float *P, const void *img, const float *norms, const float *feats, float *out, long long n,
-                     hipStream_t st);
+                     int act, hipStream_t st);
 // cscale: device pointer to C Delta^2 for unit-coefficient networks, else null
 template <typename T>
 int launch_les_mlp_flux(const Dims &d, const void *img, const float *norms, const T *U, T *Q, float dxi, float dyi,
-                        float dzi, const double *cscale, hipStream_t st);
+                        float dzi, const double *cscale, int act, hipStream_t st);
 int launch_dyn_labels(const float *feats, long long n, float pr_t_inv, float *labels, hipStream_t st);
 template <typename T> int launch_box_filter(const Dims &d, const T *U, int r, float *coarse, float *labels, hipStream_t st);
 int launch_coarse_features(const float *C, int nx, int ny, int nz, int lu, float dxi, float dyi, float dzi,

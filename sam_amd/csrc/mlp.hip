@@ -1,4 +1,4 @@
-// mlp.hip -- learned per-point SGS closure: 12 -> 64 -> 64 -> 64 -> 9, ReLU.
+// mlp.hip -- learned per-point SGS closure: 12 -> 64 -> 64 -> 64 -> 9, ReLU or GELU.
 //
 // Inputs (12): resolved velocity gradient du_i/dx_j (9, row-major i,j) and
 // dT'/dx_j (3), centred differences on the collocated grid (the same
@@ -67,14 +67,39 @@ __device__ __forceinline__ float point_r2(v4f x)
 }
 __device__ __forceinline__ float inv_radius(float r2) { return r2 > 0.f ? 1.0f / sqrtf(r2) : 0.f; }
 
-// Hidden activation: ReLU, one instruction per value.  (A GELU needs an exp and a
-// reciprocal, quarter-rate instructions: with 48 activations per lane per
-// 16-point tile it took over half of the inference kernel's cycles.)
-// (as a signed max on the bits: a negative float, sign bit set, is a
-// negative integer; one v_max_i32 where fmaxf also quiets NaNs first)
-__device__ __forceinline__ float act(float x) { return __int_as_float(max(__float_as_int(x), 0)); }
-// derivative: 1 for x > 0, else 0
-__device__ __forceinline__ float act_deriv(float x) { return x > 0.0f ? 1.0f : 0.0f; }
+// Hidden activation, ACT = SAM_MLP_ACT_RELU or SAM_MLP_ACT_GELU.
+//
+// ReLU: one instruction per value, a signed max on the bits (a negative
+// float, sign bit set, is a negative integer; fmaxf would also quiet NaNs
+// first).  Derivative 1 for x > 0, else 0.
+//
+// GELU, tanh form: 0.5 x (1 + tanh u) with u = c (x + a x^3), evaluated as
+// x * sigmoid(2u) = x / (1 + 2^(-2u log2 e)): one exp2 and one reciprocal,
+// quarter-rate instructions.  With 48 activations per lane per 16-point tile
+// they take about 40% of the inference kernel's time.  e -> inf for large
+// negative x gives exactly -0; e -> 0 gives x.
+constexpr float GELU_C = 0.7978845608028654f, GELU_A = 0.044715f;
+constexpr float GELU_K1 = -2.0f * GELU_C * 1.4426950408889634f, GELU_K2 = GELU_K1 * GELU_A;
+__device__ __forceinline__ float gelu_sig(float x)
+{
+    const float e = __builtin_amdgcn_exp2f(x * __builtin_fmaf(GELU_K2, x * x, GELU_K1));
+    return __builtin_amdgcn_rcpf(1.0f + e);
+}
+template <int ACT> __device__ __forceinline__ float act(float x)
+{
+    if constexpr (ACT == SAM_MLP_ACT_GELU) return x * gelu_sig(x);
+    else return __int_as_float(max(__float_as_int(x), 0));
+}
+// GELU derivative: s + g (1 - s) 2c (1 + 3 a x^2), s = sigmoid(2u), g = x s
+template <int ACT> __device__ __forceinline__ float act_deriv(float x)
+{
+    if constexpr (ACT == SAM_MLP_ACT_GELU) {
+        const float sg = gelu_sig(x), g = x * sg;
+        return __builtin_fmaf(g * (1.0f - sg), __builtin_fmaf(6.0f * GELU_C * GELU_A, x * x, 2.0f * GELU_C), sg);
+    } else {
+        return x > 0.0f ? 1.0f : 0.0f;
+    }
+}
 
 // ---- weight tiles in LDS ---------------------------------------------
 // tile t (64 lanes x 4 bf16): forward tiles  0..3  W1 (mb),  4..19 W2 (mb*4+kb),
@@ -140,13 +165,14 @@ struct Acts {
     v4f out;         // outputs 4g..4g+3
 };
 
-__device__ __forceinline__ v4s act_bf4(v4f z)
+template <int ACT> __device__ __forceinline__ v4s act_bf4(v4f z)
 {
-    return bf4(v4f{act(z[0]), act(z[1]), act(z[2]), act(z[3])});
+    return bf4(v4f{act<ACT>(z[0]), act<ACT>(z[1]), act<ACT>(z[2]), act<ACT>(z[3])});
 }
 
 // one hidden layer 64 -> 64: z = W a + b, tiles tb..tb+15 (mb*4+kb); the
 // four output blocks' MFMA chains are interleaved (kb outer)
+template <int ACT>
 __device__ __forceinline__ void fw_hidden(const float *bias, const v4s *tiles, int tb, int boff, int lane,
                                           const v4s *ain, v4f *z, v4s *aout)
 {
@@ -161,12 +187,13 @@ __device__ __forceinline__ void fw_hidden(const float *bias, const v4s *tiles, i
 #pragma unroll
     for (int mb = 0; mb < 4; mb++) {
         z[mb] = acc[mb];
-        aout[mb] = act_bf4(acc[mb]);
+        aout[mb] = act_bf4<ACT>(acc[mb]);
     }
 }
 
 // forward pass of one 16-point tile; x: this lane's 4 normalised inputs;
 // bias and tiles in LDS
+template <int ACT>
 __device__ __forceinline__ void mlp_forward(const float *bias, const v4s *tiles, int lane, v4f x, Acts &A)
 {
     const int g = lane >> 4;
@@ -177,26 +204,26 @@ __device__ __forceinline__ void mlp_forward(const float *bias, const v4s *tiles,
 #pragma unroll
     for (int mb = 0; mb < 4; mb++) {
         A.z[0][mb] = z[mb];
-        A.a[0][mb] = act_bf4(z[mb]);
+        A.a[0][mb] = act_bf4<ACT>(z[mb]);
     }
-    fw_hidden(bias, tiles, T_W2, BO_2, lane, A.a[0], A.z[1], A.a[1]);
-    fw_hidden(bias, tiles, T_W3, BO_3, lane, A.a[1], A.z[2], A.a[2]);
+    fw_hidden<ACT>(bias, tiles, T_W2, BO_2, lane, A.a[0], A.z[1], A.a[1]);
+    fw_hidden<ACT>(bias, tiles, T_W3, BO_3, lane, A.a[1], A.z[2], A.a[2]);
     v4f o = bias4(bias, BO_4, 4 * g);
 #pragma unroll
     for (int kb = 0; kb < 4; kb++) o = mfma(tiles[(T_W4 + kb) * 64 + lane], A.a[2][kb], o);
     A.out = o;
 }
 
-__device__ __forceinline__ v4f act_grad4(v4f z)
+template <int ACT> __device__ __forceinline__ v4f act_grad4(v4f z)
 {
-    return v4f{act_deriv(z[0]), act_deriv(z[1]), act_deriv(z[2]), act_deriv(z[3])};
+    return v4f{act_deriv<ACT>(z[0]), act_deriv<ACT>(z[1]), act_deriv<ACT>(z[2]), act_deriv<ACT>(z[3])};
 }
 
-__device__ __forceinline__ void bw_act(const v4f *z, const v4f *da, v4s *dz)
+template <int ACT> __device__ __forceinline__ void bw_act(const v4f *z, const v4f *da, v4s *dz)
 {
 #pragma unroll
     for (int mb = 0; mb < 4; mb++) {
-        const v4f gd = act_grad4(z[mb]);
+        const v4f gd = act_grad4<ACT>(z[mb]);
         dz[mb] = bf4(v4f{da[mb][0] * gd[0], da[mb][1] * gd[1], da[mb][2] * gd[2], da[mb][3] * gd[3]});
     }
 }
@@ -213,6 +240,7 @@ __device__ __forceinline__ void bw_matvec(const v4s *tiles, int tb, int lane, co
 }
 
 // backward of one tile from dL/d(out); returns dz of each hidden layer (bf16)
+template <int ACT>
 __device__ __forceinline__ void mlp_backward(const v4s *tiles, int lane, const Acts &A, v4f dout, v4s dz[3][4],
                                              v4s &dout_b)
 {
@@ -220,11 +248,11 @@ __device__ __forceinline__ void mlp_backward(const v4s *tiles, int lane, const A
     v4f da[4];
 #pragma unroll
     for (int mb = 0; mb < 4; mb++) da[mb] = mfma(tiles[(T_W4T + mb) * 64 + lane], dout_b, v4f{0, 0, 0, 0});
-    bw_act(A.z[2], da, dz[2]);
+    bw_act<ACT>(A.z[2], da, dz[2]);
     bw_matvec(tiles, T_W3T, lane, dz[2], da);
-    bw_act(A.z[1], da, dz[1]);
+    bw_act<ACT>(A.z[1], da, dz[1]);
     bw_matvec(tiles, T_W2T, lane, dz[1], da);
-    bw_act(A.z[0], da, dz[0]);
+    bw_act<ACT>(A.z[0], da, dz[0]);
 }
 
 // ---------------------------------------------------------------------
@@ -241,6 +269,7 @@ __device__ __forceinline__ void store4(short *base, int ld, int p, int u0, v4s v
     *reinterpret_cast<v4s *>(base + (long long)p * ld + u0) = v;
 }
 
+template <int ACT>
 __global__ __launch_bounds__(256) void k_mlp_fb(const float *__restrict__ P, const v4s *__restrict__ img,
                                                  const float *__restrict__ norms,
                                                  const float *__restrict__ feats, const float *__restrict__ labels,
@@ -269,7 +298,7 @@ __global__ __launch_bounds__(256) void k_mlp_fb(const float *__restrict__ P, con
         const int zo = opaque_zero();
         const v4s *tl = tiles + zo;
         Acts A;
-        mlp_forward(reinterpret_cast<const float *>(biasv) + zo, tl, lane, x, A);
+        mlp_forward<ACT>(reinterpret_cast<const float *>(biasv) + zo, tl, lane, x, A);
         // MSE over the 9 normalised outputs: L = mean over batch*9 of (out - y/os)^2
         v4f dout;
 #pragma unroll
@@ -287,7 +316,7 @@ __global__ __launch_bounds__(256) void k_mlp_fb(const float *__restrict__ P, con
             dout[r] = 2.0f * d / (float)(batch * MLP_OUT);
         }
         v4s dz[3][4], dob;
-        mlp_backward(tl, lane, A, dout, dz, dob);
+        mlp_backward<ACT>(tl, lane, A, dout, dz, dob);
         store4(tb.a0, 16, p, 4 * g, A.a0);
         store4(tb.dout, 16, p, 4 * g, dob);
 #pragma unroll
@@ -410,6 +439,7 @@ __device__ __forceinline__ void deviatoric(int g, v4f &o)
     }
 }
 
+template <int ACT>
 __global__ __launch_bounds__(256) void k_mlp_infer(const float *__restrict__ P, const v4s *__restrict__ img,
                                                     const float *__restrict__ norms, const float *__restrict__ feats,
                                                     float *__restrict__ out, long long n)
@@ -429,7 +459,7 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const float *__restrict__ P, 
         x *= inv_radius(r2);
         const int zo = opaque_zero();
         Acts A;
-        mlp_forward(reinterpret_cast<const float *>(biasv) + zo, tiles + zo, lane, x, A);
+        mlp_forward<ACT>(reinterpret_cast<const float *>(biasv) + zo, tiles + zo, lane, x, A);
         v4f o = A.out;
         for (int r = 0; r < 4; r++) o[r] *= (4 * g + r < MLP_OUT) ? norms[16 + 4 * g + r] * r2 : 0.f;
         deviatoric(g, o);
@@ -490,7 +520,7 @@ __device__ __forceinline__ void stencil_load(const T *__restrict__ F, int lx, in
 // features and 4 point values go through a per-wave LDS slot to the MFMA
 // layout (lane (i, g) holds features 4g..4g+3).  The next tile's stencil is
 // loaded while the current one is computed.
-template <typename T>
+template <typename T, int ACT>
 __global__ __launch_bounds__(256) void k_les_mlp_flux(const v4s *__restrict__ img, const float *__restrict__ norms,
                                                        const T *__restrict__ U, T *__restrict__ Q, int lx, int ly,
                                                        int lz, int lu, float dxi, float dyi, float dzi,
@@ -553,7 +583,7 @@ __global__ __launch_bounds__(256) void k_les_mlp_flux(const v4s *__restrict__ im
 
         const int zo = opaque_zero();
         Acts A;
-        mlp_forward(reinterpret_cast<const float *>(biasv) + zo, tiles + zo, lane, xin, A);
+        mlp_forward<ACT>(reinterpret_cast<const float *>(biasv) + zo, tiles + zo, lane, xin, A);
         v4f o = A.out;
 #pragma unroll
         for (int r = 0; r < 4; r++) o[r] *= osc[r] * r2;
@@ -702,11 +732,15 @@ int launch_mlp_pack(const float *P, void *img, hipStream_t st)
 
 int launch_mlp_fb(const float *P, const void *img, const float *norms, const float *feats, const float *labels,
                   long long npool, long long base, long long pa, long long pb, int batch, const MlpTrainBufs &b,
-                  double *loss_part, int nblk, hipStream_t st)
+                  double *loss_part, int nblk, int act, hipStream_t st)
 {
     TrainBufs tb{b.a0, b.a1, b.a2, b.a3, b.dz1, b.dz2, b.dz3, b.dout};
-    hipLaunchKernelGGL(k_mlp_fb, dim3(nblk), dim3(256), 0, st, P, (const v4s *)img, norms, feats, labels, npool, base,
-                       pa, pb, batch, tb, loss_part);
+    if (act == SAM_MLP_ACT_GELU)
+        hipLaunchKernelGGL(k_mlp_fb<SAM_MLP_ACT_GELU>, dim3(nblk), dim3(256), 0, st, P, (const v4s *)img, norms, feats,
+                           labels, npool, base, pa, pb, batch, tb, loss_part);
+    else
+        hipLaunchKernelGGL(k_mlp_fb<SAM_MLP_ACT_RELU>, dim3(nblk), dim3(256), 0, st, P, (const v4s *)img, norms, feats,
+                           labels, npool, base, pa, pb, batch, tb, loss_part);
     return launched_mlp();
 }
 
@@ -732,12 +766,17 @@ int launch_sum_loss(const double *part, int n, double *out, hipStream_t st)
 }
 
 int launch_mlp_infer(const float *P, const void *img, const float *norms, const float *feats, float *out, long long n,
-                     hipStream_t st)
+                     int act, hipStream_t st)
 {
     long long nt = (n + 15) / 16;
     unsigned nb = (unsigned)((nt + 3) / 4);
     if (nb > 2048) nb = 2048;
-    hipLaunchKernelGGL(k_mlp_infer, dim3(nb), dim3(256), 0, st, P, (const v4s *)img, norms, feats, out, n);
+    if (act == SAM_MLP_ACT_GELU)
+        hipLaunchKernelGGL(k_mlp_infer<SAM_MLP_ACT_GELU>, dim3(nb), dim3(256), 0, st, P, (const v4s *)img, norms, feats,
+                           out, n);
+    else
+        hipLaunchKernelGGL(k_mlp_infer<SAM_MLP_ACT_RELU>, dim3(nb), dim3(256), 0, st, P, (const v4s *)img, norms, feats,
+                           out, n);
     return launched_mlp();
 }
 
@@ -749,15 +788,19 @@ int launch_dyn_labels(const float *feats, long long n, float pr_t_inv, float *la
 
 template <typename T>
 int launch_les_mlp_flux(const Dims &d, const void *img, const float *norms, const T *U, T *Q, float dxi, float dyi,
-                        float dzi, const double *cscale, hipStream_t st)
+                        float dzi, const double *cscale, int act, hipStream_t st)
 {
     // 32-bit point indices; x-rows of at least one 16-point tile
     if (d.pfield >= (1ll << 31) || d.nx < 16) return SAM_ERR_UNSUPPORTED;
     long long nt = d.pfield / 16;
     unsigned nb = (unsigned)((nt + 3) / 4);
     if (nb > 2048) nb = 2048;
-    hipLaunchKernelGGL((k_les_mlp_flux<T>), dim3(nb), dim3(256), 0, st, (const v4s *)img, norms, U, Q,
-                       __builtin_ctz(d.nx), __builtin_ctz(d.ny), __builtin_ctz(d.nz), d.lu, dxi, dyi, dzi, cscale);
+    if (act == SAM_MLP_ACT_GELU)
+        hipLaunchKernelGGL((k_les_mlp_flux<T, SAM_MLP_ACT_GELU>), dim3(nb), dim3(256), 0, st, (const v4s *)img, norms, U,
+                           Q, __builtin_ctz(d.nx), __builtin_ctz(d.ny), __builtin_ctz(d.nz), d.lu, dxi, dyi, dzi, cscale);
+    else
+        hipLaunchKernelGGL((k_les_mlp_flux<T, SAM_MLP_ACT_RELU>), dim3(nb), dim3(256), 0, st, (const v4s *)img, norms, U,
+                           Q, __builtin_ctz(d.nx), __builtin_ctz(d.ny), __builtin_ctz(d.nz), d.lu, dxi, dyi, dzi, cscale);
     return launched_mlp();
 }
 
@@ -788,9 +831,9 @@ int launch_pool_norms(const float *feats, const float *labels, long long n, doub
 }
 
 template int launch_les_mlp_flux<double>(const Dims &, const void *, const float *, const double *, double *, float,
-                                         float, float, const double *, hipStream_t);
+                                         float, float, const double *, int, hipStream_t);
 template int launch_les_mlp_flux<float>(const Dims &, const void *, const float *, const float *, float *, float,
-                                        float, float, const double *, hipStream_t);
+                                        float, float, const double *, int, hipStream_t);
 template int launch_box_filter<double>(const Dims &, const double *, int, float *, float *, hipStream_t);
 template int launch_box_filter<float>(const Dims &, const float *, int, float *, float *, hipStream_t);
 

@@ -39,7 +39,7 @@ EXPORTS = (
     "sam_prof_enable", "sam_prof_read", "sam_sync", "sam_device_copy",
     "sam_mlp_create", "sam_mlp_destroy", "sam_mlp_get_params", "sam_mlp_set_params",
     "sam_mlp_add_samples", "sam_mlp_add_samples_host", "sam_mlp_pool_size", "sam_mlp_get_samples",
-    "sam_mlp_clear_pool", "sam_mlp_set_labels",
+    "sam_mlp_clear_pool", "sam_mlp_set_labels", "sam_mlp_set_activation", "sam_mlp_get_activation",
     "sam_mlp_fit_norms", "sam_mlp_train", "sam_mlp_grad", "sam_mlp_infer",
     "sam_attach_mlp", "sam_set_sgs", "sam_sgs_check", "sam_set_mlp_refresh",
     "sam_set_forcing", "sam_set_cl", "sam_set_planes", "sam_planes", "sam_set_dyn_refresh",
@@ -47,6 +47,8 @@ EXPORTS = (
 
 MLP_NPARAM = 10448
 LABELS_FILTERED, LABELS_DYNAMIC = 0, 1
+# hidden activation of the learned closure (SAM_MLP_ACT_*)
+ACT_RELU, ACT_GELU = 0, 1
 
 # kernel classes of sam_prof_read (SAM_K_* in include/samhip.h)
 KERNEL_CLASSES = ("zb", "yb", "xrow", "final", "yf", "zrhs", "spec", "xb", "xf", "sgs", "misc")
@@ -146,6 +148,8 @@ def load():
     L.sam_mlp_get_samples.argtypes = [_hp, ctypes.c_longlong, ctypes.c_longlong, _fp, _fp]
     L.sam_mlp_fit_norms.argtypes = [_hp]
     L.sam_mlp_set_labels.argtypes = [_hp, ctypes.c_int, ctypes.c_double]
+    L.sam_mlp_set_activation.argtypes = [_hp, ctypes.c_int]
+    L.sam_mlp_get_activation.argtypes = [_hp, ctypes.POINTER(ctypes.c_int)]
     L.sam_mlp_train.argtypes = [_hp, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_uint, _fp]
     L.sam_mlp_grad.argtypes = [_hp, _fp, _fp, ctypes.c_int, _fp, ctypes.POINTER(ctypes.c_double)]
     L.sam_mlp_infer.argtypes = [_hp, _fp, _fp, ctypes.c_longlong]
@@ -394,6 +398,15 @@ class Mlp:
         """LABELS_FILTERED (subfilter stress of the box filter) or LABELS_DYNAMIC
         (dynamic-model stress at unit coefficient); set before adding samples"""
         _chk(self.L.sam_mlp_set_labels(self.h, int(mode), ctypes.c_double(pr_t)))
+
+    def set_activation(self, act):
+        """ACT_RELU (default) or ACT_GELU; part of the network, set before training"""
+        _chk(self.L.sam_mlp_set_activation(self.h, int(act)))
+
+    def activation(self):
+        a = ctypes.c_int(-1)
+        _chk(self.L.sam_mlp_get_activation(self.h, ctypes.byref(a)))
+        return a.value
 
     def fit_norms(self):
         _chk(self.L.sam_mlp_fit_norms(self.h))

@@ -27,36 +27,59 @@ def random_params(seed, scale=0.25):
     return p
 
 
-def test_inference_matches_oracle():
+ACTS = [(S.ACT_RELU, M.RELU), (S.ACT_GELU, M.GELU)]
+
+
+@pytest.mark.parametrize("act,oact", ACTS)
+def test_inference_matches_oracle(act, oact):
     p = random_params(3)
     m = S.Mlp(seed=1)
+    m.set_activation(act)
     m.set_params(p)
     rng = np.random.default_rng(4)
     f = rng.standard_normal((1000, 12)).astype(np.float32)
     got = m.infer(f)
-    ref = M.Net(p, bf=True).infer(f)
+    ref = M.Net(p, bf=True, act=oact).infer(f)
     assert rel(got, ref) < 2e-3, rel(got, ref)
     # and close to the unrounded network (BF16 operands, FP32 accumulate)
-    assert rel(got, M.Net(p, bf=False).infer(f)) < 5e-2
+    assert rel(got, M.Net(p, bf=False, act=oact).infer(f)) < 5e-2
 
 
-def test_gradient_matches_oracle():
+@pytest.mark.parametrize("act,oact", ACTS)
+def test_gradient_matches_oracle(act, oact):
     p = random_params(5)
     m = S.Mlp(seed=1)
+    m.set_activation(act)
     m.set_params(p)
     rng = np.random.default_rng(6)
     n = 4096
     f = rng.standard_normal((n, 12)).astype(np.float32)
     y = rng.standard_normal((n, 9)).astype(np.float32)
     loss, g = m.grad(f, y)
-    loss_ref, g_ref = M.Net(p, bf=True).loss_grad(f, y)
+    loss_ref, g_ref = M.Net(p, bf=True, act=oact).loss_grad(f, y)
     assert abs(loss - loss_ref) < 1e-4 * loss_ref
     assert rel(g, g_ref) < 5e-3, rel(g, g_ref)
     for k in ("W1", "W2", "W3", "W4", "b1", "b4"):
         sl = slice(M.OFF[k], M.OFF[k] + int(np.prod(M.SHAPES[k])))
         assert rel(g[sl], g_ref[sl]) < 1e-2, k
-    loss64, g64 = M.Net(p, bf=False).loss_grad(f, y)
+    loss64, g64 = M.Net(p, bf=False, act=oact).loss_grad(f, y)
     assert rel(g, g64) < 5e-2
+
+
+def test_activation_choice():
+    """the two activations give different networks; unknown values are rejected"""
+    p = random_params(3)
+    m = S.Mlp(seed=1)
+    assert m.activation() == S.ACT_RELU
+    m.set_params(p)
+    f = np.random.default_rng(4).standard_normal((256, 12)).astype(np.float32)
+    relu = m.infer(f)
+    m.set_activation(S.ACT_GELU)
+    assert m.activation() == S.ACT_GELU
+    assert rel(m.infer(f), relu) > 1e-2
+    with pytest.raises(S.SamError):
+        m.set_activation(2)
+    assert m.activation() == S.ACT_GELU
 
 
 def test_gradient_with_zero_output_columns():
@@ -155,13 +178,15 @@ def test_zero_network_les_equals_dns(oracle_mod):
     assert rel(les.get_state(), dns.get_state()) < 1e-12
 
 
-def test_les_rhs_with_network(oracle_mod):
+@pytest.mark.parametrize("act,oact", ACTS)
+def test_les_rhs_with_network(oracle_mod, act, oact):
     """RHS of the LES step with a random network against the oracle's
     transforms + projection applied to u_i u_j + tau_ij(MLP oracle)"""
     O = oracle_mod
     n = 32
     p = random_params(13, scale=0.1)
     m = S.Mlp(seed=0)
+    m.set_activation(act)
     m.set_params(p)
     les = S.Sam(S.make_params(n, sgs=S.SGS_MLP, i_strat=1, grav=16.0, lapse=1.0, vis=5e-3))
     S.attach_mlp(les, m)
@@ -177,7 +202,7 @@ def test_les_rhs_with_network(oracle_mod):
         fld, d = (k // 3, k % 3) if k < 9 else (3, k - 9)
         feats[..., k] = 0.5 * (np.roll(ph[..., fld], -1, d) - np.roll(ph[..., fld], 1, d)) * dxi
     flat = feats.reshape(-1, 12, order="F").astype(np.float32)
-    out = M.Net(p, bf=True).infer(flat).reshape(n, n, n, 9, order="F")
+    out = M.Net(p, bf=True, act=oact).infer(flat).reshape(n, n, n, 9, order="F")
     prod = o.products(ph)
     for oi, slot in enumerate((0, 2, 5, 1, 4, 3, 6, 7, 8)):
         prod[..., slot] += out[..., oi]

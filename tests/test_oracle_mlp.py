@@ -3,6 +3,8 @@ The closure has no reference counterpart; its oracle is pinned by finite
 differences of the unrounded network, the BF16 rounding rule and the
 activation's derivative."""
 import numpy as np
+import pytest
+from scipy.special import erf
 
 from oracle import mlp_oracle as M
 
@@ -40,22 +42,40 @@ def test_activation_and_derivative():
     assert M.activation(np.array([0.0]))[0][0] == 0.0 and M.activation(np.array([0.0]))[1][0] == 0.0
 
 
-def test_gradient_finite_differences():
+def test_gelu_activation_and_derivative():
+    """tanh-form GELU: derivative by finite differences; within 1e-3 of the
+    exact x Phi(x) (the tanh form's known accuracy); limits 0 and x; and the
+    kernel's rewrite x sigmoid(2u) = 0.5 x (1 + tanh u)"""
+    x = np.linspace(-6, 6, 241)
+    g, dg = M.activation(x, M.GELU)
+    h = 1e-6
+    fd = (M.activation(x + h, M.GELU)[0] - M.activation(x - h, M.GELU)[0]) / (2 * h)
+    assert np.abs(dg - fd).max() < 1e-8
+    exact = 0.5 * x * (1 + erf(x / np.sqrt(2)))
+    assert np.abs(g - exact).max() < 1e-3
+    assert M.activation(np.array([0.0]), M.GELU)[0][0] == 0.0
+    assert abs(M.activation(np.array([0.0]), M.GELU)[1][0] - 0.5) < 1e-15
+    big = np.array([-30.0, 30.0])
+    assert np.allclose(M.activation(big, M.GELU)[0], [0.0, 30.0], atol=1e-12)
+    u = 0.7978845608028654 * (x + 0.044715 * x ** 3)
+    assert np.abs(x / (1 + np.exp(-2 * u)) - g).max() < 1e-12
+
+
+@pytest.mark.parametrize("act", [M.RELU, M.GELU])
+def test_gradient_finite_differences(act):
     p = random_params(1)
     rng = np.random.default_rng(2)
     f = rng.standard_normal((64, 12))
     y = rng.standard_normal((64, 9))
-    net = M.Net(p, bf=False)
+    net = M.Net(p, bf=False, act=act)
     loss, g = net.loss_grad(f, y)
     idx = rng.choice(np.flatnonzero(g), 40, replace=False)
     for j in idx:
-        for s in (+1, -1):
-            pass
         h = 1e-6
         pp, pm = p.copy(), p.copy()
         pp[j] += h
         pm[j] -= h
-        fd = (M.Net(pp, bf=False).loss_grad(f, y)[0] - M.Net(pm, bf=False).loss_grad(f, y)[0]) / (2 * h)
+        fd = (M.Net(pp, bf=False, act=act).loss_grad(f, y)[0] - M.Net(pm, bf=False, act=act).loss_grad(f, y)[0]) / (2 * h)
         assert abs(fd - g[j]) < 1e-6 * max(1.0, abs(g[j])) + 1e-9, (j, fd, g[j])
 
 
