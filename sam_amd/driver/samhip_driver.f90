@@ -19,6 +19,8 @@
 !                coefficient dynamic-model labels and is scaled by C Delta^2
 !                from the dynamic procedure; 0 trained on box-filter labels
 !   mlp_refresh  i_les 4, mlp_labels 1: steps between C Delta^2 updates (20)
+!   mlp_act      i_les 4: hidden activation the network was trained with,
+!                0 ReLU (default), 1 GELU (tanh form)
 !   cl_couple    i_les 1: 0 (default) the Chollet-Lesieur nu_t(k) enters the
 !                spectra only, as in SAM; 1 also the integrating factor
 ! n_skip_p: planes every n_skip_p steps (write_planes.f): time.out, mean.out,
@@ -35,7 +37,7 @@ program samhip_driver
   use samhip_mod
   implicit none
 
-  integer, parameter :: NIN = 72, NDYN = 6, NPAR = NIN + NDYN, NEXT = 6
+  integer, parameter :: NIN = 72, NDYN = 6, NPAR = NIN + NDYN, NEXT = 7
   character(len=12) :: labels(NPAR + NEXT)
   real(8) :: values(NPAR + NEXT), defaults(NPAR + NEXT)
   logical :: required(NPAR + NEXT), fixed(NPAR + NEXT), found(NPAR + NEXT)
@@ -202,6 +204,7 @@ contains
     rc = sam_mlp_create(mlp, 0_c_int, 0_c_long_long)
     if (rc /= SAM_OK) call fail(int(rc), 'sam_mlp_create: '//trim(sam_error_message()))
     rc = sam_mlp_set_labels(mlp, int(iv('mlp_labels'), c_int), rv('pr_t'))
+    if (rc == SAM_OK) rc = sam_mlp_set_activation(mlp, int(iv('mlp_act'), c_int))
     if (rc == SAM_OK) rc = sam_mlp_set_params(mlp, mlp_params)
     if (rc == SAM_OK) rc = sam_attach_mlp(h, mlp)
     if (rc == SAM_OK) rc = sam_set_mlp_refresh(h, int(iv('mlp_refresh'), c_int))
@@ -284,8 +287,9 @@ contains
     end do
     labels(73:78) = [character(len=12) :: 'time', 'dt', 't_stat', 'nt', 'n_frame_p', 'n_hist']
     defaults(73:78) = 0d0; required(73:78) = .true.; fixed(73:78) = .false.
-    labels(79:84) = [character(len=12) :: 'i_prec', 'i_lapse', 'pr_t', 'mlp_labels', 'mlp_refresh', 'cl_couple']
-    defaults(79:84) = [0d0, 0d0, 1d0, 1d0, 20d0, 0d0]; required(79:84) = .false.; fixed(79:84) = .false.
+    labels(79:85) = [character(len=12) :: 'i_prec', 'i_lapse', 'pr_t', 'mlp_labels', 'mlp_refresh', 'cl_couple', &
+                     'mlp_act']
+    defaults(79:85) = [0d0, 0d0, 1d0, 1d0, 20d0, 0d0, 0d0]; required(79:85) = .false.; fixed(79:85) = .false.
   end subroutine set_labels
 
   ! read_line.f:24-89: one "label value [comment]" per line; '#' in column 1
@@ -406,6 +410,7 @@ contains
     call range_check('mlp_labels', 0, 1)
     call range_check('mlp_refresh', 1, 1000000)
     call range_check('cl_couple', 0, 1)
+    call range_check('mlp_act', 0, 1)
   contains
     subroutine range_check(name, lo, hi)
       character(len=*), intent(in) :: name

@@ -119,6 +119,12 @@ module samhip_mod
        real(c_double), value :: pr_t
      end function sam_mlp_set_labels
 
+     integer(c_int) function sam_mlp_set_activation(m, act) bind(c, name='sam_mlp_set_activation')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: m
+       integer(c_int), value :: act
+     end function sam_mlp_set_activation
+
      integer(c_int) function sam_attach_mlp(h, m) bind(c, name='sam_attach_mlp')
        import :: c_int, c_ptr
        type(c_ptr), value :: h, m

@@ -93,20 +93,24 @@ def test_learned_closure_needs_its_network(tmp_path):
     assert r.returncode == 10 and "mlp.params" in r.stdout
     r = run(tmp_path, **dict(TG64, i_les=4, mlp_labels=2))
     assert r.returncode == 9 and "mlp_labels" in r.stdout
+    r = run(tmp_path, **dict(TG64, i_les=4, mlp_act=2))
+    assert r.returncode == 9 and "mlp_act" in r.stdout
 
 
 @pytest.mark.gpu
 @needs_driver
-def test_learned_closure_run_matches_library(tmp_path):
+@pytest.mark.parametrize("act", [0, 1])
+def test_learned_closure_run_matches_library(tmp_path, act):
     """i_les 4 from SAM's run files gives the state of the same network driven
     through the library directly (unit-coefficient labels, C Delta^2 every 5
-    steps), 32^3 Taylor-Green, 12 steps"""
+    steps), 32^3 Taylor-Green, 12 steps; ReLU (mlp_act 0) and GELU (1)"""
     from sam_amd import les as L
     from sam_amd import samhip as S
     dns = S.Sam(S.make_params(64, vis=1e-3, dt0=0.01))
     dns.set_state(ic.random_solenoidal(64, k0=4, seed=11))
     dns.step(5)
     mlp = S.Mlp(seed=5, pool_capacity=(64 // 2) ** 3)
+    mlp.set_activation(act)
     mlp.set_labels(S.LABELS_DYNAMIC, 0.7)
     mlp.add_samples(dns, 2)
     L.train(mlp, 60, batch=4096, lr=2e-3, seed=1)
@@ -114,7 +118,7 @@ def test_learned_closure_run_matches_library(tmp_path):
     params.tofile(str(tmp_path / "mlp.params"))
 
     run_p = dict(TG64, Nx=32, Ny=32, Nz=32, vis=1e-3, n_steps=12, n_skip_h=4, i_les=4, pr_t=0.7,
-                 mlp_labels=1, mlp_refresh=5)
+                 mlp_labels=1, mlp_refresh=5, mlp_act=act)
     u0 = ic.taylor_green(32)
     sio.write_vel(str(tmp_path / "vel.000000"), u0)
     sio.write_header(str(tmp_path / "header.000000"), run_p, time=0.0, dt=0.01, nt=0)
@@ -123,6 +127,7 @@ def test_learned_closure_run_matches_library(tmp_path):
 
     les = S.Sam(S.make_params(32, vis=1e-3, dt0=0.01, sgs=S.SGS_MLP, pr_t=0.7))
     net = S.Mlp(seed=0)
+    net.set_activation(act)
     net.set_labels(S.LABELS_DYNAMIC, 0.7)
     net.set_params(params)
     S.attach_mlp(les, net)
