@@ -26,6 +26,7 @@
 namespace samhip {
 
 typedef short v4s __attribute__((ext_vector_type(4)));
+typedef short v8s __attribute__((ext_vector_type(8)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
 static inline int launched_mlp()
@@ -55,6 +56,18 @@ __device__ __forceinline__ v4s bf4(v4f x)
 }
 
 __device__ __forceinline__ v4f mfma(v4s a, v4s b, v4f c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
+// Two K blocks of 16 in one gfx950 v_mfma_f32_16x16x32_bf16 (twice the work
+// per instruction of the 16x16x16 form).  Lane (i, g) supplies K slots
+// 8g..8g+7: the 4 values it holds for block 0 and then the 4 it holds for
+// block 1.  The K order inside one MFMA is free as long as A and B agree, so
+// the operands are the 16x16x16 lane tiles of the two blocks side by side.
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+__device__ __forceinline__ v4f mfma2(v4s a0, v4s a1, v4s b0, v4s b1, v4f c)
+{
+    const v8s a = __builtin_shufflevector(a0, a1, 0, 1, 2, 3, 4, 5, 6, 7);
+    const v8s b = __builtin_shufflevector(b0, b1, 0, 1, 2, 3, 4, 5, 6, 7);
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8bf, a), __builtin_bit_cast(v8bf, b), c, 0, 0, 0);
+}
 
 // |x|^2 of a point's scaled features, held 4 per lane in lanes i, i+16, i+32
 // (lane i+48 holds zeros); every lane of the point gets the sum
@@ -181,9 +194,11 @@ __device__ __forceinline__ void fw_hidden(const float *bias, const v4s *tiles, i
 #pragma unroll
     for (int mb = 0; mb < 4; mb++) acc[mb] = bias4(bias, boff, 16 * mb + 4 * g);
 #pragma unroll
-    for (int kb = 0; kb < 4; kb++)
+    for (int kb = 0; kb < 4; kb += 2)
 #pragma unroll
-        for (int mb = 0; mb < 4; mb++) acc[mb] = mfma(tiles[(tb + mb * 4 + kb) * 64 + lane], ain[kb], acc[mb]);
+        for (int mb = 0; mb < 4; mb++)
+            acc[mb] = mfma2(tiles[(tb + mb * 4 + kb) * 64 + lane], tiles[(tb + mb * 4 + kb + 1) * 64 + lane], ain[kb],
+                            ain[kb + 1], acc[mb]);
 #pragma unroll
     for (int mb = 0; mb < 4; mb++) {
         z[mb] = acc[mb];
@@ -210,7 +225,8 @@ __device__ __forceinline__ void mlp_forward(const float *bias, const v4s *tiles,
     fw_hidden<ACT>(bias, tiles, T_W3, BO_3, lane, A.a[1], A.z[2], A.a[2]);
     v4f o = bias4(bias, BO_4, 4 * g);
 #pragma unroll
-    for (int kb = 0; kb < 4; kb++) o = mfma(tiles[(T_W4 + kb) * 64 + lane], A.a[2][kb], o);
+    for (int kb = 0; kb < 4; kb += 2)
+        o = mfma2(tiles[(T_W4 + kb) * 64 + lane], tiles[(T_W4 + kb + 1) * 64 + lane], A.a[2][kb], A.a[2][kb + 1], o);
     A.out = o;
 }
 
@@ -234,7 +250,8 @@ __device__ __forceinline__ void bw_matvec(const v4s *tiles, int tb, int lane, co
     for (int mb = 0; mb < 4; mb++) {
         v4f s = {0, 0, 0, 0};
 #pragma unroll
-        for (int kb = 0; kb < 4; kb++) s = mfma(tiles[(tb + mb * 4 + kb) * 64 + lane], dz[kb], s);
+        for (int kb = 0; kb < 4; kb += 2)
+            s = mfma2(tiles[(tb + mb * 4 + kb) * 64 + lane], tiles[(tb + mb * 4 + kb + 1) * 64 + lane], dz[kb], dz[kb + 1], s);
         da[mb] = s;
     }
 }
