@@ -183,10 +183,22 @@ template <int ACT> __device__ __forceinline__ v4s act_bf4(v4f z)
     return bf4(v4f{act<ACT>(z[0]), act<ACT>(z[1]), act<ACT>(z[2]), act<ACT>(z[3])});
 }
 
+// Weight-tile sources of the forward pass: tile k of this lane from the LDS
+// image, or from registers loaded once per wavefront (forward tiles only).
+struct LdsTiles {
+    const v4s *t;
+    int lane;
+    __device__ __forceinline__ v4s operator()(int k) const { return t[k * 64 + lane]; }
+};
+struct RegTiles {
+    v4s w[T_W4T];
+    __device__ __forceinline__ v4s operator()(int k) const { return w[k]; }
+};
+
 // one hidden layer 64 -> 64: z = W a + b, tiles tb..tb+15 (mb*4+kb); the
 // four output blocks' MFMA chains are interleaved (kb outer)
-template <int ACT>
-__device__ __forceinline__ void fw_hidden(const float *bias, const v4s *tiles, int tb, int boff, int lane,
+template <int ACT, class TS>
+__device__ __forceinline__ void fw_hidden(const float *bias, const TS &tiles, int tb, int boff, int lane,
                                           const v4s *ain, v4f *z, v4s *aout)
 {
     const int g = lane >> 4;
@@ -197,8 +209,7 @@ __device__ __forceinline__ void fw_hidden(const float *bias, const v4s *tiles, i
     for (int kb = 0; kb < 4; kb += 2)
 #pragma unroll
         for (int mb = 0; mb < 4; mb++)
-            acc[mb] = mfma2(tiles[(tb + mb * 4 + kb) * 64 + lane], tiles[(tb + mb * 4 + kb + 1) * 64 + lane], ain[kb],
-                            ain[kb + 1], acc[mb]);
+            acc[mb] = mfma2(tiles(tb + mb * 4 + kb), tiles(tb + mb * 4 + kb + 1), ain[kb], ain[kb + 1], acc[mb]);
 #pragma unroll
     for (int mb = 0; mb < 4; mb++) {
         z[mb] = acc[mb];
@@ -207,26 +218,26 @@ __device__ __forceinline__ void fw_hidden(const float *bias, const v4s *tiles, i
 }
 
 // forward pass of one 16-point tile; x: this lane's 4 normalised inputs;
-// bias and tiles in LDS
-template <int ACT>
-__device__ __forceinline__ void mlp_forward(const float *bias, const v4s *tiles, int lane, v4f x, Acts &A)
+// bias in LDS, weight tiles from `tiles`
+template <int ACT, class TS>
+__device__ __forceinline__ void mlp_forward(const float *bias, const TS &tiles, int lane, v4f x, Acts &A)
 {
     const int g = lane >> 4;
     A.a0 = bf4(x);
     v4f z[4];
 #pragma unroll
-    for (int mb = 0; mb < 4; mb++) z[mb] = mfma(tiles[(T_W1 + mb) * 64 + lane], A.a0, bias4(bias, BO_1, 16 * mb + 4 * g));
+    for (int mb = 0; mb < 4; mb++) z[mb] = mfma(tiles(T_W1 + mb), A.a0, bias4(bias, BO_1, 16 * mb + 4 * g));
 #pragma unroll
     for (int mb = 0; mb < 4; mb++) {
         A.z[0][mb] = z[mb];
         A.a[0][mb] = act_bf4<ACT>(z[mb]);
     }
-    fw_hidden<ACT>(bias, tiles, T_W2, BO_2, lane, A.a[0], A.z[1], A.a[1]);
-    fw_hidden<ACT>(bias, tiles, T_W3, BO_3, lane, A.a[1], A.z[2], A.a[2]);
+    fw_hidden<ACT, TS>(bias, tiles, T_W2, BO_2, lane, A.a[0], A.z[1], A.a[1]);
+    fw_hidden<ACT, TS>(bias, tiles, T_W3, BO_3, lane, A.a[1], A.z[2], A.a[2]);
     v4f o = bias4(bias, BO_4, 4 * g);
 #pragma unroll
     for (int kb = 0; kb < 4; kb += 2)
-        o = mfma2(tiles[(T_W4 + kb) * 64 + lane], tiles[(T_W4 + kb + 1) * 64 + lane], A.a[2][kb], A.a[2][kb + 1], o);
+        o = mfma2(tiles(T_W4 + kb), tiles(T_W4 + kb + 1), A.a[2][kb], A.a[2][kb + 1], o);
     A.out = o;
 }
 
@@ -315,7 +326,7 @@ __global__ __launch_bounds__(256) void k_mlp_fb(const float *__restrict__ P, con
         const int zo = opaque_zero();
         const v4s *tl = tiles + zo;
         Acts A;
-        mlp_forward<ACT>(reinterpret_cast<const float *>(biasv) + zo, tl, lane, x, A);
+        mlp_forward<ACT>(reinterpret_cast<const float *>(biasv) + zo, LdsTiles{tl, lane}, lane, x, A);
         // MSE over the 9 normalised outputs: L = mean over batch*9 of (out - y/os)^2
         v4f dout;
 #pragma unroll
@@ -476,7 +487,7 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const float *__restrict__ P, 
         x *= inv_radius(r2);
         const int zo = opaque_zero();
         Acts A;
-        mlp_forward<ACT>(reinterpret_cast<const float *>(biasv) + zo, tiles + zo, lane, x, A);
+        mlp_forward<ACT>(reinterpret_cast<const float *>(biasv) + zo, LdsTiles{tiles + zo, lane}, lane, x, A);
         v4f o = A.out;
         for (int r = 0; r < 4; r++) o[r] *= (4 * g + r < MLP_OUT) ? norms[16 + 4 * g + r] * r2 : 0.f;
         deviatoric(g, o);
@@ -554,6 +565,10 @@ __global__ __launch_bounds__(256) void k_les_mlp_flux(const v4s *__restrict__ im
     const unsigned ntile = np >> 4;
     float *feat = reinterpret_cast<float *>(featv[w]);
     T *cw = ctr[w];
+    // the forward weight tiles, held in registers for every tile of the wave
+    RegTiles wt;
+#pragma unroll
+    for (int k = 0; k < T_W4T; k++) wt.w[k] = tiles[k * 64 + lane];
 
     // lane constants
     const bool has_f = g < lu;
@@ -600,7 +615,7 @@ __global__ __launch_bounds__(256) void k_les_mlp_flux(const v4s *__restrict__ im
 
         const int zo = opaque_zero();
         Acts A;
-        mlp_forward<ACT>(reinterpret_cast<const float *>(biasv) + zo, tiles + zo, lane, xin, A);
+        mlp_forward<ACT>(reinterpret_cast<const float *>(biasv) + zo, wt, lane, xin, A);
         v4f o = A.out;
 #pragma unroll
         for (int r = 0; r < 4; r++) o[r] *= osc[r] * r2;
