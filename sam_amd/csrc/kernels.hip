@@ -166,7 +166,9 @@ __global__ __launch_bounds__(C * N / EPT) void k_yb(const T *__restrict__ Zr, T 
     TL::template run_from_lds<+1>(lds, tw, c, j0, v);
     cpx<T> *Y = reinterpret_cast<cpx<T> *>(Yr) + (((long long)f * d.nz + z0 + c) * d.nxm + ix) * N;
 #pragma unroll
-    for (int s = 0; s < EPT; s++) st_nt(Y + j0 + TL::kout(s) * TL::TPC, v[s]);
+    // default-policy stores: measured 2% (FP64) and 5% (FP32) faster here
+    // than nt stores
+    for (int s = 0; s < EPT; s++) Y[j0 + TL::kout(s) * TL::TPC] = v[s];
 }
 
 template <typename T>
