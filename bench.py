@@ -37,9 +37,11 @@ def grid_counts(n, i_strat=1):
     return dict(S=S, Z=Z, Y=Y, P=n ** 3, lu=lu, lr=lr)
 
 
-def kernel_bytes(n, bc, i_strat=1):
+def kernel_bytes(n, bc, i_strat=1, zrhs_fused=False):
     """Algorithmic HBM bytes per launch of each step kernel (fused 3-pass
-    model): state/RHS in FP64 (16 B), transform elements bc = 16 or 8 B."""
+    model): state/RHS in FP64 (16 B), transform elements bc = 16 or 8 B.
+    zrhs_fused: k_zrhs also writes the next substep's z inverse (Lu Z
+    fields), which k_zb then does not compute."""
     g = grid_counts(n, i_strat)
     S, Z, Y, lu, lr = g["S"], g["Z"], g["Y"], g["lu"], g["lr"]
     return {
@@ -47,8 +49,21 @@ def kernel_bytes(n, bc, i_strat=1):
         "yb": lu * Z * bc + lu * Y * bc,
         "xrow": lu * Y * bc + lr * Y * bc,
         "yf": lr * Y * bc + lr * Z * bc,
-        "zrhs": lr * Z * bc + 4 * lu * S * 16,
+        "zrhs": lr * Z * bc + 4 * lu * S * 16 + (lu * Z * bc if zrhs_fused else 0),
     }
+
+
+def zrhs_fused(prof):
+    """the timed region's k_zrhs wrote the next z inverse: fewer k_zb than
+    k_zrhs launches (k_zb runs only where the previous k_zrhs did not)"""
+    return prof.get("zb", (0.0, 0))[1] < prof.get("zrhs", (0.0, 0))[1]
+
+
+def bytes_moved(n, bc, prof, steps):
+    """bytes per step the launched kernels move by the same model: the
+    contract's count (step_bytes) less the state reads a fused k_zrhs saves"""
+    kb = kernel_bytes(n, bc, zrhs_fused=zrhs_fused(prof))
+    return sum(kb[k] * prof.get(k, (0.0, 0))[1] for k in kb) / steps
 
 
 def step_bytes(n, bc, i_strat=1, nrk=3):
@@ -90,7 +105,7 @@ def run_gpu(n, precision, steps, warmup, u0, S, keep_state=True):
 
 
 def dominant_roofline(n, bc, prof, steps):
-    kb = kernel_bytes(n, bc)
+    kb = kernel_bytes(n, bc, zrhs_fused=zrhs_fused(prof))
     best = None
     for name, nbytes in kb.items():
         ms, cnt = prof.get(name, (0.0, 0))
@@ -355,6 +370,7 @@ def main():
     roof, per_kernel = dominant_roofline(n, 16, r64["prof"], args.steps)
     roof["traffic"] = measured_traffic(n, roof["kernel"], "f64")
     sb = step_bytes(n, 16)
+    bm64 = bytes_moved(n, 16, r64["prof"], args.steps)
     step_frac = sb / (r64["wall"] / args.steps) / HBM_PEAK
     log(f"[bench] FP64 {ms_step:.2f} ms/step, {value:.1f} Mpt-steps/s, step HBM frac {step_frac:.3f}")
 
@@ -375,8 +391,12 @@ def main():
         "config": {"workload": f"stratified Boussinesq DNS {n}^3, N=4, nu=kappa=5e-4, Pr=1, RK3 CFL 0.5, "
                                f"2/3 dealias, FP64 plain", "n": n, "precision": "fp64"},
         "roofline": roof,
+        # contract model bytes (fused 5-kernel model) and the bytes the
+        # launched kernels move by the same count (less with the fused z inverse)
         "step_hbm": {"bytes_per_step": sb, "frac": round(step_frac, 4),
-                     "achieved_GBps": round(sb / (r64["wall"] / args.steps) / 1e9, 1)},
+                     "achieved_GBps": round(sb / (r64["wall"] / args.steps) / 1e9, 1),
+                     "bytes_moved_per_step": int(bm64),
+                     "frac_moved": round(bm64 / (r64["wall"] / args.steps) / HBM_PEAK, 4)},
         "kernels": per_kernel,
     }
 
@@ -411,6 +431,9 @@ def main():
             "ms_per_step": round(ms_sp, 3),
             "ms_per_step_median_single": round(rsp["median"] * 1e3, 3),
             "step_hbm_frac": round(sb32 / (rsp["wall"] / args.steps) / HBM_PEAK, 4),
+            "step_bytes_moved": int(bytes_moved(n, 8, rsp["prof"], args.steps)),
+            "step_hbm_frac_moved": round(bytes_moved(n, 8, rsp["prof"], args.steps) / (rsp["wall"] / args.steps)
+                                         / HBM_PEAK, 4),
             "roofline": roof32,
             "kernels": pk32,
             "ke_rel_err_vs_fp64": float(abs(e32 - e64) / abs(e64)),

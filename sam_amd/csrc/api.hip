@@ -85,6 +85,10 @@ struct sam_handle {
     int mlp_refresh = 20;        // steps between dynamic-coefficient updates (unit-coefficient networks)
     int mlp_since = 1 << 30;     // steps since the last update (large: update at the next step)
     int dyn_refresh = 1;         // SAM_SGS_DYN: 1 = dynamic procedure every substep (dm.f), N = every N steps
+    // Zb's fields 0..lu-1 hold the z inverse of U[cur] (k_zb's output),
+    // written by the last substep's fused k_zrhs; cleared by every other
+    // writer of Zb or of the state
+    int z_ready = 0;
     int dyn_since = 1 << 30;
     // HIP-event timing of the step's kernels (sam_prof_*)
     bool prof = false;
@@ -401,6 +405,7 @@ extern "C" int sam_set_state(sam_handle *h, const double *u, int ldz)
 {
     if (!h || !u) return set_err(SAM_ERR_VALUE, "null argument");
     h->have_spectra = 0;
+    h->z_ready = 0;
     return h2d_spec(h, h->U[h->cur], u, ldz, h->d.lu);
 }
 
@@ -576,10 +581,11 @@ template <typename T> static int fluxes_to_z(sam_handle *h, int stats)
     const Dims &d = h->d;
     T *Z = (T *)h->Zb, *Y = (T *)h->Yb;
     const int pi = h->prec;
-    {
+    if (!h->z_ready) {
         PROF(SAM_K_ZB);
         RCHK(launch_zb<T>(d, h->U[h->cur], d.nzm, d.lu, Z, h->tw[2][pi], h->st));
     }
+    h->z_ready = 0;   // Zb is overwritten below
     {
         PROF(SAM_K_YB);
         RCHK(launch_yb<T>(d, Z, d.lu, Y, h->tw[1][pi], h->st));
@@ -633,6 +639,7 @@ static int spectra(sam_handle *h, bool target)
 static int force(sam_handle *h)
 {
     if (h->k_force < 0) return SAM_OK;
+    h->z_ready = 0;   // the state changes
     PROF(SAM_K_MISC);
     RCHK(launch_force_shells(h->d, h->k_force, h->U[h->cur], h->ekg, h->st));
     if (!h->ek0_valid) {
@@ -646,6 +653,7 @@ static int force(sam_handle *h)
 template <typename T> static int substep(sam_handle *h, int m, int spectra_now)
 {
     const Dims &d = h->d;
+    bool fused = false;
     if (m == 0) {
         int rc = cl_energy(h);
         if (rc) return rc;
@@ -655,14 +663,17 @@ template <typename T> static int substep(sam_handle *h, int m, int spectra_now)
     {
         PROF(SAM_K_ZRHS);
         const int mode = (h->keep_p && m == h->p.nrk_max - 1) ? 2 : 0;
+        // the DNS substep also writes the next substep's z inverse (Zb in place)
+        T *zout = h->p.sgs == SAM_SGS_NONE ? (T *)h->Zb : nullptr;
         RCHK(launch_zrhs<T>(d, h->k, (const T *)h->Zb, h->U[h->cur], h->R, h->U[1 - h->cur], h->diag, m, mode,
-                            h->tw[2][h->prec], h->st));
+                            h->tw[2][h->prec], h->st, zout, &fused));
     }
     if (spectra_now) {
         rc = spectra(h, true);
         if (rc) return rc;
     }
     h->cur = 1 - h->cur;
+    h->z_ready = fused;
     return SAM_OK;
 }
 
@@ -752,6 +763,7 @@ template <typename T> static int planes_batch(sam_handle *h, int batch, int nf, 
 {
     const Dims &d = h->d;
     const int pi = h->prec;
+    h->z_ready = 0;
     RCHK(launch_planes_spec(d, batch, h->U[h->cur], h->R, (double *)h->planes_spec.p, h->st));
     RCHK(launch_zb<T>(d, (const double *)h->planes_spec.p, d.nzm, nf, (T *)h->Zb, h->tw[2][pi], h->st));
     if (meancol) {
@@ -872,6 +884,7 @@ static int check_nvar(const sam_handle *h, int nvar)
 template <typename T> static int zb_stage(sam_handle *h, const double *u, int ldz, double *out, int nvar)
 {
     const Dims &d = h->d;
+    h->z_ready = 0;   // Zb is the stage buffer
     int rc = ensure(h->stage0, sizeof(double) * 2 * d.ss * nvar);
     if (rc) return rc;
     rc = h2d_spec(h, (double *)h->stage0.p, u, ldz, nvar);
@@ -891,6 +904,7 @@ extern "C" int sam_z_trans_b(sam_handle *h, const double *u, int ldz, double *ou
 template <typename T> static int xyb_stage(sam_handle *h, const double *in, double *out, int nvar)
 {
     const Dims &d = h->d;
+    h->z_ready = 0;   // Zb is the stage buffer
     int rc = ensure(h->Pb, sizeof(T) * d.pfield * d.lr);
     if (rc) return rc;
     rc = upload_T<T>(h, (T *)h->Zb, in, 2 * d.zfield * nvar);
@@ -911,6 +925,7 @@ extern "C" int sam_xy_trans_b(sam_handle *h, const double *in, double *out, int 
 template <typename T> static int xyf_stage(sam_handle *h, const double *in, double *out, int nvar)
 {
     const Dims &d = h->d;
+    h->z_ready = 0;   // Zb is the stage buffer
     int rc = ensure(h->Pb, sizeof(T) * d.pfield * d.lr);
     if (rc) return rc;
     rc = upload_T<T>(h, (T *)h->Pb.p, in, d.pfield * nvar);
@@ -931,6 +946,7 @@ extern "C" int sam_xy_trans_f(sam_handle *h, const double *in, double *out, int 
 template <typename T> static int zf_stage(sam_handle *h, const double *in, double *out, int nvar)
 {
     const Dims &d = h->d;
+    h->z_ready = 0;   // Zb is the stage buffer
     int rc = ensure(h->stage0, sizeof(double) * 2 * d.ss * nvar);
     if (rc) return rc;
     rc = upload_T<T>(h, (T *)h->Zb, in, 2 * d.zfield * nvar);
@@ -1273,6 +1289,7 @@ template <typename T> static int physical_velocity(sam_handle *h)
     int rc = ensure(h->Pb, sizeof(T) * d.pfield * (d.lu + d.lr));
     if (rc) return rc;
     const int pi = h->prec;
+    h->z_ready = 0;
     RCHK(launch_zb<T>(d, h->U[h->cur], d.nzm, d.lu, (T *)h->Zb, h->tw[2][pi], h->st));
     RCHK(launch_yb<T>(d, (const T *)h->Zb, d.lu, (T *)h->Yb, h->tw[1][pi], h->st));
     RCHK(launch_xb<T>(d, (const T *)h->Yb, d.lu, (T *)h->Pb.p, h->tw[0][pi], h->st));

@@ -66,10 +66,13 @@ template <typename T> int launch_zf(const Dims &d, const T *Z, int nvar, double 
 template <typename T> int launch_xrow(const Dims &d, T *Y, int want_stats, double *partials, DevDiag *diag,
                                       const void *twx, hipStream_t st, int *nblocks_out);
 // fused z-forward + projection + RK/IF update (mode 0; mode 2 also keeps the
-// pressure in R's field Lu) or RHS only with the pressure (mode 1)
+// pressure in R's field Lu) or RHS only with the pressure (mode 1).  Zout
+// (may equal Z; null = none): in the update modes also write the z inverse
+// of the new state there (k_zb's output for the next substep); *fused tells
+// whether it was written (the instantiation's columns fit in one wavefront).
 template <typename T> int launch_zrhs(const Dims &d, const StepConsts &k, const T *Z, const double *U, double *R,
                                       double *Unew, const DevDiag *diag, int m, int mode, const void *twz,
-                                      hipStream_t st);
+                                      hipStream_t st, T *Zout = nullptr, bool *fused = nullptr);
 int launch_stats_phys(const Dims &d, const void *P, int is_double, int nvar, double *partials, DevDiag *diag,
                       hipStream_t st, int *nblocks_out);
 int launch_finalize(const Dims &d, const StepConsts &k, const double *partials, int nblocks, DevDiag *diag,

@@ -982,16 +982,34 @@ __device__ __forceinline__ double cl_vist(const StepConsts &k, double fac_cl, in
     return fac_cl * (k.cl_a + k.cl_b * exp(-(k.cl_c * k.k_cl) / sqrt((double)ksq)));
 }
 
-// k_zrhs: LR z-columns of one (kx,ky) -> RHS and RK/IF update of Lu fields
+// k_zrhs: LR z-columns of one (kx,ky) -> RHS and RK/IF update of Lu fields.
+//
+// Zout (optional, update modes only): also the next substep's z inverse of
+// the new state, i.e. what k_zb would write for it, into fields 0..LU-1 of
+// Zout.  Zout may be Zr itself: a block reads only its own (kx,ky) columns,
+// all of them before it writes any.  Thread kz reads the products at z
+// position p(kz) of every column and is the only thread that touches p(kz),
+// so the new state goes into columns 0..LU-1 of the tile right after those
+// reads.  The truncated band of those columns is zeroed and the LU columns
+// are transformed.  The transform runs only in the 8-element instantiations
+// whose columns are each within one wavefront (no workgroup barrier inside
+// it).  Its twiddles come from LDS, so it does not wait on the state and RHS
+// stores in flight.
 template <typename T, int N, int LU, int LR, int EPT>
-__global__ __launch_bounds__(LR * N / EPT) void k_zrhs(const T *__restrict__ Zr, const double *__restrict__ Ur,
+__global__ __launch_bounds__(LR * N / EPT) void k_zrhs(const T *Zr, const double *__restrict__ Ur,
                                                         double *__restrict__ Rr, double *__restrict__ Unr, Dims d,
                                                         StepConsts k, const DevDiag *__restrict__ diag,
-                                                        const cpx<T> *__restrict__ tw, int m, int mode)
+                                                        const cpx<T> *__restrict__ tw, int m, int mode, T *Zout)
 {
     using TL = Tile<T, N, EPT>;
     constexpr int NT = LR * N / EPT;
     __shared__ cpx<T> lds[LR * N];
+    // the fused z inverse is built into the 8-element instantiations only
+    // (with 16 elements per thread it spills)
+    constexpr bool CAN_FUSE = TL::WAVE && EPT == 8;
+    __shared__ cpx<T> twt[CAN_FUSE ? TL::TWL_SIZE : 1];
+    const bool fuse = CAN_FUSE && Zout != nullptr && mode != 1;
+    if (CAN_FUSE && fuse) TL::twl_fill(tw, twt, threadIdx.x, NT);   // ordered by the barrier below
     const long long col = blockIdx.x;
     const int ix = (int)(col / d.nym), iy = (int)(col % d.nym);
     const int q = threadIdx.x / TL::TPC, j0 = threadIdx.x % TL::TPC;
@@ -1058,8 +1076,10 @@ __global__ __launch_bounds__(LR * N / EPT) void k_zrhs(const T *__restrict__ Zr,
                 cpx<double> u = us[n];
                 if (m > 0) u = {u.x + dtf2 * rp[n].x, u.y + dtf2 * rp[n].y};
                 u = {u.x + dtf1 * r[n].x, u.y + dtf1 * r[n].y};
-                st_nt(Un + n * ss + si, cpx<double>{u.x * vis_fac, u.y * vis_fac});
+                const cpx<double> un = {u.x * vis_fac, u.y * vis_fac};
+                st_nt(Un + n * ss + si, un);
                 st_nt(R + n * ss + si, r[n]);
+                if (fuse) lds[lds_idx<T, N>(n, p)] = {(T)un.x, (T)un.y};
             }
             if (mode == 2) st_nt(R + LU * ss + si, pr);   // kept for the planes output
         } else {
@@ -1068,30 +1088,57 @@ __global__ __launch_bounds__(LR * N / EPT) void k_zrhs(const T *__restrict__ Zr,
             R[LU * ss + si] = pr;
         }
     }
+    if constexpr (CAN_FUSE) {
+        if (fuse) {
+            // zero band of the new-state columns (positions kzm+1 .. N-kzm-1)
+            const int nzero = N - d.nzm;
+            for (int e = threadIdx.x; e < LU * nzero; e += NT)
+                lds[lds_idx<T, N>(e / nzero, d.kzm + 1 + e % nzero)] = {(T)0, (T)0};
+            lds_barrier();
+            if (q < LU) {
+                cpx<T> v2[EPT];
+                TL::template run_from_lds<+1>(lds, typename TL::TwLds{twt}, q, j0, v2);
+                cpx<T> *Zo = reinterpret_cast<cpx<T> *>(Zout) + (q * d.zfield + col * N);
+#pragma unroll
+                for (int s = 0; s < EPT; s++) st_nt(Zo + j0 + TL::kout(s) * TL::TPC, v2[s]);
+            }
+        }
+    }
 }
 
 template <typename T>
 int launch_zrhs(const Dims &d, const StepConsts &k, const T *Z, const double *U, double *R, double *Unew,
-                const DevDiag *diag, int m, int mode, const void *twz, hipStream_t st)
+                const DevDiag *diag, int m, int mode, const void *twz, hipStream_t st, T *Zout, bool *fused)
 {
+    // whether the instantiation launched below writes Zout (8 elements per
+    // thread and columns within one wavefront)
+    auto note = [&](bool wave) {
+        if (fused) *fused = wave && Zout != nullptr && mode != 1;
+    };
     const unsigned nb = (unsigned)((long long)d.nxm * d.nym);
     SAM_SWITCH_N(d.nz, {
         if (d.lu == 4) {
-            if constexpr (9 * N / 8 <= 1024 && 9 * N * (int)sizeof(cpx<T>) <= 160 * 1024)
+            if constexpr (9 * N / 8 <= 1024 && 9 * N * (int)sizeof(cpx<T>) <= 160 * 1024) {
+                note(Tile<T, N, 8>::WAVE);
                 hipLaunchKernelGGL((k_zrhs<T, N, 4, 9, 8>), dim3(nb), dim3(9 * N / 8), 0, st, Z, U, R, Unew, d, k, diag,
-                                   (const cpx<T> *)twz, m, mode);
-            else if constexpr (9 * N * (int)sizeof(cpx<T>) <= 160 * 1024)
+                                   (const cpx<T> *)twz, m, mode, Zout);
+            } else if constexpr (9 * N * (int)sizeof(cpx<T>) <= 160 * 1024) {
+                note(false);
                 hipLaunchKernelGGL((k_zrhs<T, N, 4, 9, 16>), dim3(nb), dim3(9 * N / 16), 0, st, Z, U, R, Unew, d, k,
-                                   diag, (const cpx<T> *)twz, m, mode);
-            else
+                                   diag, (const cpx<T> *)twz, m, mode, Zout);
+            } else {
                 return SAM_ERR_UNSUPPORTED;
+            }
         } else {
-            if constexpr (6 * N / 8 <= 1024)
+            if constexpr (6 * N / 8 <= 1024) {
+                note(Tile<T, N, 8>::WAVE);
                 hipLaunchKernelGGL((k_zrhs<T, N, 3, 6, 8>), dim3(nb), dim3(6 * N / 8), 0, st, Z, U, R, Unew, d, k, diag,
-                                   (const cpx<T> *)twz, m, mode);
-            else
+                                   (const cpx<T> *)twz, m, mode, Zout);
+            } else {
+                note(false);
                 hipLaunchKernelGGL((k_zrhs<T, N, 3, 6, 16>), dim3(nb), dim3(6 * N / 16), 0, st, Z, U, R, Unew, d, k,
-                                   diag, (const cpx<T> *)twz, m, mode);
+                                   diag, (const cpx<T> *)twz, m, mode, Zout);
+            }
         }
     });
     return launched();
@@ -1774,7 +1821,7 @@ long long partials_capacity(const Dims &d)
     template int launch_zf<T>(const Dims &, const T *, int, double *, const void *, hipStream_t);                     \
     template int launch_xrow<T>(const Dims &, T *, int, double *, DevDiag *, const void *, hipStream_t, int *);       \
     template int launch_zrhs<T>(const Dims &, const StepConsts &, const T *, const double *, double *, double *,     \
-                                const DevDiag *, int, int, const void *, hipStream_t);                               \
+                                const DevDiag *, int, int, const void *, hipStream_t, T *, bool *);                  \
     template int launch_convert<T>(const T *, double *, long long, hipStream_t);                                      \
     template int launch_convert_to<T>(const double *, T *, long long, hipStream_t);
 SAM_INST(double)
