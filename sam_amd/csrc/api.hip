@@ -3,7 +3,9 @@
 // One handle owns the device-resident state of one grid:
 //   U[2]   spectral state, ping-pong          (Nz_min,Ny_min,Nx_min,Lu) c16
 //   R      RHS of the last substep (+ pressure) (Nz_min,Ny_min,Nx_min,Lu+1) c16
-//   Zb     z-physical work, Lr fields          (Nz,Ny_min,Nx_min,Lr) complex T
+//   Zb     z-physical work, Lr fields          (Nz,Ny_min,Nx_min,Lr) complex T;
+//          between DNS substeps its first Lu fields hold the z inverse of
+//          U[cur] (z_ready)
 //   Yb     y,z-physical work, Lr fields        (Ny,Nx_min,Nz,Lr) complex T
 //   Pb     physical fields (LES path and stage calls only), allocated on demand
 // T = double (SAM_PREC_FP64) or float (SAM_PREC_SPLIT).

@@ -9,7 +9,9 @@
 //   k_yf    Y     -> Z        y forward FFT + y truncation     (xy_trans_f.f:39-54, z2x_decomp.f)
 //   k_zrhs  Z, u, r -> r, u'  z forward FFT, x1/Nz, truncation, divergence, buoyancy,
 //                             projection, integrating-factor RK update
-//                             (z_trans_f.f:21-31, rhs.f:108-142, sam.f:210-221,293-314)
+//                             (z_trans_f.f:21-31, rhs.f:108-142, sam.f:210-221,293-314);
+//                             in the DNS step also k_zb's output for u' (Z, in place),
+//                             so k_zb runs only in the first substep after set_state
 // The unfused k_xb / k_xf / k_zf serve the stage entry points and the LES path.
 #include "internal.hpp"
 
