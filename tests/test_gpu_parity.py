@@ -124,8 +124,9 @@ def test_rhs_full(oracle_mod, strat):
 
 
 def test_taylor_green_64_steps(oracle_mod):
-    """64^3 Taylor-Green, nu = 1/400, dt = 0.01: N steps, FP64 parity 1e-12"""
-    nsteps = 20
+    """The CPU-runnable workload in full: 64^3 Taylor-Green, nu = 1/400,
+    dt = 0.01, 200 steps (t = 2), FP64 parity 1e-12"""
+    nsteps = 200
     o, s = pair(oracle_mod, 64, vis=0.0025, dt0=0.01)
     u0 = ic.taylor_green(64)
     s.set_state(u0)
