@@ -213,6 +213,35 @@ def test_split_precision_bounded(oracle_mod):
     assert err < 1e-5, err
 
 
+def test_stratified_256_500_steps(oracle_mod):
+    """The stratified 256^3 workload in full (seeded IC, N = 4, nu = kappa =
+    5e-4, Pr = 1, CFL 0.5, 500 steps): FP64 parity over the first 2 steps
+    (1e-12), then 500 steps in FP64 and in the split mode.  The split run's
+    kinetic energy is within 1e-5 of FP64, and its dissipation and E(k)
+    (relative L2 over the shells) within 1e-4.  Both stay divergence-free."""
+    kw = dict(i_strat=1, grav=16.0, lapse0=1.0, vis=5e-4, pr=1.0, i_cfl=1, cfl0=0.5)
+    u0 = ic.random_solenoidal(256, k0=4, seed=20261203, i_strat=1)
+    o, s64 = pair(oracle_mod, 256, **kw)
+    s64.set_state(u0)
+    s64.step(2)
+    ref = o.step(u0, 2)[0]
+    assert rel_l2(s64.get_state(), ref) < TOL64
+    s64.step(498, want_spectra=True)
+    sp64 = s64.spectra()
+    _, s32 = pair(oracle_mod, 256, prec=S.PREC_SPLIT, **kw)
+    s32.set_state(u0)
+    s32.step(500, want_spectra=True)
+    sp32 = s32.spectra()
+    ke = abs(sp32["energy_w"] - sp64["energy_w"]) / sp64["energy_w"]
+    diss = abs(sp32["dk"].sum() - sp64["dk"].sum()) / abs(sp64["dk"].sum())
+    ek = rel_l2(sp32["ek"], sp64["ek"])
+    assert ke < 1e-5, ke
+    assert diss < 1e-4, diss
+    assert ek < 1e-4, ek
+    for sp in (sp64, sp32):
+        assert sp["div_max"] < 1e-10, sp["div_max"]
+
+
 # ---------------------------------------------------------------------------
 # LES: Smagorinsky (sm.f) and dynamic (dm.f) closures on the collocated grid
 # ---------------------------------------------------------------------------
