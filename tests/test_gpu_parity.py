@@ -138,6 +138,22 @@ def test_taylor_green_64_steps(oracle_mod):
     np.testing.assert_allclose(d["energy"], st[12], rtol=1e-12)
 
 
+@pytest.mark.parametrize("g", GRIDS, ids=lambda g: "-".join(f"{k}{v}" for k, v in g.items()))
+def test_steps_every_grid(oracle_mod, g):
+    """3 full steps on every stage-test grid (non-cubic, unequal lengths,
+    n_dealias 0 and 2, Lu = 3 and 4): the step's fused z inverse, whose
+    zero band depends on kz_max, against the oracle"""
+    g = dict(g)
+    n, ny, nz = g.pop("n"), g.pop("ny", None), g.pop("nz", None)
+    o, s = pair(oracle_mod, n, ny, nz, vis=5e-3, dt0=0.005, **(dict(grav=16.0, lapse0=1.0) if g.get("i_strat") else {}),
+                **g)
+    u0 = rand_spec(o, o.lu, seed=11) * 1e-2
+    ref = o.step(u0, 3)[0]
+    s.set_state(u0)
+    s.step(3)
+    assert rel_l2(s.get_state(), ref) < TOL64
+
+
 def test_stratified_cfl_steps(oracle_mod):
     """stratified Boussinesq at 32^3: N = 4 (N^2 = grav*lapse/To = 16), CFL dt"""
     nsteps = 10
