@@ -1108,6 +1108,144 @@ __global__ __launch_bounds__(LR * N / EPT) void k_zrhs(const T *Zr, const double
     }
 }
 
+// RHS and RK/IF update at one kz of a (kx,ky) column (k_zrhs_p): state us,
+// previous RHS rp, fluxes F -> new state un, RHS r, pressure pr
+template <int LU>
+__device__ __forceinline__ void zrhs_update(const cpx<double> *us, const cpx<double> *rp, const cpx<double> *F, int kz,
+                                            int ix, int iy, double rkx, double rky, double rk_sq2, const Dims &d,
+                                            const StepConsts &k, const DevDiag *__restrict__ diag, int m, double dtf1,
+                                            double dtf2, double vf, cpx<double> *un, cpx<double> *r, cpx<double> &pr)
+{
+    const double rkz = d.wave_z * (double)kwave(kz, d.kzm, d.nzm);
+    const double rk_sq = rk_sq2 + rkz * rkz;
+    cpx<double> wsav = {0, 0}, tsav = {0, 0};
+    if constexpr (LU == 4) {
+        wsav = us[2];
+        tsav = us[3];
+    }
+    rhs_mode<LU>(F, rkx, rky, rkz, rk_sq, wsav, tsav, k, r, pr);
+    double vt = 0.0;
+    if (k.cl == 2) {
+        const int kzw = kwave(kz, d.kzm, d.nzm), kyw = kwave(iy, d.kym, d.nym);
+        vt = cl_vist(k, diag->fac_cl, ix * ix + kyw * kyw + kzw * kzw) * (dtf1 + dtf2);
+    }
+#pragma unroll
+    for (int n = 0; n < LU; n++) {
+        const double vfn = (n == 3) ? vf * k.pr_inv + vt * k.pr_t_inv : vf + vt;
+        const double vis_fac = exp(-vfn * rk_sq);
+        cpx<double> u = us[n];
+        if (m > 0) u = {u.x + dtf2 * rp[n].x, u.y + dtf2 * rp[n].y};
+        u = {u.x + dtf1 * r[n].x, u.y + dtf1 * r[n].y};
+        un[n] = {u.x * vis_fac, u.y * vis_fac};
+    }
+}
+
+// Persistent k_zrhs for the update modes with the fused z inverse, N = 512
+// (one 512-point column per wavefront).  A block walks (kx,ky) columns
+// c = blockIdx.x + j gridDim.x.  The next column's LR z columns stream from
+// HBM into a second LDS buffer (global_load_lds_dwordx4, no VGPRs) while this
+// column is updated, stored and inverse-transformed.  At one block per CU
+// (VGPRs and LDS) the z columns' HBM latency is then no longer serial with
+// the rest of the block's work.
+// - The DMA is issued after the column's state loads are consumed (the
+//   compiler's waits for them would otherwise drain it) and after its state
+//   and RHS stores, so it overlaps the z inverse.  At the top of the next
+//   iteration the only operations newer than the DMA are the z-inverse
+//   stores, and a wave waits with vmcnt(their number).
+// - Every transform reads its twiddles from LDS: a global twiddle read would
+//   wait on the DMA in flight.
+template <typename T, int N, int LU, int LR>
+__global__ __launch_bounds__(LR * N / 8) void k_zrhs_p(const T *Zr, const double *__restrict__ Ur,
+                                                        double *__restrict__ Rr, double *__restrict__ Unr, Dims d,
+                                                        StepConsts k, const DevDiag *__restrict__ diag,
+                                                        const cpx<T> *__restrict__ tw, int m, int mode, T *Zout,
+                                                        long long ncol)
+{
+    using TL = Tile<T, N, 8>;
+    constexpr int EPT = 8, NT = LR * N / EPT, NW = NT / 64;
+    constexpr int ESZ = (int)sizeof(cpx<T>), CPC = N * ESZ / 1024, NCH = LR * CPC, EPC = 1024 / ESZ;
+    static_assert(TL::TPC == 64 && NT % 64 == 0, "one column per wavefront");
+    static_assert(N * ESZ % 1024 == 0, "columns of whole 1 KB DMA chunks");
+    __shared__ cpx<T> lds[LR * N];   // transform tile
+    __shared__ cpx<T> stg[LR * N];   // next column's z columns, linear (column q at q*N)
+    __shared__ cpx<T> twt[TL::TWL_SIZE];
+    TL::twl_fill(tw, twt, threadIdx.x, NT);
+    const typename TL::TwLds twl{twt};
+    const int lane = threadIdx.x & 63, q = threadIdx.x >> 6, j0 = lane;
+    auto issue = [&](long long c) {
+        for (int i = q; i < NCH; i += NW) {
+            const int qq = i / CPC, part = i % CPC;
+            const cpx<T> *src = reinterpret_cast<const cpx<T> *>(Zr) + (qq * d.zfield + c * N + part * EPC) +
+                                lane * (16 / ESZ);
+            lds_dma16(src, stg + qq * N + part * EPC);
+        }
+    };
+    const cpx<double> *U = reinterpret_cast<const cpx<double> *>(Ur);
+    cpx<double> *R = reinterpret_cast<cpx<double> *>(Rr);
+    cpx<double> *Un = reinterpret_cast<cpx<double> *>(Unr);
+    const long long ss = d.ss;
+    const double inv = 1.0 / (double)N;
+    const double dt = diag->dt, dtf2 = k.zeta[m] * dt, dtf1 = k.gamma[m] * dt, vf = k.vis * (dtf1 + dtf2);
+    const int kz = threadIdx.x;          // one kz per thread (nzm <= NT, checked by the launcher)
+    const bool act = kz < d.nzm;
+    const int p = (kz <= d.kzm) ? kz : kz + N - d.nzm;
+    const int nzero = N - d.nzm;
+    long long c = blockIdx.x;
+    if (c < ncol) issue(c);
+    for (bool first = true; c < ncol; c += gridDim.x, first = false) {
+        const long long nxt = c + gridDim.x;
+        // this wave's chunks of column c have landed; the newer operations
+        // allowed in flight are the previous column's z-inverse stores
+        wait_vmem_le(first || q >= LU ? 0 : EPT);
+        lds_barrier();   // all chunks in stg; the previous column's LDS work is done
+        const int ix = (int)(c / d.nym), iy = (int)(c % d.nym);
+        cpx<T> v[EPT];
+#pragma unroll
+        for (int s = 0; s < EPT; s++) v[s] = stg[q * N + j0 + TL::kin(0, s) * TL::TPC];
+        TL::template run_regs<-1>(lds, twl, q, j0, v);
+        TL::store_natural(lds, q, j0, v);
+        lds_barrier();   // tile complete; stg no longer read
+        const long long si = c * d.nzm + kz;
+        if (act) {
+            cpx<double> us[LU], rp[LU];
+#pragma unroll
+            for (int n = 0; n < LU; n++) {
+                us[n] = ld_nt(U + n * ss + si);
+                rp[n] = m > 0 ? ld_nt(R + n * ss + si) : cpx<double>{0, 0};
+            }
+            const double rkx = d.wave_x * (double)ix;
+            const double rky = d.wave_y * (double)kwave(iy, d.kym, d.nym);
+            cpx<double> un[LU], r[4], pr;
+            cpx<double> F[LR];
+#pragma unroll
+            for (int qq = 0; qq < LR; qq++) {
+                const cpx<T> t = lds[lds_idx<T, N>(qq, p)];
+                F[qq] = {(double)t.x * inv, (double)t.y * inv};
+            }
+            zrhs_update<LU>(us, rp, F, kz, ix, iy, rkx, rky, rkx * rkx + rky * rky, d, k, diag, m, dtf1, dtf2, vf, un,
+                            r, pr);
+#pragma unroll
+            for (int n = 0; n < LU; n++) {
+                st_nt(Un + n * ss + si, un[n]);
+                st_nt(R + n * ss + si, r[n]);
+                lds[lds_idx<T, N>(n, p)] = {(T)un[n].x, (T)un[n].y};
+            }
+            if (mode == 2) st_nt(R + LU * ss + si, pr);
+        }
+        if (nxt < ncol) issue(nxt);
+        for (int e = threadIdx.x; e < LU * nzero; e += NT)
+            lds[lds_idx<T, N>(e / nzero, d.kzm + 1 + e % nzero)] = {(T)0, (T)0};
+        lds_barrier();
+        if (q < LU) {
+            cpx<T> v2[EPT];
+            TL::template run_from_lds<+1>(lds, twl, q, j0, v2);
+            cpx<T> *Zo = reinterpret_cast<cpx<T> *>(Zout) + (q * d.zfield + c * N);
+#pragma unroll
+            for (int s = 0; s < EPT; s++) st_nt(Zo + j0 + TL::kout(s) * TL::TPC, v2[s]);
+        }
+    }
+}
+
 template <typename T>
 int launch_zrhs(const Dims &d, const StepConsts &k, const T *Z, const double *U, double *R, double *Unew,
                 const DevDiag *diag, int m, int mode, const void *twz, hipStream_t st, T *Zout, bool *fused)
@@ -1118,6 +1256,25 @@ int launch_zrhs(const Dims &d, const StepConsts &k, const T *Z, const double *U,
         if (fused) *fused = wave && Zout != nullptr && mode != 1;
     };
     const unsigned nb = (unsigned)((long long)d.nxm * d.nym);
+    // N = 512 update with the fused z inverse: the persistent DMA-pipelined kernel
+    if (d.nz == 512 && Zout != nullptr && mode != 1) {
+        constexpr int N = 512;
+        const long long ncol = (long long)d.nxm * d.nym;
+        if (d.lu == 4 && d.nzm <= 9 * N / 8) {
+            const unsigned g = persistent_grid((const void *)k_zrhs_p<T, N, 4, 9>, 9 * N / 8, ncol);
+            hipLaunchKernelGGL((k_zrhs_p<T, N, 4, 9>), dim3(g), dim3(9 * N / 8), 0, st, Z, U, R, Unew, d, k, diag,
+                               (const cpx<T> *)twz, m, mode, Zout, ncol);
+            note(true);
+            return launched();
+        }
+        if (d.lu == 3 && d.nzm <= 6 * N / 8) {
+            const unsigned g = persistent_grid((const void *)k_zrhs_p<T, N, 3, 6>, 6 * N / 8, ncol);
+            hipLaunchKernelGGL((k_zrhs_p<T, N, 3, 6>), dim3(g), dim3(6 * N / 8), 0, st, Z, U, R, Unew, d, k, diag,
+                               (const cpx<T> *)twz, m, mode, Zout, ncol);
+            note(true);
+            return launched();
+        }
+    }
     SAM_SWITCH_N(d.nz, {
         if (d.lu == 4) {
             if constexpr (9 * N / 8 <= 1024 && 9 * N * (int)sizeof(cpx<T>) <= 160 * 1024) {
