@@ -229,6 +229,26 @@ def test_split_precision_bounded(oracle_mod):
     assert err < 1e-5, err
 
 
+def test_stratified_512_step(oracle_mod):
+    """The headline configuration itself: one step of the 512^3 stratified DNS
+    (the bench's workload and seeded IC; persistent z kernel, double-staged
+    x rows, 512-point y and z transforms) against the oracle.  FP64 1e-12,
+    FP32-split 1e-5."""
+    kw = dict(i_strat=1, grav=16.0, lapse0=1.0, vis=5e-4, pr=1.0, i_cfl=1, cfl0=0.5)
+    u0 = ic.random_solenoidal(512, k0=4, seed=20261203, i_strat=1)
+    o, s64 = pair(oracle_mod, 512, **kw)
+    ref = o.step(u0, 1)[0]
+    s64.set_state(u0)
+    s64.step(1)
+    assert rel_l2(s64.get_state(), ref) < TOL64
+    s64.close()
+    _, s32 = pair(oracle_mod, 512, prec=S.PREC_SPLIT, **kw)
+    s32.set_state(u0)
+    s32.step(1)
+    assert rel_l2(s32.get_state(), ref) < 1e-5
+    s32.close()
+
+
 def test_stratified_256_500_steps(oracle_mod):
     """The stratified 256^3 workload in full (seeded IC, N = 4, nu = kappa =
     5e-4, Pr = 1, CFL 0.5, 500 steps): FP64 parity over the first 2 steps
