@@ -114,44 +114,47 @@ int launch_dm_fields(const Dims &d, const T *P, const T *smag, const T *S6, T *M
 
 // (1,4,1)/6 in x and y (within a z plane), then in z; blockIdx.y = field,
 // so one launch per direction filters every field
+// Block (256 consecutive points of one x-y plane, z + nz * field): x and y
+// from a mask and a shift (power-of-two sizes; nx * ny >= 256), no 64-bit
+// division per point; periodic neighbours from masks.  Sums in FP64, as
+// filter.f.
 template <typename T>
-__global__ __launch_bounds__(256) void k_filter_xy(const T *__restrict__ in, T *__restrict__ out, Grid3 g)
+__global__ __launch_bounds__(256) void k_filter_xy(const T *__restrict__ in, T *__restrict__ out, Grid3 g, int lx)
 {
-    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (i >= g.np) return;
-    in += blockIdx.y * g.np;
-    out += blockIdx.y * g.np;
-    const int x = (int)(i % g.nx), y = (int)((i / g.nx) % g.ny), z = (int)(i / ((long long)g.nx * g.ny));
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    const int x = e & (g.nx - 1), y = e >> lx;
+    const long long plane = (long long)blockIdx.z * g.ny * g.nx;   // field-major: (field, z) planes
+    const int xm = (x - 1) & (g.nx - 1), xp = (x + 1) & (g.nx - 1);
+    const T *r[3] = {in + plane + (long long)((y - 1) & (g.ny - 1)) * g.nx, in + plane + (long long)y * g.nx,
+                     in + plane + (long long)((y + 1) & (g.ny - 1)) * g.nx};
     double acc = 0.0;
 #pragma unroll
-    for (int b = -1; b <= 1; b++) {
-        const double wb = b == 0 ? 4.0 : 1.0;
-        const double r = (double)in[g.at(x - 1, y + b, z)] + 4.0 * (double)in[g.at(x, y + b, z)] +
-                         (double)in[g.at(x + 1, y + b, z)];
-        acc += wb * r;
+    for (int b = 0; b < 3; b++) {
+        const double row = (double)r[b][xm] + 4.0 * (double)r[b][x] + (double)r[b][xp];
+        acc += (b == 1 ? 4.0 : 1.0) * row;
     }
-    out[i] = (T)(acc * (1.0 / 36.0));
+    out[plane + e] = (T)(acc * (1.0 / 36.0));
 }
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_filter_z(const T *__restrict__ in, T *__restrict__ out, Grid3 g)
 {
-    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (i >= g.np) return;
-    in += blockIdx.y * g.np;
-    out += blockIdx.y * g.np;
-    const int x = (int)(i % g.nx), y = (int)((i / g.nx) % g.ny), z = (int)(i / ((long long)g.nx * g.ny));
-    const double r = (double)in[g.at(x, y, z - 1)] + 4.0 * (double)in[g.at(x, y, z)] + (double)in[g.at(x, y, z + 1)];
-    out[i] = (T)(r * (1.0 / 6.0));
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    const int nzf = blockIdx.z, z = nzf & (g.nz - 1);
+    const long long pl = (long long)g.ny * g.nx, fbase = (long long)(nzf - z) * pl;   // field start
+    const double v = (double)in[fbase + ((z - 1) & (g.nz - 1)) * pl + e] + 4.0 * (double)in[fbase + z * pl + e] +
+                     (double)in[fbase + ((z + 1) & (g.nz - 1)) * pl + e];
+    out[fbase + z * pl + e] = (T)(v * (1.0 / 6.0));
 }
 
 // filters nvar fields in -> out using tmp (nvar fields) as scratch
 template <typename T> int launch_filter(const Dims &d, const T *in, T *out, T *tmp, int nvar, hipStream_t st)
 {
     Grid3 g{d.nx, d.ny, d.nz, d.pfield};
-    const unsigned nb = (unsigned)((g.np + 255) / 256);
-    hipLaunchKernelGGL((k_filter_xy<T>), dim3(nb, nvar), dim3(256), 0, st, in, tmp, g);
-    hipLaunchKernelGGL((k_filter_z<T>), dim3(nb, nvar), dim3(256), 0, st, tmp, out, g);
+    if ((long long)d.nx * d.ny % 256) return SAM_ERR_UNSUPPORTED;
+    const dim3 grid((unsigned)((long long)d.nx * d.ny / 256), 1, (unsigned)(d.nz * nvar));
+    hipLaunchKernelGGL((k_filter_xy<T>), grid, dim3(256), 0, st, in, tmp, g, __builtin_ctz(d.nx));
+    hipLaunchKernelGGL((k_filter_z<T>), grid, dim3(256), 0, st, tmp, out, g);
     return launched_sgs();
 }
 
