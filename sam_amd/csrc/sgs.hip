@@ -65,7 +65,8 @@ __global__ __launch_bounds__(256) void k_strain(const T *__restrict__ P, T *__re
 {
     const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
     if (i >= g.np) return;
-    const int x = (int)(i % g.nx), y = (int)((i / g.nx) % g.ny), z = (int)(i / ((long long)g.nx * g.ny));
+    const int lx = __builtin_ctz(g.nx), ly = __builtin_ctz(g.ny);   // power-of-two sizes: no division
+    const int x = (int)(i & (g.nx - 1)), y = (int)((i >> lx) & (g.ny - 1)), z = (int)(i >> (lx + ly));
     double s[6];
     strain_c(P, g, x, y, z, dxi, dyi, dzi, s);
     smag[i] = (T)smag6(s);
@@ -256,7 +257,8 @@ __global__ __launch_bounds__(256) void k_sgs_flux(const T *__restrict__ P, const
 {
     const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
     if (i >= g.np) return;
-    const int x = (int)(i % g.nx), y = (int)((i / g.nx) % g.ny), z = (int)(i / ((long long)g.nx * g.ny));
+    const int lx = __builtin_ctz(g.nx), ly = __builtin_ctz(g.ny);   // power-of-two sizes: no division
+    const int x = (int)(i & (g.nx - 1)), y = (int)((i >> lx) & (g.ny - 1)), z = (int)(i >> (lx + ly));
     double s[6];
     strain_c(P, g, x, y, z, dxi, dyi, dzi, s);
     const double nut = diag->c_delta_sq * smag6(s);
