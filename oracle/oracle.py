@@ -94,6 +94,11 @@ def ref():
         R.ref_z_trans_f.argtypes = [_dp, ctypes.c_int]
         R.ref_rfft.argtypes = [ctypes.c_int, _dp, ctypes.c_int]
         R.ref_cfft.argtypes = [ctypes.c_int, _dp, ctypes.c_int]
+        R.ref_setup_phys.argtypes = [_ip, _dp, _ip]
+        R.ref_rhs.argtypes = [_dp, _dp, _dp]
+        R.ref_vel_max.argtypes = [_dp, ctypes.c_double, _dp]
+        R.ref_spectra.argtypes = [_dp, _dp, _dp]
+        R.ref_div_check.argtypes = [_dp, _dp]
         _ref = R
     return _ref
 
@@ -368,3 +373,59 @@ class Reference:
         c = np.array(c, dtype=np.complex128, copy=True)
         ref().ref_cfft(len(c), _p(c), direction)
         return c
+
+
+class ReferencePhys:
+    """The reference's rhs.f, vel_max.f, spectra.f and div_check.f
+    (oracle/_ref, linked with MPICH, one rank) for one grid and parameter
+    set.  The reference keeps its state in COMMON blocks, so only the most
+    recently set up instance is valid."""
+
+    def __init__(self, cfg: OrcCfg):
+        self.R = ref()
+        ip = (ctypes.c_int * 7)(cfg.nx, cfg.ny, cfg.nz, cfg.n_dealias, cfg.i_strat, cfg.nrk_max, cfg.i_cfl)
+        rp = (ctypes.c_double * 10)(cfg.xl, cfg.yl, cfg.zl, cfg.vis, cfg.pr, cfg.grav, cfg.to, cfg.lapse,
+                                    cfg.cfl0, cfg.dt0)
+        d = (ctypes.c_int * 10)()
+        self.R.ref_setup_phys(ip, rp, d)
+        (self.kxm, self.kym, self.kzm, self.nxm, self.nym, self.nzm,
+         self.nze, self.lu, self.lr, self.lm) = list(d)
+        self.nx, self.ny, self.nz = cfg.nx, cfg.ny, cfg.nz
+
+    def _full(self, u, nvar):
+        """(Nz_min or Nze, Ny_min, Nx_min, nvar) -> (Nze, ...) zero padded"""
+        out = np.zeros((self.nze, self.nym, self.nxm, nvar), np.complex128, order="F")
+        out[: min(u.shape[0], self.nze)] = u[: self.nze, ..., :nvar]
+        return out
+
+    def rhs(self, u):
+        """sam.f's first-substep RHS: r (Nz_min,Ny_min,Nx_min,Lu+1) with the
+        pressure last, and the vel_max statistics st[0:17]"""
+        uf = self._full(u, self.lu)
+        r = np.zeros((self.nze, self.nym, self.nxm, self.lu + 1), np.complex128, order="F")
+        st = np.zeros(17)
+        self.R.ref_rhs(_p(uf), _p(r), _p(st))
+        return np.asfortranarray(r[: self.nzm]), st
+
+    def vel_max(self, phys, dt_in):
+        phys = np.asfortranarray(phys, dtype=np.float64)
+        st = np.zeros(17)
+        self.R.ref_vel_max(_p(phys), dt_in, _p(st))
+        return st
+
+    def spectra(self, u, r):
+        uf = self._full(u, self.lu)
+        rf = self._full(r, self.lr) if r.shape[3] >= self.lr else self._full(
+            np.concatenate([r, np.zeros(r.shape[:3] + (self.lr - r.shape[3],), np.complex128)], axis=3), self.lr)
+        lm = self.lm
+        out = np.zeros(5 * lm + 4)
+        self.R.ref_spectra(_p(uf), _p(rf), _p(out))
+        return dict(ek=out[:lm], tk=out[lm:2 * lm], dk=out[2 * lm:3 * lm], vt=out[3 * lm:4 * lm],
+                    sample=out[4 * lm:5 * lm], energy_w=out[5 * lm], div_rms=out[5 * lm + 1],
+                    div_max=out[5 * lm + 2], n_div=out[5 * lm + 3])
+
+    def div_check(self, u):
+        uf = self._full(u, self.lu)
+        out = np.zeros(3)
+        self.R.ref_div_check(_p(uf), _p(out))
+        return dict(div_max=out[0], div_rms=out[1], n_div=out[2])

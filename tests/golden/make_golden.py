@@ -2,8 +2,9 @@
 
 Runs in the development container only (it needs oracle/_ref/libsamref.so,
 which `make -C oracle ref` compiles from /root/reference's fft.f, init.f,
-fft_init.f, set_range.f, z_trans_b.f, z_trans_f.f, xy_trans_b.f and
-xy_trans_f.f).  The fixtures are inputs and the reference routines' outputs
+fft_init.f, set_range.f, z_trans_b.f, z_trans_f.f, xy_trans_b.f,
+xy_trans_f.f, x2z_decomp.f, z2x_decomp.f, vel_max.f, rhs.f, spectra.f and
+div_check.f).  The fixtures are inputs and the reference routines' outputs
 on them; tests compare the oracle and the GPU against them.
 
     python tests/golden/make_golden.py
@@ -58,6 +59,36 @@ def transforms_fixture(name, nx, ny, nz, xl, yl, zl, n_dealias, i_strat, seed):
                         p_in=p_in, xy_trans_f=xyf, z_trans_f=zf)
 
 
+def rhs_spectra_fixture(name, n, i_strat, i_cfl, seed):
+    """rhs.f (first substep, with vel_max.f), spectra.f and div_check.f of
+    the reference on a seeded solenoidal state, and spectra.f/div_check.f
+    on a state with divergence (a random field), so that div_rms, div_max
+    and n_div are not zero"""
+    from sam_amd import ic
+    cfg = O.make_cfg(n, i_strat=i_strat, grav=16.0, lapse0=1.0, vis=5e-3, i_cfl=i_cfl, cfl0=0.5, dt0=0.01)
+    R = O.ReferencePhys(cfg)
+    u = ic.random_solenoidal(n, k0=4, seed=seed, i_strat=i_strat)
+    if i_strat:
+        u[..., 3] = 0.1 * ic.random_solenoidal(n, k0=3, seed=seed + 1)[..., 0]
+    r, st = R.rhs(u)
+    sp = R.spectra(u, r)
+    dv = R.div_check(u)
+    rng = np.random.default_rng(seed + 2)
+    sh = (R.nzm, R.nym, R.nxm, R.lu)
+    w = np.asfortranarray(rng.standard_normal(sh) + 1j * rng.standard_normal(sh)) * 1e-2
+    w[:, :, 0] = u[:, :, 0]          # the kx = 0 plane stays Hermitian
+    spw = R.spectra(w, r)
+    dvw = R.div_check(w)
+    keys = ("ek", "tk", "dk", "vt", "sample")
+    out = dict(params=np.array([n, i_strat, i_cfl, seed]), u=u, r=r, stats=st, w=w)
+    for tag, s_, d_ in (("", sp, dv), ("w_", spw, dvw)):
+        for k in keys:
+            out[f"{tag}{k}"] = s_[k]
+        out[f"{tag}scalars"] = np.array([s_["energy_w"], s_["div_rms"], s_["div_max"], s_["n_div"]])
+        out[f"{tag}div_check"] = np.array([d_["div_max"], d_["div_rms"], d_["n_div"]])
+    np.savez_compressed(os.path.join(HERE, f"rhs_{name}.npz"), **out)
+
+
 if __name__ == "__main__":
     if not O.ref_available():
         sys.exit("oracle/_ref/libsamref.so missing: run `make -C oracle ref` in the container with /root/reference")
@@ -65,4 +96,6 @@ if __name__ == "__main__":
     transforms_fixture("16s", 16, 16, 16, 2 * np.pi, 2 * np.pi, 2 * np.pi, 1, 1, 7)
     transforms_fixture("16x8x32", 16, 8, 32, 3.0, 5.0, 7.0, 1, 0, 8)
     transforms_fixture("16d0", 16, 16, 16, 2 * np.pi, 2 * np.pi, 2 * np.pi, 0, 0, 9)
+    rhs_spectra_fixture("16u", 16, 0, 0, 21)
+    rhs_spectra_fixture("32s", 32, 1, 1, 22)
     print("golden fixtures written to", HERE)

@@ -1,0 +1,150 @@
+"""Write tests/golden/sam_runs.npz: whole runs of the reference program.
+
+Runs in the development container only.  `make -C oracle ref-sam` compiles
+the reference's `sam` program (dependencies.make's object list, linked with
+the image's MPICH) into oracle/_ref/sam; this script runs it as a one-rank
+MPI singleton in scratch folders under oracle/_ref/runs/ and keeps the
+velocity files it writes (read_field.f / write_field.f layout) together
+with the inputs that produced them.  Nothing of the reference is kept but
+these input/output vectors.
+
+    make -C oracle ref-sam && python tests/golden/make_ref_runs.py
+
+Cases (every field is u(Nz_min, Ny_min, Nx_min, Lu) complex):
+  tg32       Taylor-Green 32^3 from vel.000000 (restart), nu = 1/400,
+             fixed dt 0.01: states after 1 and 10 steps
+  strat32    fresh start (initial_field.f white noise, T' noise), N^2 = 16,
+             nu = 5e-3, CFL 0.5, lapse = lapse0: the IC (n_steps 0 run) and
+             the states after 1 and 10 steps; dt and time from header.*
+  nrk1_16    Euler (nrk_max 1), stratified fresh start 16^3, fixed dt
+  nrk2_16    RK2 (nrk_max 2), stratified fresh start 16^3, CFL 0.5
+  restart16  the strat IC at 16^3 run from a restart, where lapse stays 0
+             (initial_field.f:37 is not executed): 5 steps
+  force16    fresh stratified start with force.f (i_force 1, k_force 2): 5 steps
+  midrk16    n_skip_v 2, 4 steps: vel.000002 holds the state written inside
+             step 2's last substep (sam.f:277-284), vel.000004 the final one
+"""
+import json
+import os
+import shutil
+import subprocess
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from sam_amd import ic  # noqa: E402
+from sam_amd import io as sio  # noqa: E402
+
+SAM = os.path.join(ROOT, "oracle", "_ref", "sam")
+RUNS = os.path.join(ROOT, "oracle", "_ref", "runs")
+TWO_PI = 2 * np.pi
+
+
+def dims(n, strat):
+    km = int((n - 0.1) / 3.0)
+    return 2 * km + 1, 2 * km + 1, km + 1, 4 if strat else 3
+
+
+def run(case, params, files=None):
+    d = os.path.join(RUNS, case)
+    if os.path.isdir(d):
+        shutil.rmtree(d)
+    os.makedirs(d)
+    for name, writer in (files or {}).items():
+        writer(os.path.join(d, name))
+    sio.write_input(os.path.join(d, "input.dat"), **params)
+    r = subprocess.run([SAM], cwd=d, capture_output=True, text=True, timeout=300)
+    if r.returncode != 0:
+        raise RuntimeError(f"{case}: reference exited {r.returncode}\n{r.stdout}\n{r.stderr}")
+    return d
+
+
+def vel(d, nt, n, strat):
+    nzm, nym, nxm, lu = dims(n, strat)
+    return sio.read_vel(os.path.join(d, f"vel.{nt:06d}"), nzm, nym, nxm, lu)
+
+
+def clock(d, nt):
+    h = sio.parse_labels(os.path.join(d, f"header.{nt:06d}"))
+    return np.array([h["time"], h["dt"]])
+
+
+def main():
+    out = {}
+    cases = {}
+
+    # -- Taylor-Green 32^3 from a restart (unstratified, fixed dt)
+    tg = dict(Nx=32, Ny=32, Nz=32, xL=TWO_PI, yL=TWO_PI, zL=TWO_PI, vis=0.0025, i_restart=1, nt_restart=0,
+              i_cfl=0, dt0=0.01, nrk_max=3, n_dealias=1, i_fs=0)
+    u0 = ic.taylor_green(32)
+    files = {"vel.000000": lambda p: sio.write_vel(p, u0),
+             "header.000000": lambda p: sio.write_header(p, tg, time=0.0, dt=0.01, nt=0)}
+    for ns in (1, 10):
+        d = run(f"tg32_{ns}", dict(tg, n_steps=ns), files)
+        out[f"tg32_u{ns}"] = vel(d, ns, 32, 0)
+        out[f"tg32_clock{ns}"] = clock(d, ns)
+    cases["tg32"] = tg
+
+    # -- stratified fresh starts: the IC comes from initial_field.f
+    def strat(n, **kw):
+        p = dict(Nx=n, Ny=n, Nz=n, xL=TWO_PI, yL=TWO_PI, zL=TWO_PI, vis=5e-3, i_restart=0, i_strat=1, grav=16.0,
+                 lapse0=1.0, To=1.0, Pr=1.0, z0=0.0, i_prob=0, flct_u=1.0, flct_t=0.1, i_cfl=1, cfl0=0.5,
+                 nrk_max=3, n_dealias=1, i_fs=0)
+        p.update(kw)
+        return p
+
+    s32 = strat(32)
+    d = run("strat32_0", dict(s32, n_steps=0))
+    out["strat32_u0"] = vel(d, 0, 32, 1)
+    for ns in (1, 10):
+        d = run(f"strat32_{ns}", dict(s32, n_steps=ns))
+        out[f"strat32_u{ns}"] = vel(d, ns, 32, 1)
+        out[f"strat32_clock{ns}"] = clock(d, ns)
+    cases["strat32"] = s32
+
+    for name, kw in (("nrk1_16", dict(nrk_max=1, i_cfl=0, dt0=0.005)), ("nrk2_16", dict(nrk_max=2)),
+                     ("force16", dict(i_force=1, k_force=2))):
+        p = strat(16, **kw)
+        d = run(name + "_0", dict(p, n_steps=0))
+        out[f"{name}_u0"] = vel(d, 0, 16, 1)
+        d = run(name + "_5", dict(p, n_steps=5))
+        out[f"{name}_u5"] = vel(d, 5, 16, 1)
+        out[f"{name}_clock5"] = clock(d, 5)
+        cases[name] = p
+
+    # -- restart of the 16^3 stratified IC: lapse stays 0 (initial_field.f:37 not run)
+    p16 = strat(16)
+    d = run("restart16_ic", dict(p16, n_steps=0))
+    u16 = vel(d, 0, 16, 1)
+    rs = dict(p16, i_restart=1, nt_restart=0)
+    for k in ("i_prob", "flct_u", "flct_t"):
+        rs.pop(k)
+    files = {"vel.000000": lambda p: sio.write_vel(p, u16),
+             "header.000000": lambda p: sio.write_header(p, rs, time=0.0, dt=0.01, nt=0)}
+    d = run("restart16_5", dict(rs, n_steps=5), files)
+    out["restart16_u0"] = u16
+    out["restart16_u5"] = vel(d, 5, 16, 1)
+    out["restart16_clock5"] = clock(d, 5)
+    cases["restart16"] = rs
+
+    # -- the mid-step velocity write of sam.f:277-284
+    mk = strat(16, n_skip_v=2)
+    d = run("midrk16", dict(mk, n_steps=4))
+    out["midrk16_u0"] = u16
+    out["midrk16_u2"] = vel(d, 2, 16, 1)
+    out["midrk16_u4"] = vel(d, 4, 16, 1)
+    out["midrk16_clock2"] = clock(d, 2)
+    out["midrk16_clock4"] = clock(d, 4)
+    cases["midrk16"] = mk
+
+    out["cases_json"] = np.array(json.dumps(cases))
+    np.savez_compressed(os.path.join(HERE, "sam_runs.npz"), **out)
+    print("wrote", os.path.join(HERE, "sam_runs.npz"), sorted(out))
+
+
+if __name__ == "__main__":
+    main()
