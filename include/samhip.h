@@ -101,6 +101,12 @@ const char *sam_last_error(void);
  * set, zero-filled on get. */
 int  sam_set_state(sam_handle *h, const double *u, int ldz);
 int  sam_get_state(sam_handle *h, double *u, int ldz);
+/* The velocity of substep nrk (1..nrk_max) of the coming step after its
+ * zeta update (sam.f:210-221), which is what sam.f:277-284 writes to vel.*
+ * at nt < nt_end (the write inside the step's last substep, nrk = nrk_max,
+ * is the one the file keeps).  nrk = 1 is the state itself.  Runs nrk-1
+ * substeps ahead and restores the handle (state, RHS, diagnostics). */
+int  sam_get_state_substep(sam_handle *h, int nrk, double *u, int ldz);
 /* clock carried in header.* (time, dt, nt; write_header.f:19-25) */
 int  sam_set_clock(sam_handle *h, double time, double dt, long long nt);
 
@@ -158,14 +164,23 @@ int  sam_planes(sam_handle *h, int n_xy, const int *k_xy, int n_xz, const int *j
                 double *mean, double *xy, double *xz, double *yz);
 
 /* ---- stage entry points (parity with the reference's routines) -------- */
-/* z_trans_b.f (case 0): u (ldz,Ny_min,Nx_min,nvar) -> out (Nz,Ny_min,Nx_min,nvar).
- * The reference scatters to (Ny_min,Nx_min,Nz,nvar); this is the same data
- * with z leading, the layout the next stage reads. */
+/* z_trans_b.f:15-52 (case 0, one rank): u (ldz,Ny_min,Nx_min,nvar) ->
+ * out (Ny_min,Nx_min,Nz,nvar), the layout z_trans_b hands to x2z_decomp. */
 int  sam_z_trans_b(sam_handle *h, const double *u, int ldz, double *out, int nvar);
-/* x2z_decomp.f + xy_trans_b.f: (Nz,Ny_min,Nx_min,nvar) -> physical (Nx,Ny,Nz,nvar) real */
+/* x2z_decomp.f:24-152 on one rank (the all-to-all is a copy): in
+ * (Ny_min,Nx_min,Nz,nvar) -> out (Ny,Nx/2,Nz,nvar), y padded with zeros
+ * between ky_max and the stored negative ky; x columns >= Nx_min zero. */
+int  sam_x2z(sam_handle *h, const double *in, double *out, int nvar);
+/* xy_trans_b.f:11-63: in (Ny,Nx/2,Nz,nvar) complex, as x2z_decomp writes it
+ * -> physical out (Nx,Ny,Nz,nvar) real.  Reads the stored modes only (the
+ * first Nx_min columns and the Ny_min stored y rows): xy_trans_b.f never
+ * reads the columns past kx_max, and x2z_decomp leaves the y band zero. */
 int  sam_xy_trans_b(sam_handle *h, const double *in, double *out, int nvar);
-/* xy_trans_f.f + z2x_decomp.f: physical (Nx,Ny,Nz,nvar) -> (Nz,Ny_min,Nx_min,nvar) */
+/* xy_trans_f.f:13-57: physical (Nx,Ny,Nz,nvar) -> (Nz,Ny_min,Nx_min,nvar) */
 int  sam_xy_trans_f(sam_handle *h, const double *in, double *out, int nvar);
+/* z2x_decomp.f:23-68 on one rank (identity reorder, Nze = Nz):
+ * (Nz,Ny_min,Nx_min,nvar) -> (Nze,Ny_min,Nx_min,nvar) */
+int  sam_z2x(sam_handle *h, const double *in, double *out, int nvar);
 /* z_trans_f.f (case 0): (Nz,Ny_min,Nx_min,nvar) -> (Nz_min,Ny_min,Nx_min,nvar) */
 int  sam_z_trans_f(sam_handle *h, const double *in, double *out, int nvar);
 /* rhs.f:32-61 products: physical (Nx,Ny,Nz,Lu) -> (Nx,Ny,Nz,Lr) */
@@ -176,6 +191,16 @@ int  sam_vel_max(sam_handle *h, const double *phys, double dt_in, double *stats)
 /* rhs.f:74-142: F (Nz_min,Ny_min,Nx_min,Lr), usav (Nz_min,Ny_min,Nx_min,Lu)
  * -> r (Nz_min,Ny_min,Nx_min,Lu+1), last field = pressure */
 int  sam_project(sam_handle *h, const double *F, const double *usav, double *r);
+/* The RK update of substep nrk (1..nrk_max) on u (ldz,Ny_min,Nx_min,Lu):
+ *   r_prev != NULL and nrk > 1: u += zeta(nrk) dt r_prev          (sam.f:210-221)
+ *   r != NULL: u = (u + gamma(nrk) dt r) exp(-nu_f (gamma+zeta)(nrk) dt |k|^2),
+ *              nu_f = vis (vis/Pr for T')                         (sam.f:293-314)
+ * r_prev and r are (ldz,Ny_min,Nx_min,Lu); either may be NULL. */
+int  sam_rk_update(sam_handle *h, int nrk, double dt, double *u, const double *r_prev, const double *r, int ldz);
+/* div_check.f:11-36 of the current state: out[0] div_max = sqrt(max |k.u|^2),
+ * out[1] div_rms = sqrt(sum |k.u|^2), out[2] n_div = number of stored modes
+ * with |k.u|^2 > 1e-20 (kx >= 0 modes, no conjugate factor). */
+int  sam_div_check(sam_handle *h, double *out);
 
 /* ---- subgrid-scale pieces (sm.f, dm.f, filter.f) ---------------------- */
 /* eddy viscosity of a physical velocity (Nx,Ny,Nz,Lu) with the handle's

@@ -74,6 +74,8 @@ def lib():
         L.orc_cl_fac.restype = ctypes.c_double
         L.orc_cl_fac.argtypes = [ctypes.c_void_p, ctypes.c_double]
         L.orc_force.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
+        L.orc_div_check.argtypes = [ctypes.c_void_p, _dp, _dp]
+        L.orc_rk_update.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, _dp, _dp, _dp]
         _lib = L
     return _lib
 
@@ -296,6 +298,22 @@ class Oracle:
         return dict(ek=out[:lm], tk=out[lm:2 * lm], dk=out[2 * lm:3 * lm],
                     vt=out[3 * lm:4 * lm], sample=out[4 * lm:5 * lm],
                     energy_w=out[5 * lm], div_rms=out[5 * lm + 1], div_max=out[5 * lm + 2])
+
+    def div_check(self, u):
+        """div_check.f: div_max, div_rms, n_div"""
+        u = np.asfortranarray(u)
+        out = np.zeros(3)
+        self.L.orc_div_check(self.h, _p(u), _p(out))
+        return dict(div_max=out[0], div_rms=out[1], n_div=out[2])
+
+    def rk_update(self, nrk, dt, u, r_prev=None, r=None):
+        """sam.f:210-221 (r_prev) and 293-314 (r) on a copy of u (S layout)"""
+        u = np.array(u, order="F", copy=True)
+        rp = None if r_prev is None else np.asfortranarray(r_prev[..., : self.lu])
+        rr = None if r is None else np.asfortranarray(r[..., : self.lu])
+        self.L.orc_rk_update(self.h, nrk, dt, _p(u), _p(rp) if rp is not None else None,
+                             _p(rr) if rr is not None else None)
+        return u
 
     def eddy_visc(self, phys):
         phys = np.asfortranarray(phys)

@@ -1008,6 +1008,62 @@ void orc_spectra(const orc *o, const cx *u, const cx *r, double fac_cl, double *
     free(acc);
 }
 
+/* div_check.f:11-36 on one rank: out = div_max, div_rms, n_div over the
+ * stored modes (no kx factor 2); sums in the reference's loop order. */
+void orc_div_check(const orc *o, const cx *u, double *out)
+{
+    const size_t ss = (size_t)o->nzm * o->nym * o->nxm;
+    double s2 = 0, mx = 0, cnt = 0;
+    for (int ix = 0; ix < o->nxm; ix++) {
+        const double rkx = o->wave_x * ix;
+        for (int iy = 0; iy < o->nym; iy++) {
+            const double rky = o->wave_y * o->ky[iy];
+            for (int k = 0; k < o->nzm; k++) {
+                const double rkz = o->wave_z * o->kz[k];
+                const size_t i = k + (size_t)o->nzm * (iy + (size_t)o->nym * ix);
+                const cx dv = rkx * u[i] + rky * u[i + ss] + rkz * u[i + 2 * ss];
+                const double dsq = creal(dv * conj(dv));
+                s2 += dsq;
+                if (dsq > mx) mx = dsq;
+                if (dsq > 1.0e-20) cnt += 1;
+            }
+        }
+    }
+    out[0] = sqrt(mx);
+    out[1] = sqrt(s2);
+    out[2] = cnt;
+}
+
+/* The RK update of substep nrk (1-based) on u (S layout): the zeta term
+ * u += zeta dt r_prev for nrk > 1 (sam.f:210-221) when r_prev is given, then
+ * u = (u + gamma dt r) exp(-vis_f |k|^2) with vis_f = vis (gamma+zeta) dt
+ * (/Pr for T') (sam.f:293-314) when r is given. */
+void orc_rk_update(const orc *o, int nrk, double dt, cx *u, const cx *r_prev, const cx *r)
+{
+    const size_t ss = (size_t)o->nzm * o->nym * o->nxm;
+    const int m = nrk - 1;
+    const double dt_fac2 = o->zet[m] * dt, dt_fac1 = o->gam[m] * dt;
+    if (r_prev && nrk > 1)
+        for (size_t i = 0; i < ss * o->lu; i++) u[i] = u[i] + dt_fac2 * r_prev[i];
+    if (!r) return;
+    for (int n = 0; n < o->lu; n++) {
+        double vis_f = o->c.vis * (dt_fac1 + dt_fac2);
+        if (n == 3) vis_f *= o->pr_inv;
+        for (int ix = 0; ix < o->nxm; ix++) {
+            const double rkx = o->wave_x * (double)ix;
+            for (int iy = 0; iy < o->nym; iy++) {
+                const double rky = o->wave_y * (double)o->ky[iy];
+                const double rk_sq2 = rkx * rkx + rky * rky;
+                for (int k = 0; k < o->nzm; k++) {
+                    const double rkz = o->wave_z * (double)o->kz[k];
+                    const size_t i = k + (size_t)o->nzm * (iy + (size_t)o->nym * ix) + ss * n;
+                    u[i] = (u[i] + dt_fac1 * r[i]) * exp(-vis_f * (rk_sq2 + rkz * rkz));
+                }
+            }
+        }
+    }
+}
+
 /* ------------------------------------------------------------------ */
 /* Visualization output: write_planes.f:18-144 with get_mean.f:8-23    */
 /* (periodic case, no jump removal) and x_deriv.f / y_deriv.f           */

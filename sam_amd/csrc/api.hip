@@ -68,7 +68,7 @@ struct sam_handle {
     int cur = 0;
     double *R = nullptr;
     void *Zb = nullptr, *Yb = nullptr;
-    DevBuf Pb, stage0, stage1, sgsw;
+    DevBuf Pb, stage0, stage1, sgsw, snap;
     double *partials = nullptr;
     double *specout = nullptr;
     DevDiag *diag = nullptr;     // [0] step, [1] stage calls
@@ -219,6 +219,7 @@ extern "C" int sam_destroy(sam_handle *h)
     hipFree(h->stage0.p);
     hipFree(h->stage1.p);
     hipFree(h->sgsw.p);
+    hipFree(h->snap.p);
     hipFree(h->partials);
     hipFree(h->specout);
     hipFree(h->ek0);
@@ -730,6 +731,84 @@ extern "C" int sam_rhs(sam_handle *h, double *r, int ldz, int want_spectra, sam_
     return rhs_pass(h, r, ldz, want_spectra, diag, true);
 }
 
+// What a call that runs substeps for a look-ahead (sam_get_state_substep,
+// sam_sgs_check) must give back: the state, R (with the kept pressure), the
+// step diagnostics, the spectra and the counters of the lagged coefficients.
+struct Snapshot {
+    DevDiag dg;
+    int cur, mlp_since, dyn_since, have_spectra, sgs;
+};
+
+static int snap_save(sam_handle *h, Snapshot &s)
+{
+    const Dims &d = h->d;
+    const size_t su = sizeof(double) * 2 * d.ss * d.lu, sr = sizeof(double) * 2 * d.ss * (d.lu + 1);
+    const size_t ssp = sizeof(double) * (5 * d.lm + 8);
+    int rc = ensure(h->snap, su + sr + ssp);
+    if (rc) return rc;
+    char *b = (char *)h->snap.p;
+    HIPCHK(hipMemcpyAsync(b, h->U[h->cur], su, hipMemcpyDeviceToDevice, h->st));
+    HIPCHK(hipMemcpyAsync(b + su, h->R, sr, hipMemcpyDeviceToDevice, h->st));
+    HIPCHK(hipMemcpyAsync(b + su + sr, h->specout, ssp, hipMemcpyDeviceToDevice, h->st));
+    HIPCHK(hipMemcpyAsync(&s.dg, h->diag, sizeof(DevDiag), hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    s.cur = h->cur;
+    s.mlp_since = h->mlp_since;
+    s.dyn_since = h->dyn_since;
+    s.have_spectra = h->have_spectra;
+    s.sgs = h->p.sgs;
+    return SAM_OK;
+}
+
+static int snap_restore(sam_handle *h, const Snapshot &s)
+{
+    const Dims &d = h->d;
+    const size_t su = sizeof(double) * 2 * d.ss * d.lu, sr = sizeof(double) * 2 * d.ss * (d.lu + 1);
+    const size_t ssp = sizeof(double) * (5 * d.lm + 8);
+    const char *b = (const char *)h->snap.p;
+    h->cur = s.cur;
+    HIPCHK(hipMemcpyAsync(h->U[h->cur], b, su, hipMemcpyDeviceToDevice, h->st));
+    HIPCHK(hipMemcpyAsync(h->R, b + su, sr, hipMemcpyDeviceToDevice, h->st));
+    HIPCHK(hipMemcpyAsync(h->specout, b + su + sr, ssp, hipMemcpyDeviceToDevice, h->st));
+    HIPCHK(hipMemcpyAsync(h->diag, &s.dg, sizeof(DevDiag), hipMemcpyHostToDevice, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    h->mlp_since = s.mlp_since;
+    h->dyn_since = s.dyn_since;
+    h->have_spectra = s.have_spectra;
+    h->p.sgs = s.sgs;
+    h->z_ready = 0;   // Zb was overwritten
+    return SAM_OK;
+}
+
+// The velocity the reference holds at substep nrk of the coming step, after
+// its zeta update (sam.f:210-221): the state sam.f:277-284 writes to vel.*
+// for nt < nt_end (the last of the nrk_max writes inside the step wins).
+extern "C" int sam_get_state_substep(sam_handle *h, int nrk, double *u, int ldz)
+{
+    if (!h || !u) return set_err(SAM_ERR_VALUE, "null argument");
+    if (nrk < 1 || nrk > h->p.nrk_max) return set_err(SAM_ERR_VALUE, "nrk must be 1..nrk_max (got %d)", nrk);
+    if (nrk == 1) return sam_get_state(h, u, ldz);
+    if (ldz < h->d.nzm) return set_err(SAM_ERR_SIZE, "ldz %d < Nz_min %d", ldz, h->d.nzm);
+    Snapshot s;
+    int rc = snap_save(h, s);
+    if (rc) return rc;
+    for (int m = 0; m < nrk - 1 && !rc; m++) rc = h->prec ? substep<float>(h, m, 0) : substep<double>(h, m, 0);
+    if (!rc) rc = ensure(h->stage0, sizeof(double) * 2 * h->d.ss * h->d.lu);
+    if (!rc) {
+        DevDiag dg;
+        double *dU = (double *)h->stage0.p;
+        if (hipMemcpyAsync(dU, h->U[h->cur], sizeof(double) * 2 * h->d.ss * h->d.lu, hipMemcpyDeviceToDevice, h->st) !=
+                hipSuccess ||
+            hipMemcpyAsync(&dg, h->diag, sizeof dg, hipMemcpyDeviceToHost, h->st) != hipSuccess ||
+            hipStreamSynchronize(h->st) != hipSuccess)
+            rc = set_err(SAM_ERR_HIP, "look-ahead copy failed: %s", hipGetErrorString(hipGetLastError()));
+        if (!rc) rc = launch_rk_update(h->d, h->k, nrk - 1, dg.dt, dU, h->R, nullptr, h->st);
+        if (!rc) rc = d2h_spec(h, u, dU, ldz, h->d.lu);
+    }
+    const int rc2 = snap_restore(h, s);
+    return rc ? rc : rc2;
+}
+
 extern "C" int sam_set_forcing(sam_handle *h, int k_force)
 {
     if (!h) return set_err(SAM_ERR_VALUE, "null handle");
@@ -888,11 +967,15 @@ template <typename T> static int zb_stage(sam_handle *h, const double *u, int ld
     const Dims &d = h->d;
     h->z_ready = 0;   // Zb is the stage buffer
     int rc = ensure(h->stage0, sizeof(double) * 2 * d.ss * nvar);
+    if (!rc) rc = ensure(h->stage1, sizeof(double) * 2 * d.zfield * nvar);
     if (rc) return rc;
     rc = h2d_spec(h, (double *)h->stage0.p, u, ldz, nvar);
     if (rc) return rc;
     RCHK(launch_zb<T>(d, (const double *)h->stage0.p, d.nzm, nvar, (T *)h->Zb, h->tw[2][h->prec], h->st));
-    return download_T<T>(h, out, (const T *)h->Zb, 2 * d.zfield * nvar);
+    RCHK(launch_z2ref<T>(d, (const T *)h->Zb, nvar, (double *)h->stage1.p, h->st));
+    HIPCHK(hipMemcpyAsync(out, h->stage1.p, sizeof(double) * 2 * d.zfield * nvar, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    return SAM_OK;
 }
 
 extern "C" int sam_z_trans_b(sam_handle *h, const double *u, int ldz, double *out, int nvar)
@@ -903,14 +986,34 @@ extern "C" int sam_z_trans_b(sam_handle *h, const double *u, int ldz, double *ou
     return h->prec ? zb_stage<float>(h, u, ldz, out, nvar) : zb_stage<double>(h, u, ldz, out, nvar);
 }
 
+// x2z_decomp.f on one rank: (Ny_min,Nx_min,Nz,nvar) -> (Ny,Nx/2,Nz,nvar)
+extern "C" int sam_x2z(sam_handle *h, const double *in, double *out, int nvar)
+{
+    if (!h || !in || !out) return set_err(SAM_ERR_VALUE, "null argument");
+    int rc = check_nvar(h, nvar);
+    if (rc) return rc;
+    const Dims &d = h->d;
+    const size_t nin = (size_t)d.zfield * nvar, nout = (size_t)d.ny * (d.nx / 2) * d.nz * nvar;
+    rc = ensure(h->stage0, sizeof(double) * 2 * nin);
+    if (!rc) rc = ensure(h->stage1, sizeof(double) * 2 * nout);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(h->stage0.p, in, sizeof(double) * 2 * nin, hipMemcpyHostToDevice, h->st));
+    RCHK(launch_x2z(d, (const double *)h->stage0.p, nvar, (double *)h->stage1.p, h->st));
+    HIPCHK(hipMemcpyAsync(out, h->stage1.p, sizeof(double) * 2 * nout, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    return SAM_OK;
+}
+
 template <typename T> static int xyb_stage(sam_handle *h, const double *in, double *out, int nvar)
 {
     const Dims &d = h->d;
     h->z_ready = 0;   // Zb is the stage buffer
+    const size_t nin = (size_t)d.ny * (d.nx / 2) * d.nz * nvar;
     int rc = ensure(h->Pb, sizeof(T) * d.pfield * d.lr);
+    if (!rc) rc = ensure(h->stage1, sizeof(double) * 2 * nin);
     if (rc) return rc;
-    rc = upload_T<T>(h, (T *)h->Zb, in, 2 * d.zfield * nvar);
-    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(h->stage1.p, in, sizeof(double) * 2 * nin, hipMemcpyHostToDevice, h->st));
+    RCHK(launch_ref2z<T>(d, (const double *)h->stage1.p, nvar, (T *)h->Zb, h->st));
     RCHK(launch_yb<T>(d, (const T *)h->Zb, nvar, (T *)h->Yb, h->tw[1][h->prec], h->st));
     RCHK(launch_xb<T>(d, (const T *)h->Yb, nvar, (T *)h->Pb.p, h->tw[0][h->prec], h->st));
     return download_T<T>(h, out, (const T *)h->Pb.p, d.pfield * nvar);
@@ -965,6 +1068,50 @@ extern "C" int sam_z_trans_f(sam_handle *h, const double *in, double *out, int n
     int rc = check_nvar(h, nvar);
     if (rc) return rc;
     return h->prec ? zf_stage<float>(h, in, out, nvar) : zf_stage<double>(h, in, out, nvar);
+}
+
+// z2x_decomp.f on one rank: (Nz,Ny_min,Nx_min,nvar) -> (Nze,Ny_min,Nx_min,nvar),
+// the identity reorder (Nze = Nz), through device memory as the stage calls
+extern "C" int sam_z2x(sam_handle *h, const double *in, double *out, int nvar)
+{
+    if (!h || !in || !out) return set_err(SAM_ERR_VALUE, "null argument");
+    int rc = check_nvar(h, nvar);
+    if (rc) return rc;
+    const size_t bytes = sizeof(double) * 2 * (size_t)h->d.zfield * nvar;
+    rc = ensure(h->stage0, bytes);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(h->stage0.p, in, bytes, hipMemcpyHostToDevice, h->st));
+    HIPCHK(hipMemcpyAsync(out, h->stage0.p, bytes, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    return SAM_OK;
+}
+
+extern "C" int sam_rk_update(sam_handle *h, int nrk, double dt, double *u, const double *r_prev, const double *r,
+                             int ldz)
+{
+    if (!h || !u) return set_err(SAM_ERR_VALUE, "null argument");
+    if (nrk < 1 || nrk > h->p.nrk_max) return set_err(SAM_ERR_VALUE, "nrk must be 1..nrk_max (got %d)", nrk);
+    const Dims &d = h->d;
+    const size_t fb = sizeof(double) * 2 * d.ss * d.lu;
+    int rc = ensure(h->stage0, 3 * fb);
+    if (rc) return rc;
+    double *dU = (double *)h->stage0.p, *dRp = dU + 2 * d.ss * d.lu, *dR = dRp + 2 * d.ss * d.lu;
+    const bool zeta = r_prev && nrk > 1;     // sam.f:212 runs the zeta term for nrk > 1 only
+    if ((rc = h2d_spec(h, dU, u, ldz, d.lu))) return rc;
+    if (zeta && (rc = h2d_spec(h, dRp, r_prev, ldz, d.lu))) return rc;
+    if (r && (rc = h2d_spec(h, dR, r, ldz, d.lu))) return rc;
+    RCHK(launch_rk_update(d, h->k, nrk - 1, dt, dU, zeta ? dRp : nullptr, r ? dR : nullptr, h->st));
+    return d2h_spec(h, u, dU, ldz, d.lu);
+}
+
+extern "C" int sam_div_check(sam_handle *h, double *out)
+{
+    if (!h || !out) return set_err(SAM_ERR_VALUE, "null argument");
+    double *dout = h->specout + 5 * h->d.lm + 3;   // 3 spare doubles past the spectra
+    RCHK(launch_div_check(h->d, h->U[h->cur], h->partials, dout, h->st));
+    HIPCHK(hipMemcpyAsync(out, dout, sizeof(double) * 3, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    return SAM_OK;
 }
 
 extern "C" int sam_products(sam_handle *h, const double *phys, double *prod)
@@ -1530,18 +1677,23 @@ extern "C" int sam_sgs_check(sam_handle *h, double tol, double *err, int *switch
     if (h->p.sgs != SAM_SGS_MLP) return SAM_OK;
     const int lm = h->d.lm;
     std::vector<double> spec(5 * lm + 3), tk[3];
+    Snapshot s;
+    int rc = snap_save(h, s);
+    if (rc) return rc;
+    // each pass starts from the saved handle: the network's pass is the first
+    // substep the run would take next (its C Delta^2 and refresh counter), the
+    // dynamic model's pass always runs the procedure
     const int models[3] = {SAM_SGS_NONE, SAM_SGS_DYN, SAM_SGS_MLP};
-    for (int i = 0; i < 3; i++) {
+    for (int i = 0; i < 3 && !rc; i++) {
         h->p.sgs = models[i];
-        int rc = rhs_pass(h, nullptr, h->d.nzm, 1, nullptr, false);
+        if (models[i] == SAM_SGS_DYN) h->dyn_since = 1 << 30;
+        rc = rhs_pass(h, nullptr, h->d.nzm, 1, nullptr, false);
         if (!rc) rc = sam_get_spectra(h, spec.data());
-        if (rc) {
-            h->p.sgs = SAM_SGS_MLP;
-            return rc;
-        }
-        tk[i].assign(spec.begin() + lm, spec.begin() + 2 * lm);
+        if (!rc) tk[i].assign(spec.begin() + lm, spec.begin() + 2 * lm);
+        const int rc2 = snap_restore(h, s);
+        if (!rc) rc = rc2;
     }
-    h->p.sgs = SAM_SGS_MLP;
+    if (rc) return rc;
     double num = 0.0, den = 0.0;
     for (int k = 0; k < lm; k++) {
         const double td = tk[1][k] - tk[0][k], tm = tk[2][k] - tk[0][k];

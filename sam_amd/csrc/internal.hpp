@@ -93,6 +93,17 @@ int launch_strided_copy(const double *src, double *dst, long long n0, long long 
 template <typename T> int launch_convert(const T *src, double *dst, long long n, hipStream_t st);
 template <typename T> int launch_convert_to(const double *src, T *dst, long long n, hipStream_t st);
 
+// stage layouts, RK update, div_check (stage.hip)
+template <typename T> int launch_z2ref(const Dims &d, const T *Z, int nvar, double *out, hipStream_t st);
+template <typename T> int launch_ref2z(const Dims &d, const double *in, int nvar, T *Z, hipStream_t st);
+int launch_x2z(const Dims &d, const double *in, int nvar, double *out, hipStream_t st);
+// zeta term with rp (may be null), gamma term + integrating factor with r (may be null); m = nrk - 1
+int launch_rk_update(const Dims &d, const StepConsts &k, int m, double dt, double *U, const double *rp,
+                     const double *r, hipStream_t st);
+long long div_check_rows(const Dims &d);
+// out (device, 3 doubles): div_max, div_rms, n_div of the state U
+int launch_div_check(const Dims &d, const double *U, double *partials, double *out, hipStream_t st);
+
 // SGS (sgs.hip)
 template <typename T> int launch_strain_smag(const Dims &d, const T *P, T *smag, T *S6, double dxi, double dyi,
                                              double dzi, hipStream_t st);

@@ -555,11 +555,30 @@ __device__ __forceinline__ void xrow_phase_a_ds(cpx<T> *lds, cpx<T> *stg, typena
     }
 }
 
-// vel_max.f partials of the block's rows: sum, sum of squares, max |u| per field
+// sum / max over the first W lanes of a wavefront (W a power of two <= 64):
+// a block of fewer than 64 threads shuffles only among the lanes it has
+template <int W> __device__ __forceinline__ double group_sum(double v)
+{
+#pragma unroll
+    for (int o = W / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+template <int W> __device__ __forceinline__ double group_max(double v)
+{
+#pragma unroll
+    for (int o = W / 2; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// vel_max.f partials of the block's rows: sum, sum of squares, max |u| per
+// field.  Blocks of short rows have fewer threads (NT = 8 for FP64 at
+// Nx = 16) than the row has entries (3 LU), so the row is written in a loop.
 template <typename T, int NT, int LU, int EPT = 8>
 __device__ __forceinline__ void xrow_stats(const cpx<T> (&uf)[LU][EPT], double (*red)[3 * LU], double *__restrict__ out)
 {
     constexpr int NW = (NT + 63) / 64;
+    constexpr int W = NT < 64 ? NT : 64;
+    static_assert((W & (W - 1)) == 0, "power-of-two lane groups");
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
     for (int f = 0; f < LU; f++) {
@@ -571,9 +590,9 @@ __device__ __forceinline__ void xrow_stats(const cpx<T> (&uf)[LU][EPT], double (
             s2 += a * a + b * b;
             mx = fmax(mx, fmax(fabs(a), fabs(b)));
         }
-        s = wave_sum(s);
-        s2 = wave_sum(s2);
-        mx = wave_max(mx);
+        s = group_sum<W>(s);
+        s2 = group_sum<W>(s2);
+        mx = group_max<W>(mx);
         if (lane == 0) {
             red[w][f] = s;
             red[w][LU + f] = s2;
@@ -581,11 +600,11 @@ __device__ __forceinline__ void xrow_stats(const cpx<T> (&uf)[LU][EPT], double (
         }
     }
     lds_barrier();
-    if (threadIdx.x < 3 * LU) {
-        const bool is_max = threadIdx.x >= 2 * LU;
-        double t = red[0][threadIdx.x];
-        for (int i = 1; i < NW; i++) t = is_max ? fmax(t, red[i][threadIdx.x]) : t + red[i][threadIdx.x];
-        out[threadIdx.x] = t;
+    for (int e = threadIdx.x; e < 3 * LU; e += NT) {
+        const bool is_max = e >= 2 * LU;
+        double t = red[0][e];
+        for (int i = 1; i < NW; i++) t = is_max ? fmax(t, red[i][e]) : t + red[i][e];
+        out[e] = t;
     }
 }
 

@@ -31,10 +31,10 @@ SGS_NONE, SGS_SMAG, SGS_DYN, SGS_MLP = 0, 1, 2, 3
 # every symbol include/samhip.h declares
 EXPORTS = (
     "sam_create", "sam_destroy", "sam_get_dims", "sam_last_error",
-    "sam_set_state", "sam_get_state", "sam_set_clock",
+    "sam_set_state", "sam_get_state", "sam_get_state_substep", "sam_set_clock",
     "sam_step", "sam_rhs", "sam_get_spectra",
-    "sam_z_trans_b", "sam_xy_trans_b", "sam_xy_trans_f", "sam_z_trans_f",
-    "sam_products", "sam_vel_max", "sam_project",
+    "sam_z_trans_b", "sam_x2z", "sam_xy_trans_b", "sam_xy_trans_f", "sam_z2x", "sam_z_trans_f",
+    "sam_products", "sam_vel_max", "sam_project", "sam_rk_update", "sam_div_check",
     "sam_eddy_visc", "sam_test_filter",
     "sam_prof_enable", "sam_prof_read", "sam_sync", "sam_device_copy",
     "sam_mlp_create", "sam_mlp_destroy", "sam_mlp_get_params", "sam_mlp_set_params",
@@ -118,17 +118,22 @@ def load():
     L.sam_last_error.restype = ctypes.c_char_p
     L.sam_set_state.argtypes = [_hp, _dp, ctypes.c_int]
     L.sam_get_state.argtypes = [_hp, _dp, ctypes.c_int]
+    L.sam_get_state_substep.argtypes = [_hp, ctypes.c_int, _dp, ctypes.c_int]
     L.sam_set_clock.argtypes = [_hp, ctypes.c_double, ctypes.c_double, ctypes.c_longlong]
     L.sam_step.argtypes = [_hp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(Diag)]
     L.sam_rhs.argtypes = [_hp, _dp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(Diag)]
     L.sam_get_spectra.argtypes = [_hp, _dp]
     L.sam_z_trans_b.argtypes = [_hp, _dp, ctypes.c_int, _dp, ctypes.c_int]
+    L.sam_x2z.argtypes = [_hp, _dp, _dp, ctypes.c_int]
+    L.sam_z2x.argtypes = [_hp, _dp, _dp, ctypes.c_int]
     L.sam_xy_trans_b.argtypes = [_hp, _dp, _dp, ctypes.c_int]
     L.sam_xy_trans_f.argtypes = [_hp, _dp, _dp, ctypes.c_int]
     L.sam_z_trans_f.argtypes = [_hp, _dp, _dp, ctypes.c_int]
     L.sam_products.argtypes = [_hp, _dp, _dp]
     L.sam_vel_max.argtypes = [_hp, _dp, ctypes.c_double, _dp]
     L.sam_project.argtypes = [_hp, _dp, _dp, _dp]
+    L.sam_rk_update.argtypes = [_hp, ctypes.c_int, ctypes.c_double, _dp, _dp, _dp, ctypes.c_int]
+    L.sam_div_check.argtypes = [_hp, _dp]
     L.sam_eddy_visc.argtypes = [_hp, _dp, _dp, _dp]
     L.sam_test_filter.argtypes = [_hp, _dp, _dp, ctypes.c_int]
     L.sam_prof_enable.argtypes = [_hp, ctypes.c_int]
@@ -236,6 +241,13 @@ class Sam:
         _chk(self.L.sam_get_state(self.h, _ptr(u), ldz))
         return u
 
+    def get_state_substep(self, nrk):
+        """the velocity sam.f writes to vel.* inside a step (after the zeta
+        update of substep nrk)"""
+        u = np.zeros((self.nzm, self.nym, self.nxm, self.lu), np.complex128, order="F")
+        _chk(self.L.sam_get_state_substep(self.h, int(nrk), _ptr(u), self.nzm))
+        return u
+
     def set_clock(self, time, dt, nt=0):
         _chk(self.L.sam_set_clock(self.h, time, dt, nt))
 
@@ -260,17 +272,35 @@ class Sam:
 
     # ---- stages (reference routine parity) ------------------------------
     def z_trans_b(self, u, nvar=None):
+        """z_trans_b.f: u (ldz, Ny_min, Nx_min, n) -> (Ny_min, Nx_min, Nz, n)"""
         u = np.asfortranarray(u, dtype=np.complex128)
         nvar = nvar or u.shape[3]
-        out = np.zeros((self.nz, self.nym, self.nxm, nvar), np.complex128, order="F")
+        out = np.zeros((self.nym, self.nxm, self.nz, nvar), np.complex128, order="F")
         _chk(self.L.sam_z_trans_b(self.h, _ptr(u), u.shape[0], _ptr(out), nvar))
         return out
 
+    def x2z(self, a):
+        """x2z_decomp.f: (Ny_min, Nx_min, Nz, n) -> (Ny, Nx/2, Nz, n)"""
+        a = np.asfortranarray(a, dtype=np.complex128)
+        nvar = a.shape[3]
+        out = np.zeros((self.ny, self.nx // 2, self.nz, nvar), np.complex128, order="F")
+        _chk(self.L.sam_x2z(self.h, _ptr(a), _ptr(out), nvar))
+        return out
+
     def xy_trans_b(self, a):
+        """xy_trans_b.f: (Ny, Nx/2, Nz, n) complex -> (Nx, Ny, Nz, n) real"""
         a = np.asfortranarray(a, dtype=np.complex128)
         nvar = a.shape[3]
         out = np.zeros((self.nx, self.ny, self.nz, nvar), np.float64, order="F")
         _chk(self.L.sam_xy_trans_b(self.h, _ptr(a), _ptr(out), nvar))
+        return out
+
+    def z2x(self, a):
+        """z2x_decomp.f: (Nz, Ny_min, Nx_min, n) -> (Nze, Ny_min, Nx_min, n)"""
+        a = np.asfortranarray(a, dtype=np.complex128)
+        nvar = a.shape[3]
+        out = np.zeros((self.nz, self.nym, self.nxm, nvar), np.complex128, order="F")
+        _chk(self.L.sam_z2x(self.h, _ptr(a), _ptr(out), nvar))
         return out
 
     def xy_trans_f(self, p):
@@ -305,6 +335,22 @@ class Sam:
         r = np.zeros((self.nzm, self.nym, self.nxm, self.lu + 1), np.complex128, order="F")
         _chk(self.L.sam_project(self.h, _ptr(F), _ptr(usav), _ptr(r)))
         return r
+
+    def rk_update(self, nrk, dt, u, r_prev=None, r=None):
+        """sam.f:210-221 (r_prev) and sam.f:293-314 (r) on a copy of u"""
+        u = np.array(u, dtype=np.complex128, order="F", copy=True)
+        rp = None if r_prev is None else np.asfortranarray(r_prev[..., : self.lu], dtype=np.complex128)
+        rr = None if r is None else np.asfortranarray(r[..., : self.lu], dtype=np.complex128)
+        for a in (rp, rr):
+            assert a is None or a.shape == u.shape
+        _chk(self.L.sam_rk_update(self.h, int(nrk), float(dt), _ptr(u), _ptr(rp), _ptr(rr), u.shape[0]))
+        return u
+
+    def div_check(self):
+        """div_check.f of the current state: div_max, div_rms, n_div"""
+        out = np.zeros(3)
+        _chk(self.L.sam_div_check(self.h, out.ctypes.data_as(_dp)))
+        return dict(div_max=out[0], div_rms=out[1], n_div=out[2])
 
     def eddy_visc(self, phys):
         phys = np.asfortranarray(phys, dtype=np.float64)

@@ -55,17 +55,19 @@ def test_stage_z_trans_b(oracle_mod, g):
     o, s = pair(oracle_mod, g.pop("n"), **g)
     u = rand_spec(o, o.lu, seed=1)
     ref = o.z_trans_b(u)                              # (Ny_min, Nx_min, Nz, n)
-    got = s.z_trans_b(u)                              # (Nz, Ny_min, Nx_min, n)
-    assert rel_l2(np.transpose(got, (1, 2, 0, 3)), ref) < TOL64
+    got = s.z_trans_b(u)                              # the same layout (z_trans_b.f:31-52)
+    assert rel_l2(got, ref) < TOL64
+    assert rel_l2(s.x2z(ref), o.x2z(ref)) == 0.0     # x2z_decomp.f: pure index map
 
 
 @pytest.mark.parametrize("g", GRIDS, ids=lambda g: "-".join(f"{k}{v}" for k, v in g.items()))
 def test_stage_xy_trans_b(oracle_mod, g):
     g = dict(g)
     o, s = pair(oracle_mod, g.pop("n"), **g)
-    zb = rand_spec(o, o.lu, seed=2, ldz=o.nz)         # (Nz, Ny_min, Nx_min, n), z leading
-    ref = o.xy_trans_b(o.x2z(np.transpose(zb, (1, 2, 0, 3)).copy(order="F")))
-    got = s.xy_trans_b(zb)
+    zb = rand_spec(o, o.lu, seed=2, ldz=o.nz)
+    y = o.x2z(np.transpose(zb, (1, 2, 0, 3)).copy(order="F"))   # (Ny, Nx/2, Nz, n) as x2z_decomp writes it
+    ref = o.xy_trans_b(y)
+    got = s.xy_trans_b(y)
     assert rel_l2(got, ref) < TOL64
 
 
@@ -121,6 +123,21 @@ def test_rhs_full(oracle_mod, strat):
     for key in ("ek", "tk", "dk", "sample"):
         np.testing.assert_allclose(sp[key], sp_ref[key], rtol=1e-11, atol=1e-14)
     np.testing.assert_allclose(sp["energy_w"], sp_ref["energy_w"], rtol=1e-12)
+    # divergence sums (spectra.f:55-60,88-89): round-off level for a solenoidal
+    # state, so compared on an absolute scale; a state with divergence below
+    assert abs(sp["div_rms"] - sp_ref["div_rms"]) < 1e-13 and abs(sp["div_max"] - sp_ref["div_max"]) < 1e-13
+    rng = np.random.default_rng(13)
+    w = np.asfortranarray(u + 1e-2 * (rng.standard_normal(u.shape) + 1j * rng.standard_normal(u.shape)))
+    w[:, :, 0] = u[:, :, 0]
+    s.set_state(w)
+    r_w, _ = s.rhs(want_spectra=True)
+    sp = s.spectra()
+    sp_ref = o.spectra(w, r_w)
+    assert sp_ref["div_max"] > 1e-3
+    np.testing.assert_allclose([sp["div_rms"], sp["div_max"]], [sp_ref["div_rms"], sp_ref["div_max"]], rtol=1e-12)
+    dv, dv_ref = s.div_check(), o.div_check(w)
+    np.testing.assert_allclose([dv["div_max"], dv["div_rms"]], [dv_ref["div_max"], dv_ref["div_rms"]], rtol=1e-12)
+    assert dv["n_div"] == dv_ref["n_div"]
 
 
 def test_taylor_green_64_steps(oracle_mod):
@@ -168,6 +185,20 @@ def test_stratified_cfl_steps(oracle_mod):
     assert abs(d["time"] - t) < 1e-12
 
 
+@pytest.mark.parametrize("strat", [0, 1])
+def test_cfl_short_x_rows(oracle_mod, strat):
+    """CFL dt on 16^3 (FP64 x-row blocks of 8 threads, fewer than the 3 Lu
+    entries of a statistics row): u_max of every field, dt and 3 steps"""
+    o, s = pair(oracle_mod, 16, i_strat=strat, grav=16.0, lapse0=1.0, vis=5e-3, i_cfl=1, cfl0=0.5)
+    u0 = ic.random_solenoidal(16, k0=3, seed=17, i_strat=strat)
+    s.set_state(u0)
+    d = s.step(3)
+    ref, t, dt, st = o.step(u0, 3)
+    assert rel_l2(s.get_state(), ref) < TOL64
+    assert abs(d["dt"] - dt) <= 1e-12 * dt
+    np.testing.assert_allclose(d["u_max"][: o.lu], st[8:8 + o.lu], rtol=1e-12)
+
+
 @pytest.mark.parametrize("prec,n,tol", [(S.PREC_FP64, 128, TOL64), (S.PREC_SPLIT, 64, 1e-5)],
                          ids=["fp64-128", "split-64"])
 def test_double_staged_x_rows(oracle_mod, prec, n, tol):
@@ -197,6 +228,25 @@ def test_long_x_rows(oracle_mod, prec, tol):
     s.set_state(u0)
     s.step(nsteps)
     ref, t, dt, st = o.step(u0, nsteps)
+    err = rel_l2(s.get_state(), ref)
+    assert err < tol, err
+
+
+@pytest.mark.parametrize("prec,tol", [(S.PREC_FP64, TOL64), (S.PREC_SPLIT, 1e-5)], ids=["fp64", "split"])
+@pytest.mark.parametrize("shape", [(32, 1024, 32), (32, 32, 1024)], ids=["y1024", "z1024"])
+def test_long_y_z_columns_1024(oracle_mod, shape, prec, tol):
+    """the 1024-point y and z transforms (16 elements per thread; k_zrhs
+    without the fused z inverse) on thin stratified boxes: 2 full steps
+    against the oracle, FP64 at 1e-12 and split at 1e-5"""
+    nx, ny, nz = shape
+    box = dict(xl=nx / 16.0, yl=ny / 16.0, zl=nz / 16.0)
+    o, s = pair(oracle_mod, nx, ny, nz, i_strat=1, grav=16.0, lapse0=1.0, vis=5e-3, i_cfl=1, cfl0=0.5, prec=prec,
+                **box)
+    u0 = ic.random_solenoidal(nx, ny, nz, k0=4, seed=12, i_strat=1, **box)
+    u0[..., 3] = 0.1 * ic.random_solenoidal(nx, ny, nz, k0=3, seed=13, **box)[..., 0]
+    s.set_state(u0)
+    s.step(2)
+    ref = o.step(u0, 2)[0]
     err = rel_l2(s.get_state(), ref)
     assert err < tol, err
 

@@ -176,3 +176,36 @@ def test_divergence_brute_force():
     sp = o.spectra(w, g["r"])
     assert abs(sp["div_rms"] - np.sqrt(d2.sum())) <= 1e-13 * np.sqrt(d2.sum())
     assert abs(sp["div_max"] - np.sqrt(d2.max())) <= 1e-13 * np.sqrt(d2.max())
+
+
+@pytest.mark.parametrize("name", ["16u", "32s"])
+def test_oracle_div_check_vs_reference(name):
+    g = np.load(os.path.join(GOLD, f"rhs_{name}.npz"))
+    n, strat = (int(v) for v in g["params"][:2])
+    o = O.Oracle(O.make_cfg(n, i_strat=strat))
+    for tag, uu in (("", g["u"]), ("w_", g["w"])):
+        ref = g[f"{tag}div_check"]
+        got = o.div_check(uu)
+        assert abs(got["div_max"] - ref[0]) <= 1e-13 * ref[0] + 1e-30
+        assert abs(got["div_rms"] - ref[1]) <= 1e-13 * ref[1] + 1e-30
+        assert got["n_div"] == ref[2]
+
+
+@pytest.mark.parametrize("name", ["strat32", "nrk1_16", "nrk2_16"])
+def test_rk_update_composes_the_reference_step(name):
+    """one step composed of the oracle's RHS and orc_rk_update (the zeta
+    update before each substep's RHS, the gamma/integrating-factor update
+    after it, sam.f:169-314) equals the reference's vel.000001 / one step"""
+    o, u0, _ = run_case(name)
+    u = np.array(u0, order="F", copy=True)
+    r_prev, dt = None, None
+    for m in range(o.cfg.nrk_max):
+        r, st = o.rhs(u)                    # the RHS of the state before the zeta update
+        if m == 0:
+            dt = st[16] if o.cfg.i_cfl == 1 else o.cfg.dt0
+        u = o.rk_update(m + 1, dt, u, r_prev=r_prev, r=r[..., : o.lu])
+        r_prev = r[..., : o.lu]
+    ref = o.step(u0, 1)[0]
+    assert rel(u, ref) < 1e-13, rel(u, ref)
+    if name == "strat32":
+        assert rel(u, RUNS["strat32_u1"]) < TOL
