@@ -188,6 +188,15 @@ int  sam_products(sam_handle *h, const double *phys, double *prod);
 /* vel_max.f: stats[0..3] u_avg, [4..7] u_var, [8..11] u_max, [12] energy,
  * [13..15] cfl_{x,y,z}_by_dt, [16] dt (dt_in unless i_cfl=1) */
 int  sam_vel_max(sam_handle *h, const double *phys, double dt_in, double *stats);
+/* No counterpart in the reference: sam_vel_max for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops).  phys (Nx,Ny,Nz,Lu) FP64 and stats (17 doubles, sam_vel_max's
+ * layout) are DEVICE pointers; the call allocates nothing and is enqueued on
+ * the handle's stream (sam_sync waits for it).  Its workspace is a
+ * file-scope device array of 6.3 MB per device, held until the process ends
+ * and shared by every call on that device: calls that overlap on one device
+ * race.  Grids above 2^27 points (512^3) return SAM_ERR_SIZE. */
+int  sam_vel_max_noalloc(sam_handle *h, const double *phys, double dt_in, double *stats);
 /* rhs.f:74-142: F (Nz_min,Ny_min,Nx_min,Lr), usav (Nz_min,Ny_min,Nx_min,Lu)
  * -> r (Nz_min,Ny_min,Nx_min,Lu+1), last field = pressure */
 int  sam_project(sam_handle *h, const double *F, const double *usav, double *r);

@@ -15,6 +15,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "internal.hpp"
@@ -258,6 +259,34 @@ static int validate(const sam_params *p)
     return SAM_OK;
 }
 
+// Workspace of sam_vel_max_noalloc, one copy per device for the life of
+// the process: the statistics reduction's partial rows (one row of 3 Lu <= 12
+// doubles per 2048 points, as k_stats_phys writes them), the rows of its
+// second level and the DevDiag it finalises into.  6.3 MB.
+constexpr long long NOALLOC_ROWS = 65536;        // grids up to 2^27 points (512^3)
+constexpr long long NOALLOC_DIAG = NOALLOC_ROWS * 12 + FINALIZE_ROWS;
+constexpr long long NOALLOC_DOUBLES = NOALLOC_DIAG + (long long)(sizeof(DevDiag) / sizeof(double));
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+constexpr int NOALLOC_MAX_DEV = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[NOALLOC_MAX_DEV];   // the array's address on each device
+
+// the device address of `scratch` on device dev (current), looked up once
+static int scratch_setup(int dev)
+{
+    if (dev < 0 || dev >= NOALLOC_MAX_DEV) return set_err(SAM_ERR_NODEVICE, "device %d beyond %d", dev, NOALLOC_MAX_DEV);
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (g_scratch_ptr[dev]) return SAM_OK;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+        (void)hipGetLastError();
+        return set_err(SAM_ERR_HIP, "hipGetSymbolAddress(scratch) failed on device %d: %s", dev, hipGetErrorString(e));
+    }
+    g_scratch_ptr[dev] = (double *)ptr;
+    return SAM_OK;
+}
+
 extern "C" int sam_create(const sam_params *p, sam_handle **out)
 {
     if (!out) return set_err(SAM_ERR_VALUE, "null handle pointer");
@@ -276,6 +305,8 @@ extern "C" int sam_create(const sam_params *p, sam_handle **out)
     HIPCHK(hipGetDeviceProperties(&prop, dev));
     if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
         return set_err(SAM_ERR_NODEVICE, "libsamhip is built for gfx950, device is %s", prop.gcnArchName);
+
+    if ((rc = scratch_setup(dev))) return rc;
 
     sam_handle *h = new sam_handle();
     h->p = *p;
@@ -1155,6 +1186,33 @@ extern "C" int sam_vel_max(sam_handle *h, const double *phys, double dt_in, doub
     stats[14] = dg.spare[1];
     stats[15] = dg.spare[2];
     stats[16] = dg.dt;
+    return SAM_OK;
+}
+
+// sam_vel_max on device buffers with no allocation: the partial rows and the
+// DevDiag live in the device array `scratch` (its address from
+// hipGetSymbolAddress at sam_create), the kernels are sam_vel_max's with the
+// same chunking, so the results equal sam_vel_max's bit for bit.  Enqueued on
+// the handle's stream; calls that overlap on one device share the array and
+// race.
+extern "C" int sam_vel_max_noalloc(sam_handle *h, const double *phys, double dt_in, double *stats)
+{
+    if (!h || !phys || !stats) return set_err(SAM_ERR_VALUE, "null argument");
+    const Dims &d = h->d;
+    const long long rows = (d.pfield + 2047) / 2048;   // k_stats_phys's blocks
+    if (rows > NOALLOC_ROWS)
+        return set_err(SAM_ERR_SIZE,
+                       "sam_vel_max_noalloc: %lld points need %lld partial rows, above the workspace's %lld "
+                       "(at most %lld points)",
+                       d.pfield, rows, NOALLOC_ROWS, NOALLOC_ROWS * 2048);
+    double *ws = (h->dev >= 0 && h->dev < NOALLOC_MAX_DEV) ? g_scratch_ptr[h->dev] : nullptr;
+    if (!ws) return set_err(SAM_ERR_STATE, "no workspace address for device %d", h->dev);
+    DevDiag *dg = reinterpret_cast<DevDiag *>(ws + NOALLOC_DIAG);
+    RCHK(launch_diag_seed(dg, dt_in, h->st));
+    int nb = 0;
+    RCHK(launch_stats_phys(d, phys, 1, d.lu, ws, dg, h->st, &nb));
+    RCHK(launch_finalize(d, h->k, ws, nb, dg, 1, h->st, ws + NOALLOC_ROWS * 12));
+    RCHK(launch_diag_stats(dg, stats, h->st));
     return SAM_OK;
 }
 

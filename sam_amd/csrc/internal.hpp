@@ -75,8 +75,11 @@ template <typename T> int launch_zrhs(const Dims &d, const StepConsts &k, const 
                                       hipStream_t st, T *Zout = nullptr, bool *fused = nullptr);
 int launch_stats_phys(const Dims &d, const void *P, int is_double, int nvar, double *partials, DevDiag *diag,
                       hipStream_t st, int *nblocks_out);
+// rows (may be null): 256 x 12 doubles for the second reduction level when
+// nblocks >= 8192; null = the tail of a partials buffer of partials_capacity(d)
 int launch_finalize(const Dims &d, const StepConsts &k, const double *partials, int nblocks, DevDiag *diag,
-                    int update_dt, hipStream_t st);
+                    int update_dt, hipStream_t st, double *rows = nullptr);
+constexpr int FINALIZE_ROWS = 256 * 12;
 int launch_advance_clock(DevDiag *diag, hipStream_t st);
 int launch_project(const Dims &d, const StepConsts &k, const double *F, const double *usav, double *r, hipStream_t st);
 int launch_products(const Dims &d, const void *P, int is_double, void *Q, hipStream_t st);
@@ -101,6 +104,9 @@ int launch_x2z(const Dims &d, const double *in, int nvar, double *out, hipStream
 int launch_rk_update(const Dims &d, const StepConsts &k, int m, double dt, double *U, const double *rp,
                      const double *r, hipStream_t st);
 long long div_check_rows(const Dims &d);
+// a DevDiag set to zero with dt = dt; and vel_max's 17 outputs of a DevDiag
+int launch_diag_seed(DevDiag *diag, double dt, hipStream_t st);
+int launch_diag_stats(const DevDiag *diag, double *stats, hipStream_t st);
 // out (device, 3 doubles): div_max, div_rms, n_div of the state U
 int launch_div_check(const Dims &d, const double *U, double *partials, double *out, hipStream_t st);
 

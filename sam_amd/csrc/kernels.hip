@@ -1488,12 +1488,12 @@ __global__ __launch_bounds__(256) void k_finalize_partial(const double *__restri
 }
 
 int launch_finalize(const Dims &d, const StepConsts &k, const double *partials, int nblocks, DevDiag *diag,
-                    int update_dt, hipStream_t st)
+                    int update_dt, hipStream_t st, double *rows)
 {
     if (nblocks >= 8 * FIN_THREADS) {
-        // two levels: FIN_G blocks reduce chunks into rows kept past every
-        // other use of the partials buffer, then one block finishes
-        double *rows = const_cast<double *>(partials) + (partials_capacity(d) - (long long)FIN_G * 12);
+        // two levels: FIN_G blocks reduce chunks into rows (by default kept
+        // past every other use of the partials buffer), then one block finishes
+        if (!rows) rows = const_cast<double *>(partials) + (partials_capacity(d) - (long long)FIN_G * 12);
         hipLaunchKernelGGL(k_finalize_partial, dim3(FIN_G), dim3(256), 0, st, partials, nblocks, d.lu, rows);
         hipLaunchKernelGGL(k_finalize, dim3(1), dim3(FIN_THREADS), 0, st, rows, FIN_G, d.lu, diag, k, d, update_dt);
         return launched();

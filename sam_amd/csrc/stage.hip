@@ -262,6 +262,44 @@ int launch_div_check(const Dims &d, const double *U, double *partials, double *o
     return launched_st();
 }
 
+__global__ void k_diag_seed(DevDiag *diag, double dt)
+{
+    double *w = reinterpret_cast<double *>(diag);
+    for (int i = threadIdx.x; i < (int)(sizeof(DevDiag) / sizeof(double)); i += 64) w[i] = 0.0;
+    __syncthreads();
+    if (threadIdx.x == 0) diag->dt = dt;
+}
+
+// sam_vel_max's layout: u_avg[4] u_var[4] u_max[4] energy cfl_{x,y,z}/dt dt
+__global__ void k_diag_stats(const DevDiag *__restrict__ diag, double *__restrict__ stats)
+{
+    const int i = threadIdx.x;
+    if (i < 4) {
+        stats[i] = diag->u_avg[i];
+        stats[4 + i] = diag->u_var[i];
+        stats[8 + i] = diag->u_max[i];
+    } else if (i == 4) {
+        stats[12] = diag->energy;
+    } else if (i < 8) {
+        stats[13 + (i - 5)] = diag->spare[i - 5];
+    } else if (i == 8) {
+        stats[16] = diag->dt;
+    }
+}
+
+int launch_diag_seed(DevDiag *diag, double dt, hipStream_t st)
+{
+    static_assert(sizeof(DevDiag) % sizeof(double) == 0, "DevDiag is whole doubles");
+    hipLaunchKernelGGL(k_diag_seed, dim3(1), dim3(64), 0, st, diag, dt);
+    return launched_st();
+}
+
+int launch_diag_stats(const DevDiag *diag, double *stats, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_diag_stats, dim3(1), dim3(64), 0, st, diag, stats);
+    return launched_st();
+}
+
 template int launch_z2ref<double>(const Dims &, const double *, int, double *, hipStream_t);
 template int launch_z2ref<float>(const Dims &, const float *, int, double *, hipStream_t);
 template int launch_ref2z<double>(const Dims &, const double *, int, double *, hipStream_t);

@@ -34,7 +34,7 @@ EXPORTS = (
     "sam_set_state", "sam_get_state", "sam_get_state_substep", "sam_set_clock",
     "sam_step", "sam_rhs", "sam_get_spectra",
     "sam_z_trans_b", "sam_x2z", "sam_xy_trans_b", "sam_xy_trans_f", "sam_z2x", "sam_z_trans_f",
-    "sam_products", "sam_vel_max", "sam_project", "sam_rk_update", "sam_div_check",
+    "sam_products", "sam_vel_max", "sam_vel_max_noalloc", "sam_project", "sam_rk_update", "sam_div_check",
     "sam_eddy_visc", "sam_test_filter",
     "sam_prof_enable", "sam_prof_read", "sam_sync", "sam_device_copy",
     "sam_mlp_create", "sam_mlp_destroy", "sam_mlp_get_params", "sam_mlp_set_params",
@@ -131,6 +131,7 @@ def load():
     L.sam_z_trans_f.argtypes = [_hp, _dp, _dp, ctypes.c_int]
     L.sam_products.argtypes = [_hp, _dp, _dp]
     L.sam_vel_max.argtypes = [_hp, _dp, ctypes.c_double, _dp]
+    L.sam_vel_max_noalloc.argtypes = [_hp, ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p]
     L.sam_project.argtypes = [_hp, _dp, _dp, _dp]
     L.sam_rk_update.argtypes = [_hp, ctypes.c_int, ctypes.c_double, _dp, _dp, _dp, ctypes.c_int]
     L.sam_div_check.argtypes = [_hp, _dp]
