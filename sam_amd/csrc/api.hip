@@ -487,33 +487,10 @@ static int fetch_diag(sam_handle *h, sam_diag *o)
 }
 
 // physical-space LES fluxes: Y (Lu) -> P -> SGS model -> Q (Lr) -> Y (Lr)
-struct LesBufs {
-    void *smag, *S6, *MS, *UU, *UH, *SH, *MH, *LH, *tmp;
-};
-
-template <typename T> static int les_bufs(sam_handle *h, LesBufs &b, bool dynamic)
+template <typename T> static int les_bufs(sam_handle *h)
 {
     const Dims &d = h->d;
-    const size_t f = sizeof(T) * d.pfield;
-    int rc = ensure(h->Pb, f * (d.lu + d.lr));
-    if (rc) return rc;
-    rc = ensure(h->sgsw, f * (dynamic ? 46 : 1));   // tmp: 6 fields (one filter call)
-    if (rc) return rc;
-    char *w = (char *)h->sgsw.p;
-    b.smag = w;
-    if (!dynamic) {
-        b.S6 = b.MS = b.UU = b.UH = b.SH = b.MH = b.LH = b.tmp = nullptr;
-        return SAM_OK;
-    }
-    b.S6 = w + f * 1;
-    b.MS = w + f * 7;
-    b.UU = w + f * 13;
-    b.UH = w + f * 19;
-    b.SH = w + f * 22;
-    b.MH = w + f * 28;
-    b.LH = w + f * 34;
-    b.tmp = w + f * 40;
-    return SAM_OK;
+    return ensure(h->Pb, sizeof(T) * d.pfield * (d.lu + d.lr));
 }
 
 static double delta_sq(const sam_handle *h)
@@ -522,25 +499,19 @@ static double delta_sq(const sam_handle *h)
     return pow(dx * dy * dz, 2.0 / 3.0);   // init.f:28
 }
 
-// eddy-viscosity coefficient C Delta^2 into diag->c_delta_sq (and |S| into b.smag)
+// eddy-viscosity coefficient C Delta^2 into dg->c_delta_sq: the dynamic
+// procedure of dm.f (dyn) or c_smag Delta^2 of sm.f
 template <typename T>
-static int sgs_coefficient(sam_handle *h, const T *P, LesBufs &b, DevDiag *dg, int report, bool dyn)
+static int sgs_coefficient(sam_handle *h, const T *P, DevDiag *dg, int report, bool dyn)
 {
     const Dims &d = h->d;
-    const double dxi = d.nx / h->p.xl, dyi = d.ny / h->p.yl, dzi = d.nz / h->p.zl;   // 1/dx (init.f:22-27)
-    RCHK(launch_strain_smag<T>(d, P, (T *)b.smag, dyn ? (T *)b.S6 : nullptr, dxi, dyi, dzi, h->st));
     if (!dyn) {
         RCHK(launch_dm_coef(h->partials, 0, h->p.c_smag * delta_sq(h), 0, report, dg, h->st));
         return SAM_OK;
     }
-    RCHK(launch_dm_fields<T>(d, P, (const T *)b.smag, (const T *)b.S6, (T *)b.MS, (T *)b.UU, h->st));
-    RCHK(launch_filter<T>(d, P, (T *)b.UH, (T *)b.tmp, 3, h->st));
-    RCHK(launch_filter<T>(d, (const T *)b.S6, (T *)b.SH, (T *)b.tmp, 6, h->st));
-    RCHK(launch_filter<T>(d, (const T *)b.MS, (T *)b.MH, (T *)b.tmp, 6, h->st));
-    RCHK(launch_filter<T>(d, (const T *)b.UU, (T *)b.LH, (T *)b.tmp, 6, h->st));
+    const double dxi = d.nx / h->p.xl, dyi = d.ny / h->p.yl, dzi = d.nz / h->p.zl;   // 1/dx (init.f:22-27)
     int nb = 0;
-    RCHK(launch_dm_reduce<T>(d, (const T *)b.UH, (const T *)b.SH, (const T *)b.MH, (const T *)b.LH, h->partials,
-                             h->st, &nb));
+    RCHK(launch_dm_fused<T>(d, P, dxi, dyi, dzi, h->partials, h->st, &nb));
     RCHK(launch_dm_coef(h->partials, nb, 0.0, 1, report, dg, h->st));
     return SAM_OK;
 }
@@ -554,8 +525,7 @@ template <typename T> static int les_fluxes(sam_handle *h, int stats)
     const int pi = h->prec;
     if (h->p.sgs == SAM_SGS_MLP && !h->mlp) return set_err(SAM_ERR_STATE, "sgs = MLP but no network attached");
     const bool mlp_dyn = h->p.sgs == SAM_SGS_MLP && h->mlp && mlp_unit_coefficient(h->mlp);
-    LesBufs b;
-    int rc = les_bufs<T>(h, b, h->p.sgs == SAM_SGS_DYN || mlp_dyn);
+    int rc = les_bufs<T>(h);
     if (rc) return rc;
     T *P = (T *)h->Pb.p, *Q = P + d.pfield * d.lu;
     T *Y = (T *)h->Yb;
@@ -575,7 +545,7 @@ template <typename T> static int les_fluxes(sam_handle *h, int stats)
             // unit-coefficient network: the dynamic procedure supplies C Delta^2
             // at the first substep of every mlp_refresh-th step
             if (mlp_dyn && stats && ++h->mlp_since >= h->mlp_refresh) {
-                rc = sgs_coefficient<T>(h, P, b, h->diag, stats, true);
+                rc = sgs_coefficient<T>(h, P, h->diag, stats, true);
                 if (rc) return rc;
                 h->mlp_since = 0;
             }
@@ -590,15 +560,12 @@ template <typename T> static int les_fluxes(sam_handle *h, int stats)
                 procedure = stats && ++h->dyn_since >= h->dyn_refresh;
                 if (procedure) h->dyn_since = 0;
             }
-            if (dyn && !procedure) {
-                RCHK(launch_strain_smag<T>(d, P, (T *)b.smag, nullptr, d.nx / h->p.xl, d.ny / h->p.yl, d.nz / h->p.zl,
-                                           h->st));
-            } else {
-                rc = sgs_coefficient<T>(h, P, b, h->diag, stats, dyn);
+            if (!dyn || procedure) {
+                rc = sgs_coefficient<T>(h, P, h->diag, stats, dyn);
                 if (rc) return rc;
             }
-            RCHK(launch_sgs_flux<T>(d, P, (const T *)b.smag, h->diag, Q, d.nx / h->p.xl, d.ny / h->p.yl,
-                                    d.nz / h->p.zl, 1.0 / h->p.pr_t, h->st));
+            RCHK(launch_sgs_flux<T>(d, P, h->diag, Q, d.nx / h->p.xl, d.ny / h->p.yl, d.nz / h->p.zl,
+                                    1.0 / h->p.pr_t, h->st));
         }
     }
     {
@@ -1237,18 +1204,18 @@ extern "C" int sam_project(sam_handle *h, const double *F, const double *usav, d
 template <typename T> static int eddy_stage(sam_handle *h, const double *phys, double *vt, double *cdsq)
 {
     const Dims &d = h->d;
-    LesBufs b;
-    int rc = les_bufs<T>(h, b, h->p.sgs == SAM_SGS_DYN);
+    int rc = les_bufs<T>(h);
+    if (!rc) rc = ensure(h->sgsw, sizeof(T) * d.pfield);
+    if (!rc) rc = ensure(h->stage0, sizeof(double) * d.pfield);
     if (rc) return rc;
-    T *P = (T *)h->Pb.p;
+    T *P = (T *)h->Pb.p, *smag = (T *)h->sgsw.p;
     rc = upload_T<T>(h, P, phys, d.pfield * d.lu);
     if (rc) return rc;
     DevDiag *sd = h->diag + 1;
-    rc = sgs_coefficient<T>(h, P, b, sd, 1, h->p.sgs == SAM_SGS_DYN);
+    rc = sgs_coefficient<T>(h, P, sd, 1, h->p.sgs == SAM_SGS_DYN);
     if (rc) return rc;
-    rc = ensure(h->stage0, sizeof(double) * d.pfield);
-    if (rc) return rc;
-    RCHK(launch_nut<T>(d, (const T *)b.smag, sd, (double *)h->stage0.p, h->st));
+    RCHK(launch_strain_smag<T>(d, P, smag, nullptr, d.nx / h->p.xl, d.ny / h->p.yl, d.nz / h->p.zl, h->st));
+    RCHK(launch_nut<T>(d, (const T *)smag, sd, (double *)h->stage0.p, h->st));
     HIPCHK(hipMemcpyAsync(vt, h->stage0.p, sizeof(double) * d.pfield, hipMemcpyDeviceToHost, h->st));
     DevDiag dg;
     HIPCHK(hipMemcpyAsync(&dg, sd, sizeof dg, hipMemcpyDeviceToHost, h->st));

@@ -114,13 +114,14 @@ int launch_div_check(const Dims &d, const double *U, double *partials, double *o
 template <typename T> int launch_strain_smag(const Dims &d, const T *P, T *smag, T *S6, double dxi, double dyi,
                                              double dzi, hipStream_t st);
 template <typename T> int launch_filter(const Dims &d, const T *in, T *out, T *tmp, int nvar, hipStream_t st);
-template <typename T> int launch_dm_fields(const Dims &d, const T *P, const T *smag, const T *S6, T *MS, T *UU,
-                                           hipStream_t st);
-template <typename T> int launch_dm_reduce(const Dims &d, const T *uh, const T *sh, const T *mh, const T *lh,
-                                           double *partials, hipStream_t st, int *nblocks_out);
+// dm.f's dynamic procedure in one pass: block sums of L:M and M:M (2 doubles
+// per block, dm_fused_rows(d) blocks) for launch_dm_coef
+template <typename T> int launch_dm_fused(const Dims &d, const T *P, double dxi, double dyi, double dzi,
+                                          double *partials, hipStream_t st, int *nblocks_out);
+long long dm_fused_rows(const Dims &d);
 int launch_dm_coef(const double *partials, int nblocks, double c_fixed, int dynamic, int report, DevDiag *diag,
                    hipStream_t st);
-template <typename T> int launch_sgs_flux(const Dims &d, const T *P, const T *smag, const DevDiag *diag, T *Q,
+template <typename T> int launch_sgs_flux(const Dims &d, const T *P, const DevDiag *diag, T *Q,
                                           double dxi, double dyi, double dzi, double pr_t_inv, hipStream_t st);
 template <typename T> int launch_nut(const Dims &d, const T *smag, const DevDiag *diag, double *vt, hipStream_t st);
 

@@ -1,10 +1,13 @@
 // sgs.hip -- subgrid-scale closures of the LES step on the collocated grid.
 //
 //   k_strain      |S| and S_ij by centred differences (sm.f:52-75, dm.f:80-110)
-//   k_dm_fields   |S| S_ij and u_i u_j, the fields dm.f filters (dm.f:101-110)
+//   k_dm_fused    the dynamic procedure in one pass over u, v, w: strain,
+//                 |S| S_ij, u_i u_j, the (1,4,1)/6 test filter of those and
+//                 of u and S, L_ij, M_ij and the block sums of L:M, M:M
+//                 (dm.f:60-149, filter.f:9-42)
 //   k_filter_xy / k_filter_z
-//                 periodic (1,4,1)/6 test filter, separable (filter.f:9-42)
-//   k_dm_reduce   L_ij, M_ij and the volume sums <L:M>, <M:M> (dm.f:113-147)
+//                 periodic (1,4,1)/6 test filter, separable (filter.f:9-42),
+//                 for the filter's own entry point
 //   k_dm_coef     C Delta^2 = -0.5 <L:M>/<M:M> (dm.f:176), or c_smag Delta^2 (sm.f:32)
 //   k_sgs_flux    F_ij = u_i u_j - 2 nu_t S_ij, F_Tj = u_j T - nu_t/Pr_t dT/dx_j
 //
@@ -52,6 +55,12 @@ __device__ __forceinline__ void strain_c(const T *__restrict__ P, const Grid3 &g
     s[5] = 0.25 * (wx + uz);
 }
 
+template <typename C> __device__ __forceinline__ C smag6c(const C s[6])
+{
+    const C sum = s[0] * s[0] + s[1] * s[1] + s[2] * s[2] + (C)2 * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5]);
+    return sqrt((C)2 * sum);
+}
+
 __device__ __forceinline__ double smag6(const double s[6])
 {
     const double sum = s[0] * s[0] + s[1] * s[1] + s[2] * s[2] + 2.0 * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5]);
@@ -82,34 +91,6 @@ int launch_strain_smag(const Dims &d, const T *P, T *smag, T *S6, double dxi, do
     Grid3 g{d.nx, d.ny, d.nz, d.pfield};
     hipLaunchKernelGGL((k_strain<T>), dim3((unsigned)((g.np + 255) / 256)), dim3(256), 0, st, P, smag, S6, g, dxi, dyi,
                        dzi);
-    return launched_sgs();
-}
-
-// |S| S_n (6) and u_i u_j (6, order 11,22,33,12,23,13)
-template <typename T>
-__global__ __launch_bounds__(256) void k_dm_fields(const T *__restrict__ P, const T *__restrict__ smag,
-                                                    const T *__restrict__ S6, T *__restrict__ MS, T *__restrict__ UU,
-                                                    long long np)
-{
-    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (i >= np) return;
-    const double sm = (double)smag[i];
-    const double u = (double)P[i], v = (double)P[np + i], w = (double)P[2 * np + i];
-#pragma unroll
-    for (int n = 0; n < 6; n++) MS[n * np + i] = (T)(sm * (double)S6[n * np + i]);
-    UU[i] = (T)(u * u);
-    UU[np + i] = (T)(v * v);
-    UU[2 * np + i] = (T)(w * w);
-    UU[3 * np + i] = (T)(u * v);
-    UU[4 * np + i] = (T)(v * w);
-    UU[5 * np + i] = (T)(u * w);
-}
-
-template <typename T>
-int launch_dm_fields(const Dims &d, const T *P, const T *smag, const T *S6, T *MS, T *UU, hipStream_t st)
-{
-    hipLaunchKernelGGL((k_dm_fields<T>), dim3((unsigned)((d.pfield + 255) / 256)), dim3(256), 0, st, P, smag, S6, MS,
-                       UU, d.pfield);
     return launched_sgs();
 }
 
@@ -159,59 +140,233 @@ template <typename T> int launch_filter(const Dims &d, const T *in, T *out, T *t
     return launched_sgs();
 }
 
-// L:M and M:M sums.  uh: filtered u (3), sh: filtered S (6),
-// mh: filtered |S|S (6), lh: filtered u_i u_j (6); alpha^2 = 4.
-template <typename T>
-__global__ __launch_bounds__(256) void k_dm_reduce(const T *__restrict__ uh, const T *__restrict__ sh,
-                                                    const T *__restrict__ mh, const T *__restrict__ lh, long long np,
-                                                    long long chunk, double *__restrict__ partials)
+// ---------------------------------------------------------------------
+// k_dm_fused: dm.f's dynamic procedure (dm.f:60-149) as one stencil pass.
+//
+// A block owns an output tile of DMX x DMY points in x, y and ZC planes in
+// z, and marches z.  Per plane p:
+//   us  (LDS) u, v, w of planes p-1, p, p+1 on the tile with a 2-point halo,
+//       a ring of three planes; plane p+2 is loaded into registers while
+//       plane p is worked on and stored into the freed slot of p-1;
+//   A   threads (row, x segment of 2 outputs) form the 21 fields dm.f
+//       filters -- u_i (3), S_n (6), |S| S_n (6), u_i u_j (6), order
+//       11,22,33,12,23,13 -- at the segment's 4 points (with the x halo) and
+//       their (1,4,1) sums along x in registers, into xf (LDS), on the tile's
+//       rows with a 1-row halo;
+//   B   thread (x, y) forms the (1,4,1) sums along y, keeps its column's last
+//       two x-y-filtered planes in registers, and the sum of three planes in
+//       z completes the filter of plane p-1.  Then L_n = (u_i u_j)^ - u^_i u^_j,
+//       M_n = -(|S|S)^_n + 4 |S^| S^_n and the sums of ifact_n L_n M_n and
+//       ifact_n M_n M_n; the block's sums go to partials (2 doubles).
+// u, v, w are read from HBM once per block (plus the halo) and only the sums
+// are written: the procedure's ~150 field passes per substep become about
+// four.  The stencil and filter arithmetic runs in the step's precision T
+// (FP64, or FP32 in the split mode); the volume sums are FP64.
+// ---------------------------------------------------------------------
+constexpr int DMY = 8, DMF = 21;
+
+template <typename T, int DMX>
+__global__ __launch_bounds__(DMX * DMY) void k_dm_fused(const T *__restrict__ P, Grid3 g, int zc, double dxi, double dyi,
+                                                         double dzi, double *__restrict__ partials)
 {
-    __shared__ double red[2][4];
-    const long long b0 = (long long)blockIdx.x * chunk;
-    const long long b1 = b0 + chunk < np ? b0 + chunk : np;
+    using C = T;
+    constexpr int NT = DMX * DMY;                 // stage B: one thread per output column
+    constexpr int UX = DMX + 4, UY = DMY + 4;     // u region: 2-point halo
+    constexpr int PY = DMY + 2;                   // x-filtered rows: 1-row halo
+    constexpr int NSEG = DMX / 2, NA = NSEG * PY; // stage A items: (row, 2-point x segment)
+    static_assert(NA <= NT, "one stage-A item per thread");
+    constexpr int NU = 3 * UX * UY;               // u values of one plane
+    constexpr int NPRE = (NU + NT - 1) / NT;      // per thread
+    __shared__ T us[3][3][UY][UX];
+    __shared__ T xf[DMF][PY][DMX];
+    __shared__ double red[2][NT / 64];
+    const int tid = threadIdx.x;
+    const int x0 = blockIdx.x * DMX, y0 = blockIdx.y * DMY, z0 = blockIdx.z * zc;
+    const int tx = tid % DMX, ty = tid / DMX;     // stage B column
+    const int arow = tid / NSEG, aseg = tid % NSEG;   // stage A item
+    const C cdx = (C)dxi, cdy = (C)dyi, cdz = (C)dzi;
+
+    // one u plane (global z = zz, periodic) into registers / into a ring slot
+    auto fetch = [&](int zz, T (&v)[NPRE]) {
+#pragma unroll
+        for (int k = 0; k < NPRE; k++) {
+            const int e = tid + k * NT;
+            T val = (T)0;
+            if (e < NU) {
+                const int f = e / (UX * UY), r = e % (UX * UY), iy = r / UX, ix = r % UX;
+                val = P[f * g.np + g.at(x0 - 2 + ix, y0 - 2 + iy, zz)];
+            }
+            v[k] = val;
+        }
+    };
+    auto store = [&](int slot, const T (&v)[NPRE]) {
+#pragma unroll
+        for (int k = 0; k < NPRE; k++) {
+            const int e = tid + k * NT;
+            if (e < NU) {
+                const int f = e / (UX * UY), r = e % (UX * UY);
+                us[slot][f][r / UX][r % UX] = v[k];
+            }
+        }
+    };
+    {
+        T v[NPRE];
+        for (int q = 0; q < 3; q++) {               // planes z0-2, z0-1, z0 -> slots 0, 1, 2
+            fetch(z0 - 2 + q, v);
+            store(q, v);
+        }
+    }
+    __syncthreads();
+
+    T m2[DMF], m1[DMF];                             // x-y filtered planes p-2, p-1 of this column
+#pragma unroll
+    for (int f = 0; f < DMF; f++) m2[f] = m1[f] = (T)0;
     double num = 0.0, den = 0.0;
     const int ip[6] = {0, 1, 2, 0, 1, 0}, jp[6] = {0, 1, 2, 1, 2, 2};
-    const double ifact[6] = {1, 1, 1, 2, 2, 2};
-    for (long long i = b0 + threadIdx.x; i < b1; i += 256) {
-        double s[6], u[3];
+    for (int p = z0 - 1; p <= z0 + zc; p++) {
+        // ring slots of planes p-1, p, p+1
+        const int sm1 = (p - 1 - (z0 - 2)) % 3, s0 = (p - (z0 - 2)) % 3, sp1 = (p + 1 - (z0 - 2)) % 3;
+        T nxt[NPRE];
+        const bool more = p + 2 <= z0 + zc + 1;
+        if (more) fetch(p + 2, nxt);
+
+        // stage A: the segment's 4 product points are tile x = 2 aseg - 1 .. 2 aseg + 2
+        // (u columns ux = 2 aseg + 1 + j), on row arow - 1 of the tile (uy = arow + 1)
+        if (tid < NA) {
+            const int uy = arow + 1, ub = 2 * aseg;
+            C a[4][3], sn[4][6], smg[4];
 #pragma unroll
-        for (int n = 0; n < 6; n++) s[n] = (double)sh[n * np + i];
+            for (int f = 0; f < 3; f++) {
+                C c[6], yu[4], yd[4], zu[4], zd[4];
 #pragma unroll
-        for (int n = 0; n < 3; n++) u[n] = (double)uh[n * np + i];
-        const double sm = smag6(s);
+                for (int k = 0; k < 6; k++) c[k] = us[s0][f][uy][ub + k];
 #pragma unroll
-        for (int n = 0; n < 6; n++) {
-            const double M = -(double)mh[n * np + i] + 4.0 * sm * s[n];
-            const double L = (double)lh[n * np + i] - u[ip[n]] * u[jp[n]];
-            num += ifact[n] * L * M;
-            den += ifact[n] * M * M;
+                for (int j = 0; j < 4; j++) {
+                    yu[j] = us[s0][f][uy + 1][ub + 1 + j];
+                    yd[j] = us[s0][f][uy - 1][ub + 1 + j];
+                    zu[j] = us[sp1][f][uy][ub + 1 + j];
+                    zd[j] = us[sm1][f][uy][ub + 1 + j];
+                }
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    a[j][f] = c[j + 1];
+                    const C ddx = (c[j + 2] - c[j]) * cdx, ddy = (yu[j] - yd[j]) * cdy, ddz = (zu[j] - zd[j]) * cdz;
+                    // S_n from du_f/dx_k: f = 0: 11 12 13, f = 1: 12 22 23, f = 2: 13 23 33
+                    if (f == 0) { sn[j][0] = (C)0.5 * ddx; sn[j][3] = ddy; sn[j][5] = ddz; }
+                    if (f == 1) { sn[j][3] = (C)0.25 * (sn[j][3] + ddx); sn[j][1] = (C)0.5 * ddy; sn[j][4] = ddz; }
+                    if (f == 2) { sn[j][5] = (C)0.25 * (ddx + sn[j][5]); sn[j][4] = (C)0.25 * (sn[j][4] + ddy); sn[j][2] = (C)0.5 * ddz; }
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < 4; j++) smg[j] = smag6c(sn[j]);
+            const int xo = 2 * aseg;
+            auto put = [&](int f, const C (&v)[4]) {
+                xf[f][arow][xo] = v[0] + (C)4 * v[1] + v[2];
+                xf[f][arow][xo + 1] = v[1] + (C)4 * v[2] + v[3];
+            };
+#pragma unroll
+            for (int f = 0; f < 3; f++) {
+                const C v[4] = {a[0][f], a[1][f], a[2][f], a[3][f]};
+                put(f, v);
+            }
+#pragma unroll
+            for (int n = 0; n < 6; n++) {
+                const C v1[4] = {sn[0][n], sn[1][n], sn[2][n], sn[3][n]};
+                put(3 + n, v1);
+                const C v2[4] = {smg[0] * sn[0][n], smg[1] * sn[1][n], smg[2] * sn[2][n], smg[3] * sn[3][n]};
+                put(9 + n, v2);
+                const C v3[4] = {a[0][ip[n]] * a[0][jp[n]], a[1][ip[n]] * a[1][jp[n]], a[2][ip[n]] * a[2][jp[n]],
+                                 a[3][ip[n]] * a[3][jp[n]]};
+                put(15 + n, v3);
+            }
         }
+        __syncthreads();
+        // the ring slot of plane p-1 is free: plane p+2 goes there
+        if (more) store(sm1, nxt);
+        // stage B: y sums -> the x-y filtered plane p of this thread's column
+        T cur[DMF];
+#pragma unroll
+        for (int f = 0; f < DMF; f++) {
+            const C acc = xf[f][ty][tx] + (C)4 * xf[f][ty + 1][tx] + xf[f][ty + 2][tx];
+            cur[f] = acc * (C)(1.0 / 36.0);
+        }
+        if (p >= z0 + 1) {
+            // plane p - 1 is filtered: z sum of planes p-2, p-1, p
+            C fl[DMF];
+#pragma unroll
+            for (int f = 0; f < DMF; f++) fl[f] = (m2[f] + (C)4 * m1[f] + cur[f]) * (C)(1.0 / 6.0);
+            const C smh = smag6c(fl + 3);
+            C pn = 0, pd = 0;
+#pragma unroll
+            for (int n = 0; n < 6; n++) {
+                const C M = -fl[9 + n] + (C)4 * smh * fl[3 + n];
+                const C L = fl[15 + n] - fl[ip[n]] * fl[jp[n]];
+                const C w = n < 3 ? (C)1 : (C)2;
+                pn += w * L * M;
+                pd += w * M * M;
+            }
+            num += (double)pn;     // the volume sums in FP64 (dm.f:145-147)
+            den += (double)pd;
+        }
+#pragma unroll
+        for (int f = 0; f < DMF; f++) {
+            m2[f] = m1[f];
+            m1[f] = cur[f];
+        }
+        __syncthreads();           // xf is rewritten and the ring read by the next plane
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
         num += __shfl_xor(num, o, 64);
         den += __shfl_xor(den, o, 64);
     }
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) {
-        red[0][w] = num;
-        red[1][w] = den;
+    if ((tid & 63) == 0) {
+        red[0][tid >> 6] = num;
+        red[1][tid >> 6] = den;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        partials[2 * (long long)blockIdx.x] = red[0][0] + red[0][1] + red[0][2] + red[0][3];
-        partials[2 * (long long)blockIdx.x + 1] = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+    if (tid == 0) {
+        double a = 0.0, b = 0.0;
+        for (int w = 0; w < NT / 64; w++) {
+            a += red[0][w];
+            b += red[1][w];
+        }
+        const long long blk = ((long long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+        partials[2 * blk] = a;
+        partials[2 * blk + 1] = b;
     }
 }
 
-template <typename T>
-int launch_dm_reduce(const Dims &d, const T *uh, const T *sh, const T *mh, const T *lh, double *partials,
-                     hipStream_t st, int *nblocks_out)
+// planes per block: z chunks until the grid has 8192 blocks (about 32 per
+// CU), at least 8 planes (each chunk recomputes 2 product planes)
+static int dm_planes(const Dims &d, int dmx)
 {
-    const long long chunk = 4096;
-    const unsigned nb = (unsigned)((d.pfield + chunk - 1) / chunk);
-    *nblocks_out = (int)nb;
-    hipLaunchKernelGGL((k_dm_reduce<T>), dim3(nb), dim3(256), 0, st, uh, sh, mh, lh, d.pfield, chunk, partials);
+    const long long cols = (long long)(d.nx / dmx) * (d.ny / DMY);
+    int zc = d.nz;
+    while (zc > 8 && cols * (d.nz / zc) < 8192) zc >>= 1;
+    return zc;
+}
+
+long long dm_fused_rows(const Dims &d)
+{
+    const int dmx = d.nx >= 32 ? 32 : 16;
+    return (long long)(d.nx / dmx) * (d.ny / DMY) * (d.nz / dm_planes(d, dmx));
+}
+
+template <typename T>
+int launch_dm_fused(const Dims &d, const T *P, double dxi, double dyi, double dzi, double *partials, hipStream_t st,
+                    int *nblocks_out)
+{
+    Grid3 g{d.nx, d.ny, d.nz, d.pfield};
+    const int dmx = d.nx >= 32 ? 32 : 16;
+    if (d.ny % DMY) return SAM_ERR_UNSUPPORTED;
+    const int zc = dm_planes(d, dmx);
+    const dim3 grid((unsigned)(d.nx / dmx), (unsigned)(d.ny / DMY), (unsigned)(d.nz / zc));
+    *nblocks_out = (int)(grid.x * grid.y * grid.z);
+    if (dmx == 32)
+        hipLaunchKernelGGL((k_dm_fused<T, 32>), grid, dim3(32 * DMY), 0, st, P, g, zc, dxi, dyi, dzi, partials);
+    else
+        hipLaunchKernelGGL((k_dm_fused<T, 16>), grid, dim3(16 * DMY), 0, st, P, g, zc, dxi, dyi, dzi, partials);
     return launched_sgs();
 }
 
@@ -281,10 +436,9 @@ __global__ __launch_bounds__(256) void k_sgs_flux(const T *__restrict__ P, const
 }
 
 template <typename T>
-int launch_sgs_flux(const Dims &d, const T *P, const T *smag, const DevDiag *diag, T *Q, double dxi, double dyi,
-                    double dzi, double pr_t_inv, hipStream_t st)
+int launch_sgs_flux(const Dims &d, const T *P, const DevDiag *diag, T *Q, double dxi, double dyi, double dzi,
+                    double pr_t_inv, hipStream_t st)
 {
-    (void)smag;
     Grid3 g{d.nx, d.ny, d.nz, d.pfield};
     hipLaunchKernelGGL((k_sgs_flux<T>), dim3((unsigned)((g.np + 255) / 256)), dim3(256), 0, st, P, diag, Q, g, dxi,
                        dyi, dzi, pr_t_inv, d.lu);
@@ -309,10 +463,8 @@ template <typename T> int launch_nut(const Dims &d, const T *smag, const DevDiag
 #define SGS_INST(T)                                                                                                   \
     template int launch_strain_smag<T>(const Dims &, const T *, T *, T *, double, double, double, hipStream_t);       \
     template int launch_filter<T>(const Dims &, const T *, T *, T *, int, hipStream_t);                               \
-    template int launch_dm_fields<T>(const Dims &, const T *, const T *, const T *, T *, T *, hipStream_t);           \
-    template int launch_dm_reduce<T>(const Dims &, const T *, const T *, const T *, const T *, double *, hipStream_t, \
-                                     int *);                                                                          \
-    template int launch_sgs_flux<T>(const Dims &, const T *, const T *, const DevDiag *, T *, double, double, double, \
+    template int launch_dm_fused<T>(const Dims &, const T *, double, double, double, double *, hipStream_t, int *); \
+    template int launch_sgs_flux<T>(const Dims &, const T *, const DevDiag *, T *, double, double, double,           \
                                     double, hipStream_t);                                                 \
     template int launch_nut<T>(const Dims &, const T *, const DevDiag *, double *, hipStream_t);
 SGS_INST(double)

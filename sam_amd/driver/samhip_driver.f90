@@ -23,6 +23,10 @@
 !                0 ReLU (default), 1 GELU (tanh form)
 !   cl_couple    i_les 1: 0 (default) the Chollet-Lesieur nu_t(k) enters the
 !                spectra only, as in SAM; 1 also the integrating factor
+!   vel_midstep  vel.NNNNNN written for nt < nt_end: 0 (default) the state at
+!                the start of step nt; 1 what SAM writes there, the velocity
+!                of the step's last substep after its zeta update
+!                (sam.f:210-221,277-284)
 ! n_skip_p: planes every n_skip_p steps (write_planes.f): time.out, mean.out,
 !           xy[m].dat, xz[m].dat, yz[m].dat at k_xy_plane[m], j_xz_plane[m],
 !           i_yz_plane[m] (first plane of each kind N/2 when not given)
@@ -37,7 +41,7 @@ program samhip_driver
   use samhip_mod
   implicit none
 
-  integer, parameter :: NIN = 72, NDYN = 6, NPAR = NIN + NDYN, NEXT = 7
+  integer, parameter :: NIN = 72, NDYN = 6, NPAR = NIN + NDYN, NEXT = 8
   character(len=12) :: labels(NPAR + NEXT)
   real(8) :: values(NPAR + NEXT), defaults(NPAR + NEXT)
   logical :: required(NPAR + NEXT), fixed(NPAR + NEXT), found(NPAR + NEXT)
@@ -159,7 +163,12 @@ program samhip_driver
      if (mod(nt, n_skip_h) == 0 .or. nt == nt_end) want = 1
      if ((nt > 0 .and. mod(nt, n_skip_v) == 0) .or. nt == nt_end) then
         write (ext, '(i6.6)') nt
-        rc = sam_get_state(h, u, int(d%nz_min, c_int))
+        if (nt < nt_end .and. iv('vel_midstep') == 1) then
+           rc = sam_get_state_substep(h, int(p%nrk_max, c_int), u, int(d%nz_min, c_int))
+        else
+           rc = sam_get_state(h, u, int(d%nz_min, c_int))
+        end if
+        if (rc /= SAM_OK) call fail(int(rc), 'vel.'//ext//': '//trim(sam_error_message()))
         call write_header('header.'//ext, time, dt, t_stat, nt)
         call write_field('vel.'//ext)
      end if
@@ -287,9 +296,9 @@ contains
     end do
     labels(73:78) = [character(len=12) :: 'time', 'dt', 't_stat', 'nt', 'n_frame_p', 'n_hist']
     defaults(73:78) = 0d0; required(73:78) = .true.; fixed(73:78) = .false.
-    labels(79:85) = [character(len=12) :: 'i_prec', 'i_lapse', 'pr_t', 'mlp_labels', 'mlp_refresh', 'cl_couple', &
-                     'mlp_act']
-    defaults(79:85) = [0d0, 0d0, 1d0, 1d0, 20d0, 0d0, 0d0]; required(79:85) = .false.; fixed(79:85) = .false.
+    labels(79:86) = [character(len=12) :: 'i_prec', 'i_lapse', 'pr_t', 'mlp_labels', 'mlp_refresh', 'cl_couple', &
+                     'mlp_act', 'vel_midstep']
+    defaults(79:86) = [0d0, 0d0, 1d0, 1d0, 20d0, 0d0, 0d0, 0d0]; required(79:86) = .false.; fixed(79:86) = .false.
   end subroutine set_labels
 
   ! read_line.f:24-89: one "label value [comment]" per line; '#' in column 1

@@ -65,6 +65,78 @@ module samhip_mod
        integer(c_int), value :: ldz
      end function sam_get_state
 
+     ! the velocity SAM writes inside a step (after substep nrk's zeta update)
+     integer(c_int) function sam_get_state_substep(h, nrk, u, ldz) bind(c, name='sam_get_state_substep')
+       import :: c_ptr, c_int, c_double_complex
+       type(c_ptr), value :: h
+       integer(c_int), value :: nrk, ldz
+       complex(c_double_complex), intent(out) :: u(*)
+     end function sam_get_state_substep
+
+     ! stage entry points in the reference's layouts (include/samhip.h)
+     integer(c_int) function sam_z_trans_b(h, u, ldz, out, nvar) bind(c, name='sam_z_trans_b')
+       import :: c_ptr, c_int, c_double_complex
+       type(c_ptr), value :: h
+       integer(c_int), value :: ldz, nvar
+       complex(c_double_complex), intent(in) :: u(*)
+       complex(c_double_complex), intent(out) :: out(*)
+     end function sam_z_trans_b
+
+     integer(c_int) function sam_x2z(h, in, out, nvar) bind(c, name='sam_x2z')
+       import :: c_ptr, c_int, c_double_complex
+       type(c_ptr), value :: h
+       integer(c_int), value :: nvar
+       complex(c_double_complex), intent(in) :: in(*)
+       complex(c_double_complex), intent(out) :: out(*)
+     end function sam_x2z
+
+     integer(c_int) function sam_xy_trans_b(h, in, out, nvar) bind(c, name='sam_xy_trans_b')
+       import :: c_ptr, c_int, c_double, c_double_complex
+       type(c_ptr), value :: h
+       integer(c_int), value :: nvar
+       complex(c_double_complex), intent(in) :: in(*)
+       real(c_double), intent(out) :: out(*)
+     end function sam_xy_trans_b
+
+     integer(c_int) function sam_xy_trans_f(h, in, out, nvar) bind(c, name='sam_xy_trans_f')
+       import :: c_ptr, c_int, c_double, c_double_complex
+       type(c_ptr), value :: h
+       integer(c_int), value :: nvar
+       real(c_double), intent(in) :: in(*)
+       complex(c_double_complex), intent(out) :: out(*)
+     end function sam_xy_trans_f
+
+     integer(c_int) function sam_z2x(h, in, out, nvar) bind(c, name='sam_z2x')
+       import :: c_ptr, c_int, c_double_complex
+       type(c_ptr), value :: h
+       integer(c_int), value :: nvar
+       complex(c_double_complex), intent(in) :: in(*)
+       complex(c_double_complex), intent(out) :: out(*)
+     end function sam_z2x
+
+     integer(c_int) function sam_z_trans_f(h, in, out, nvar) bind(c, name='sam_z_trans_f')
+       import :: c_ptr, c_int, c_double_complex
+       type(c_ptr), value :: h
+       integer(c_int), value :: nvar
+       complex(c_double_complex), intent(in) :: in(*)
+       complex(c_double_complex), intent(out) :: out(*)
+     end function sam_z_trans_f
+
+     ! r_prev / r may be c_null_ptr: pass c_loc of the arrays otherwise
+     integer(c_int) function sam_rk_update(h, nrk, dt, u, r_prev, r, ldz) bind(c, name='sam_rk_update')
+       import :: c_ptr, c_int, c_double, c_double_complex
+       type(c_ptr), value :: h, r_prev, r
+       integer(c_int), value :: nrk, ldz
+       real(c_double), value :: dt
+       complex(c_double_complex), intent(inout) :: u(*)
+     end function sam_rk_update
+
+     integer(c_int) function sam_div_check(h, out) bind(c, name='sam_div_check')
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(out) :: out(3)
+     end function sam_div_check
+
      integer(c_int) function sam_set_clock(h, time, dt, nt) bind(c, name='sam_set_clock')
        import :: c_int, c_ptr, c_double, c_long_long
        type(c_ptr), value :: h

@@ -214,3 +214,27 @@ def test_planes_files(tmp_path):
     k = raw.index(b"Scalars u' float\nLOOKUP_TABLE default\n") + len(b"Scalars u' float\nLOOKUP_TABLE default\n")
     vals = np.frombuffer(raw[k:k + 4 * 33 * 33], dtype=">f4").reshape(33, 33)     # [y, x]
     assert np.allclose(vals[:32, :32], xy[2, 0], atol=1e-6)
+
+
+@pytest.mark.gpu
+@needs_driver
+def test_mid_step_vel_files_match_reference(tmp_path):
+    """vel_midstep 1: vel.NNNNNN at nt < nt_end holds what the reference
+    writes inside a step (sam.f:277-284); the reference's own run with
+    n_skip_v 2 (tests/golden/sam_runs.npz, midrk16) is the expected output"""
+    import json
+    g = np.load(os.path.join(ROOT, "tests", "golden", "sam_runs.npz"))
+    p = dict(json.loads(str(g["cases_json"]))["midrk16"])
+    for k in ("i_prob", "flct_u", "flct_t"):
+        p.pop(k)
+    p.update(i_restart=1, nt_restart=0, n_steps=4, n_skip_v=2, i_lapse=1, vel_midstep=1)
+    u0 = g["midrk16_u0"]
+    sio.write_vel(str(tmp_path / "vel.000000"), u0)
+    sio.write_header(str(tmp_path / "header.000000"), p, time=0.0, dt=0.01, nt=0)
+    r = run(tmp_path, **p)
+    assert r.returncode == 0, r.stdout + r.stderr
+    nzm, nym, nxm = u0.shape[:3]
+    for nt in (2, 4):
+        u = sio.read_vel(str(tmp_path / f"vel.{nt:06d}"), nzm, nym, nxm, 4)
+        ref = g[f"midrk16_u{nt}"]
+        assert np.linalg.norm(u - ref) / np.linalg.norm(ref) < 1e-12, nt
