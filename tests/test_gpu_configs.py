@@ -1,0 +1,128 @@
+"""GPU: the LES, training and large-grid workloads at their own sizes.
+
+- Stratified LES with the dynamic model (dm.f; k0 = 8, N = 8, nu = 1e-5):
+  one FP64 step at 256^3 against the oracle, and at 512^3 the FP32-split
+  run of 200 steps against an FP64 run over the first 10.
+- Closure training on box-filtered DNS: the stratified 512^3 DNS
+  (N = 4, k0 = 4, nu = 5e-4, FP32-split) filtered by 4 to 128^3, a GELU
+  network trained with minibatches of 2^20 points, and its gradient at a
+  full 2^20 batch against the CPU oracle (oracle/mlp_oracle.py).
+- One RK3 step of the 1024^3 FP32-split DNS against one FP64 step of the
+  same (spectrally padded) state: kinetic energy, dissipation, E(k).
+"""
+import numpy as np
+import pytest
+
+from oracle import mlp_oracle as M
+from sam_amd import ic
+from sam_amd import les as L
+from sam_amd import samhip as S
+
+pytestmark = pytest.mark.gpu
+
+LES3 = dict(i_strat=1, grav=64.0, to=1.0, vis=1e-5, pr=1.0, i_cfl=1, cfl0=0.5, dt0=0.01, pr_t=1.0)
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
+
+
+def test_dynamic_les_256_step_matches_oracle(oracle_mod):
+    """the dynamic-model LES workload (k0 = 8, N = 8, nu = 1e-5) at 256^3,
+    one FP64 step (three substeps of the fused dynamic procedure) against the
+    oracle's dm.f restatement: state at 1e-11, C Delta^2 at 1e-10"""
+    O = oracle_mod
+    n = 256
+    u0 = ic.random_solenoidal(n, k0=8.0, seed=20261203, i_strat=1)
+    o = O.Oracle(O.make_cfg(n, sgs=2, lapse0=1.0, **{k: v for k, v in LES3.items()}))
+    ref, t, dt, st = o.step(u0, 1)
+    s = S.Sam(S.make_params(n, sgs=S.SGS_DYN, lapse=1.0, **LES3))
+    s.set_state(u0)
+    d = s.step(1)
+    err = rel(s.get_state(), ref)
+    s.close()
+    assert err < 1e-11, err
+    assert abs(d["c_delta_sq"] - st[17]) <= 1e-10 * abs(st[17])
+
+
+def test_dynamic_les_512_split_200_steps():
+    """the 512^3 dynamic-model LES workload: FP32-split against FP64 over the
+    first 10 steps (kinetic energy within 1e-5), then 200 split steps: finite
+    state and coefficient, divergence-free"""
+    n = 512
+    u0 = ic.random_solenoidal(n, k0=8.0, seed=20261203, i_strat=1)
+    out = {}
+    for prec in (S.PREC_FP64, S.PREC_SPLIT):
+        s = S.Sam(S.make_params(n, sgs=S.SGS_DYN, lapse=1.0, precision=prec, **LES3))
+        s.set_state(u0)
+        d = s.step(10, want_spectra=True)
+        out[prec] = (s.spectra(), d)
+        if prec == S.PREC_SPLIT:
+            s.step(189)
+            d = s.step(1, want_spectra=True)
+            sp = s.spectra()
+            assert np.isfinite(sp["energy_w"]) and np.isfinite(d["c_delta_sq"]) and d["c_delta_sq"] != 0
+            assert sp["div_max"] < 1e-10, sp["div_max"]
+            assert np.all(np.isfinite(s.get_state()))
+            print(f"512^3 dynamic LES split: 200 steps, E = {sp['energy_w']:.6f}, C Delta^2 = {d['c_delta_sq']:.3e}")
+        s.close()
+    e64, e32 = out[S.PREC_FP64][0]["energy_w"], out[S.PREC_SPLIT][0]["energy_w"]
+    ke = abs(e32 - e64) / e64
+    print(f"512^3 dynamic LES, KE after 10 steps: split vs FP64 rel err {ke:.2e}")
+    assert ke < 1e-5, ke
+    c64, c32 = out[S.PREC_FP64][1]["c_delta_sq"], out[S.PREC_SPLIT][1]["c_delta_sq"]
+    assert abs(c32 - c64) < 1e-3 * abs(c64), (c32, c64)
+
+
+def test_training_512_to_128_full_batch():
+    """box-filtered 512^3 DNS -> 128^3 samples (filtered-DNS stress and flux
+    labels), a GELU network trained on 2^20-point minibatches with Adam
+    lr 1e-3; the gradient of a full 2^20-point batch equals the oracle's"""
+    n, r = 512, 4
+    kw = dict(i_strat=1, grav=16.0, to=1.0, lapse=1.0, vis=5e-4, pr=1.0, i_cfl=1, cfl0=0.5, dt0=0.01,
+              precision=S.PREC_SPLIT)
+    dns = S.Sam(S.make_params(n, **kw))
+    dns.set_state(ic.random_solenoidal(n, k0=4.0, seed=20261203, i_strat=1))
+    mlp = S.Mlp(seed=1, pool_capacity=2 * (n // r) ** 3)
+    mlp.set_activation(S.ACT_GELU)
+    dns.step(100)
+    L.collect_samples(dns, mlp, r, nsnap=2, steps_between=50)
+    dns.close()
+    assert mlp.pool_size() == 2 * (n // r) ** 3
+    losses = L.train(mlp, 100, batch=1 << 20, lr=1e-3, seed=7)
+    assert np.all(np.isfinite(losses))
+    assert losses[-10:].mean() < 0.8 * losses[:3].mean(), (losses[:3], losses[-10:])
+    nb = 1 << 20
+    f, y = mlp.get_samples(0, nb)
+    p = mlp.get_params()
+    loss, g = mlp.grad(f, y)
+    loss_ref, g_ref = M.Net(p, bf=True, act=M.GELU).loss_grad(f, y)
+    mlp.close()
+    print(f"2^20 batch: loss {loss:.6e} (oracle {loss_ref:.6e}), gradient rel err {rel(g, g_ref):.2e}")
+    assert abs(loss - loss_ref) < 1e-4 * loss_ref
+    assert rel(g, g_ref) < 5e-3, rel(g, g_ref)
+
+
+def test_split_1024_step_against_fp64():
+    """one RK3 step at 1024^3 (the 512^3 seeded state, spectrally padded) in
+    FP32-split and in FP64: kinetic energy, dissipation and E(k) within 1e-5,
+    both divergence-free"""
+    n = 1024
+    u = ic.spectral_pad(ic.random_solenoidal(512, k0=4.0, seed=20261203, i_strat=1), n)
+    kw = dict(i_strat=1, grav=16.0, to=1.0, lapse=1.0, vis=5e-4, pr=1.0, i_cfl=1, cfl0=0.5, dt0=0.01)
+    sp = {}
+    for prec in (S.PREC_FP64, S.PREC_SPLIT):
+        s = S.Sam(S.make_params(n, precision=prec, **kw))
+        s.set_state(u)
+        s.step(1)
+        s.rhs(want_spectra=True)          # spectra of the state after the step
+        sp[prec] = s.spectra()
+        s.close()
+    a, b = sp[S.PREC_SPLIT], sp[S.PREC_FP64]
+    ke = abs(a["energy_w"] - b["energy_w"]) / b["energy_w"]
+    diss = abs(a["dk"].sum() - b["dk"].sum()) / abs(b["dk"].sum())
+    ek = rel(a["ek"], b["ek"])
+    print(f"1024^3 split vs FP64 after one step: KE {ke:.2e}, dissipation {diss:.2e}, E(k) {ek:.2e}")
+    assert ke < 1e-5 and diss < 1e-5 and ek < 1e-5, (ke, diss, ek)
+    assert a["div_max"] < 1e-10 and b["div_max"] < 1e-10
