@@ -37,11 +37,14 @@ def grid_counts(n, i_strat=1):
     return dict(S=S, Z=Z, Y=Y, P=n ** 3, lu=lu, lr=lr)
 
 
-def kernel_bytes(n, bc, i_strat=1, zrhs_fused=False):
+def kernel_bytes(n, bc, i_strat=1, zrhs_fused=False, nrk=3):
     """Algorithmic HBM bytes per launch of each step kernel (fused 3-pass
     model): state/RHS in FP64 (16 B), transform elements bc = 16 or 8 B.
-    zrhs_fused: k_zrhs also writes the next substep's z inverse (Lu Z
-    fields), which k_zb then does not compute."""
+    zrhs: the mean over the nrk launches of a step -- every substep reads the
+    state and the fluxes and writes the state and the RHS, and all but the
+    first read the previous substep's RHS (the zeta term).  zrhs_fused:
+    k_zrhs also writes the next substep's z inverse (Lu Z fields), which k_zb
+    then does not compute."""
     g = grid_counts(n, i_strat)
     S, Z, Y, lu, lr = g["S"], g["Z"], g["Y"], g["lu"], g["lr"]
     return {
@@ -49,7 +52,7 @@ def kernel_bytes(n, bc, i_strat=1, zrhs_fused=False):
         "yb": lu * Z * bc + lu * Y * bc,
         "xrow": lu * Y * bc + lr * Y * bc,
         "yf": lr * Y * bc + lr * Z * bc,
-        "zrhs": lr * Z * bc + 4 * lu * S * 16 + (lu * Z * bc if zrhs_fused else 0),
+        "zrhs": lr * Z * bc + (3 + (nrk - 1) / nrk) * lu * S * 16 + (lu * Z * bc if zrhs_fused else 0),
     }
 
 
@@ -61,15 +64,82 @@ def zrhs_fused(prof):
 
 def bytes_moved(n, bc, prof, steps):
     """bytes per step the launched kernels move by the same model: the
-    contract's count (step_bytes) less the state reads a fused k_zrhs saves"""
+    contract's count (step_bytes) less the state reads a fused k_zrhs saves
+    and the RHS read the first substep does not make"""
     kb = kernel_bytes(n, bc, zrhs_fused=zrhs_fused(prof))
     return sum(kb[k] * prof.get(k, (0.0, 0))[1] for k in kb) / steps
 
 
 def step_bytes(n, bc, i_strat=1, nrk=3):
+    """the contract's model: per substep 5 Lu S 16 + 2 (Lu + Lr)(Z + Y) b_c
+    (it counts a read of the previous RHS in every substep)"""
     g = grid_counts(n, i_strat)
     per_sub = 5 * g["lu"] * g["S"] * 16 + 2 * (g["lu"] + g["lr"]) * (g["Z"] + g["Y"]) * bc
     return nrk * per_sub
+
+
+def step_bytes_algorithmic(n, bc, i_strat=1, nrk=3):
+    """the step's algorithmic bytes: the contract's less the RHS read of the
+    first substep, which the RK3 update does not make (zeta_1 = 0)"""
+    g = grid_counts(n, i_strat)
+    return step_bytes(n, bc, i_strat, nrk) - g["lu"] * g["S"] * 16
+
+
+def les_kernel_bytes(n, bc, bp, i_strat=1, nrk=3):
+    """Algorithmic HBM bytes per launch of the LES step's kernels (the stage
+    list of the step with an SGS model; physical fields of bp = 8 or 4 bytes
+    per value, P points per field):
+      zb, yb      as in the DNS step (k_zb runs every substep: no fused z inverse)
+      xb          Lu fields Y -> P                      (x inverse to physical)
+      stats       reads Lu P fields                     (vel_max, first substep)
+      dmproc      reads u, v, w once                    (fused dynamic procedure)
+      sgs         reads Lu P fields, writes Lr P fields (eddy-viscosity or
+                  network fluxes)
+      xf          Lr fields P -> Y                      (x forward)
+      yf          as in the DNS step
+      zrhs        as in the DNS step, without the fused z inverse"""
+    g = grid_counts(n, i_strat)
+    S, Z, Y, P, lu, lr = g["S"], g["Z"], g["Y"], g["P"], g["lu"], g["lr"]
+    kb = kernel_bytes(n, bc, i_strat, nrk=nrk)
+    return {
+        "zb": kb["zb"], "yb": kb["yb"],
+        "xb": lu * Y * bc + lu * P * bp,
+        "stats": lu * P * bp,
+        "dmproc": 3 * P * bp,
+        "sgs": (lu + lr) * P * bp,
+        "xf": lr * P * bp + lr * Y * bc,
+        "yf": kb["yf"], "zrhs": kb["zrhs"],
+    }
+
+
+def les_roofline(n, bc, bp, prof, steps):
+    """per-kernel achieved bandwidth of an LES run, the dominant kernel's
+    roofline and the step's algorithmic bytes (launch counts from the run)"""
+    kb = les_kernel_bytes(n, bc, bp)
+    per, tot_bytes = {}, 0.0
+    tot_ms = sum(v[0] for v in prof.values()) or 1.0
+    best = None
+    for k, nb in kb.items():
+        ms, cnt = prof.get(k, (0.0, 0))
+        if not cnt:
+            continue
+        tot_bytes += nb * cnt
+        avg = ms / cnt / 1e3
+        per[k] = dict(avg_us=round(avg * 1e6, 2), GBps=round(nb / avg / 1e9, 1), share=round(ms / tot_ms, 3),
+                      launches_per_step=round(cnt / steps, 2))
+        if best is None or ms > best[1]:
+            best = (k, ms, cnt, nb)
+    k, ms, cnt, nb = best
+    avg = ms / cnt / 1e3
+    roof = dict(kernel=k, bound="hbm", achieved=round(nb / avg / 1e9, 1), peak=HBM_PEAK / 1e9, unit="GB/s",
+                frac=round(nb / avg / HBM_PEAK, 4), bytes_per_launch=nb, avg_launch_us=round(avg * 1e6, 2),
+                launches=cnt)
+    if k == "dmproc":
+        # the fused dynamic procedure reads u, v, w once and spends its time on
+        # the 21-field stencil and filter in LDS and VALU: HBM is not its bound
+        roof["bound"] = "lds/valu"
+        roof["note"] = "achieved/frac: HBM bytes (u, v, w once) over the launch time, for reference"
+    return roof, per, tot_bytes / steps
 
 
 def make_params(S, n, precision, sgs=0):
@@ -140,60 +210,72 @@ def measured_traffic(n, kernel, prec):
 
 
 def les_comparison(S, ic, n_dns, n_les=128, les_steps=2000, refresh=20):
-    """Learned closure vs the dynamic model on a 128^3 stratified LES.
+    """The learned closure against the dynamic model on a 128^3 stratified LES.
 
-    Training: a 512^3 FP32-split DNS of the LES's physics (N = 8, seeded
-    k0 = 6 IC, nu = 5e-4) is box-filtered by 4 at four times; the network
-    learns the dynamic model's stress and flux at unit coefficient on those
-    filtered fields (SAM_LABELS_DYNAMIC), 1200 Adam steps of 2^20 points.
-    LES (nu = 1e-5, the DNS IC truncated to 128^3): `les_steps` steps with
-    the dynamic model, with the learned closure (C Delta^2 from the dynamic
-    procedure every `refresh` steps), with the dynamic model from an IC
-    perturbed by 1e-6 (the trajectories' chaotic spread, the floor of any
-    comparison), and with the dynamic model whose C Delta^2 is itself
-    refreshed every `refresh` steps (the control for that amortisation).
-    The learned closure runs twice: with ReLU hidden units (the default) and
-    with GELU (tanh form), each network trained on the same samples.
-    Whole-run wall time per step; E(k) compared at the end and time-averaged
-    over the second half (every 50 steps)."""
+    Training data: the stratified DNS of the closure-training workload at
+    n_dns^3 (N = 4, seeded k0 = 4 IC, nu = 5e-4, FP32-split), 100 steps to
+    develop, then 8 snapshots 50 steps apart box-filtered by n_dns/n_les.
+    Three networks learn from the same snapshots, 1200 Adam steps of 2^20
+    points each:
+      learned_gelu     GELU, the dynamic model's stress at unit coefficient;
+                       in the LES scaled by C Delta^2 from the dynamic
+                       procedure every `refresh` steps;
+      learned_relu     the same with ReLU hidden units;
+      learned_gelu_coef GELU, the dynamic model's stress with the coefficient
+                       of the dynamic procedure run on each filtered snapshot;
+                       the LES uses it with no procedure at all.
+    LES (N = 8, nu = 1e-5, the seeded k0 = 6 IC truncated to 128^3):
+    `les_steps` steps with the dynamic model (dm.f, the procedure every
+    substep), with each network, with the dynamic model from an IC perturbed
+    by 1e-6 (the trajectories' chaotic spread, the floor of any comparison),
+    and with the dynamic model whose C Delta^2 is refreshed every `refresh`
+    steps (the control for the refresh).  Whole-run wall time per step; E(k)
+    compared at the end and time-averaged over the second half (every 50
+    steps); the per-kernel split and rooflines of the dynamic-model run."""
     from sam_amd import les as L
     r = n_dns // n_les
-    phys_dns = dict(i_strat=1, grav=64.0, to=1.0, lapse=1.0, vis=5e-4, pr=1.0, i_cfl=1, cfl0=0.5, dt0=0.01,
+    phys_dns = dict(i_strat=1, grav=16.0, to=1.0, lapse=1.0, vis=5e-4, pr=1.0, i_cfl=1, cfl0=0.5, dt0=0.01,
                     precision=S.PREC_SPLIT)
-    phys = dict(phys_dns, vis=1e-5)
-    u0 = ic.random_solenoidal(n_dns, k0=6.0, seed=20261203, i_strat=1)
+    phys = dict(phys_dns, grav=64.0, vis=1e-5)
+    nets = {"learned_gelu": (S.ACT_GELU, S.LABELS_DYNAMIC), "learned_relu": (S.ACT_RELU, S.LABELS_DYNAMIC),
+            "learned_gelu_coef": (S.ACT_GELU, S.LABELS_DYNAMIC_COEF)}
+    nsnap = 8
+    mlps = {}
+    for name, (act, lab) in nets.items():
+        m = S.Mlp(seed=1, pool_capacity=nsnap * n_les ** 3)
+        m.set_activation(act)
+        m.set_labels(lab, 1.0)
+        mlps[name] = m
     dns = S.Sam(S.make_params(n_dns, **phys_dns))
-    dns.set_state(u0)
-    mlp = S.Mlp(seed=1, pool_capacity=4 * n_les ** 3)
-    mlp.set_labels(S.LABELS_DYNAMIC, 1.0)
-    mlp_gelu = S.Mlp(seed=1, pool_capacity=4 * n_les ** 3)
-    mlp_gelu.set_activation(S.ACT_GELU)
-    mlp_gelu.set_labels(S.LABELS_DYNAMIC, 1.0)
+    dns.set_state(ic.random_solenoidal(n_dns, k0=4.0, seed=20261203, i_strat=1))
     t0 = time.perf_counter()
-    for _ in range(4):
+    dns.step(100)
+    coefs = []
+    for _ in range(nsnap):
         dns.step(50)
-        mlp.add_samples(dns, r)
-        mlp_gelu.add_samples(dns, r)
+        for m in mlps.values():
+            m.add_samples(dns, r)
+        coefs.append(mlps["learned_gelu_coef"].last_coefficient())
     dns.close()
     t_dns = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    losses = L.train(mlp, 1200, batch=1 << 20, lr=1e-3, seed=7)
-    t_train = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    losses_gelu = L.train(mlp_gelu, 1200, batch=1 << 20, lr=1e-3, seed=7)
-    t_train_gelu = time.perf_counter() - t0
-    u_les = L.spectral_truncate(u0, n_les)
+    train = {}
+    for name, m in mlps.items():
+        t0 = time.perf_counter()
+        losses = L.train(m, 1200, batch=1 << 20, lr=1e-3, seed=7)
+        train[name] = dict(seconds=round(time.perf_counter() - t0, 2), loss_first=float(losses[0]),
+                           loss_last=float(losses[-1]))
+    u_les = L.spectral_truncate(ic.random_solenoidal(n_dns, k0=6.0, seed=20261203, i_strat=1), n_les)
     rng = np.random.default_rng(5)
-    res = {}
-    runs = (("dynamic", S.SGS_DYN, u_les), ("learned", S.SGS_MLP, u_les),
-            ("dynamic_perturbed", S.SGS_DYN, u_les * (1 + 1e-6 * rng.standard_normal(u_les.shape))),
-            ("dynamic_lagged", S.SGS_DYN, u_les), ("learned_gelu", S.SGS_MLP, u_les))
+    runs = [("dynamic", S.SGS_DYN, None, u_les),
+            ("dynamic_perturbed", S.SGS_DYN, None, u_les * (1 + 1e-6 * rng.standard_normal(u_les.shape))),
+            ("dynamic_lagged", S.SGS_DYN, None, u_les)] + [(name, S.SGS_MLP, name, u_les) for name in nets]
     half = max(1, les_steps // 2)
     chunk = min(50, max(1, half // 2))    # steps between E(k) samples of the second half
-    for name, model, u in runs:
+    res = {}
+    for name, model, net, u in runs:
         les = S.Sam(S.make_params(n_les, sgs=model, **phys))
-        if model == S.SGS_MLP:
-            S.attach_mlp(les, mlp_gelu if name == "learned_gelu" else mlp)
+        if net:
+            S.attach_mlp(les, mlps[net])
             S.set_mlp_refresh(les, refresh)
         if name == "dynamic_lagged":
             S.set_dyn_refresh(les, refresh)
@@ -208,60 +290,70 @@ def les_comparison(S, ic, n_dns, n_les=128, les_steps=2000, refresh=20):
             ns += 1
         les.sync()
         wall = time.perf_counter() - t0
+        nsteps = half + chunk * (half // chunk)
+        prof = None
+        if name == "dynamic":
+            # the kernel split from a separate run with HIP events (which add
+            # per-launch overhead the timed runs do not carry)
+            les.prof_enable(True)
+            les.step(200)
+            les.sync()
+            prof = les.prof_read()
+            les.prof_enable(False)
         check = None
         if model == S.SGS_MLP:
             err, sw = S.sgs_check(les, 1.0)
             check = {"sgs_transfer_rel_err_vs_dynamic": round(float(err), 4), "fell_back": bool(sw)}
-        res[name] = dict(ms=wall / (half + chunk * (half // chunk)) * 1e3, final=L.shell_energy(les.spectra()),
-                         avg=acc / ns, check=check)
+        res[name] = dict(ms=wall / nsteps * 1e3, final=L.shell_energy(les.spectra()), avg=acc / ns, check=check,
+                         prof=prof, steps=nsteps)
         les.close()
+    for m in mlps.values():
+        m.close()
 
     def err(a, b, lo=1):
         k1 = len(b) - 1
         return round(float(np.linalg.norm(a[lo:k1] - b[lo:k1]) / np.linalg.norm(b[lo:k1])), 4)
 
-    d, m, p, g = res["dynamic"], res["learned"], res["dynamic_perturbed"], res["dynamic_lagged"]
-    mg = res["learned_gelu"]
-    return {
+    def errs(x, ref):
+        return {"time_avg_second_half": err(x["avg"], ref["avg"]),
+                "time_avg_second_half_k_ge_8": err(x["avg"], ref["avg"], 8),
+                "final": err(x["final"], ref["final"]), "final_k_ge_8": err(x["final"], ref["final"], 8)}
+
+    d = res["dynamic"]
+    roof, per, les_bytes = les_roofline(n_les, 8, 4, d["prof"], 200)
+    out = {
         "workload": f"stratified LES {n_les}^3, N=8, nu=1e-5, IC: the seeded k0=6 DNS IC truncated, "
                     f"FP32 transforms / FP64 state, {les_steps} steps",
         "dynamic_ms_per_step": round(d["ms"], 3),
-        "learned_ms_per_step": round(m["ms"], 3),
-        "speedup_learned_vs_dynamic": round(d["ms"] / m["ms"], 3),
-        "learned_refresh_steps": refresh,
-        "ek_rel_err_learned_vs_dynamic": {
-            "time_avg_second_half": err(m["avg"], d["avg"]), "time_avg_second_half_k_ge_8": err(m["avg"], d["avg"], 8),
-            "final": err(m["final"], d["final"]), "final_k_ge_8": err(m["final"], d["final"], 8)},
-        "ek_rel_err_dynamic_perturbed_vs_dynamic": {
-            "time_avg_second_half": err(p["avg"], d["avg"]), "time_avg_second_half_k_ge_8": err(p["avg"], d["avg"], 8),
-            "final": err(p["final"], d["final"]), "final_k_ge_8": err(p["final"], d["final"], 8)},
-        "a_posteriori_check_at_end": m["check"],
-        # control: the dynamic model itself with C Delta^2 refreshed every
-        # `refresh` steps (the amortisation the learned closure uses)
-        "dynamic_lagged_ms_per_step": round(g["ms"], 3),
-        "ek_rel_err_dynamic_lagged_vs_dynamic": {
-            "time_avg_second_half": err(g["avg"], d["avg"]), "time_avg_second_half_k_ge_8": err(g["avg"], d["avg"], 8),
-            "final": err(g["final"], d["final"]), "final_k_ge_8": err(g["final"], d["final"], 8)},
-        # the same closure with GELU hidden units (tanh form)
-        "learned_gelu_ms_per_step": round(mg["ms"], 3),
-        "speedup_learned_gelu_vs_dynamic": round(d["ms"] / mg["ms"], 3),
-        "ek_rel_err_learned_gelu_vs_dynamic": {
-            "time_avg_second_half": err(mg["avg"], d["avg"]), "time_avg_second_half_k_ge_8": err(mg["avg"], d["avg"], 8),
-            "final": err(mg["final"], d["final"]), "final_k_ge_8": err(mg["final"], d["final"], 8)},
-        "a_posteriori_check_at_end_gelu": mg["check"],
-        "training": {"samples": mlp.pool_size(), "labels": "dynamic model at unit coefficient",
-                     "source": f"{n_dns}^3 DNS box-filtered by {r}, 4 snapshots 50 steps apart",
-                     "dns_seconds": round(t_dns, 2), "adam_steps": 1200, "batch": 1 << 20,
-                     "seconds": round(t_train, 2), "loss_first": float(losses[0]), "loss_last": float(losses[-1]),
-                     "gelu_seconds": round(t_train_gelu, 2), "gelu_loss_last": float(losses_gelu[-1])},
+        "dynamic_roofline": roof,
+        "dynamic_step_hbm_frac": round(les_bytes / (d["ms"] / 1e3) / HBM_PEAK, 4),
+        "dynamic_kernels": per,
+        "ek_rel_err_dynamic_perturbed_vs_dynamic": errs(res["dynamic_perturbed"], d),
+        "dynamic_lagged_ms_per_step": round(res["dynamic_lagged"]["ms"], 3),
+        "ek_rel_err_dynamic_lagged_vs_dynamic": errs(res["dynamic_lagged"], d),
+        "learned_refresh_steps": {"learned_gelu": refresh, "learned_relu": refresh, "learned_gelu_coef": None},
+        "training": {"source": f"{n_dns}^3 DNS (N=4, k0=4, nu=5e-4) box-filtered by {r}, {nsnap} snapshots "
+                               f"50 steps apart after 100", "samples": nsnap * n_les ** 3, "batch": 1 << 20,
+                     "adam_steps": 1200, "dns_seconds": round(t_dns, 2),
+                     "c_delta_sq_of_snapshots": [float("%.4e" % c) for c in coefs], "nets": train},
     }
+    for name in nets:
+        x = res[name]
+        out[f"{name}_ms_per_step"] = round(x["ms"], 3)
+        out[f"speedup_{name}_vs_dynamic"] = round(d["ms"] / x["ms"], 3)
+        out[f"speedup_{name}_vs_dynamic_lagged"] = round(res["dynamic_lagged"]["ms"] / x["ms"], 3)
+        out[f"ek_rel_err_{name}_vs_dynamic"] = errs(x, d)
+        out[f"a_posteriori_check_at_end_{name}"] = x["check"]
+    return out
 
 
 def les_dynamic(S, ic, n, steps, warmup):
     """The stratified LES with the dynamic model (dm.f math, (1,4,1) test
-    filter) at n^3: seeded k0 = 8 IC, N = 8, nu = 1e-5, FP32 transforms and
-    products, FP64 state; `steps` timed steps after `warmup`.  Reports the
-    step time and the split over kernel classes (HIP events)."""
+    filter, the procedure fused into one stencil pass) at n^3: seeded k0 = 8
+    IC, N = 8, nu = 1e-5, FP32 transforms and products, FP64 state; `steps`
+    timed steps after `warmup`.  Reports the step time, the split over kernel
+    classes (HIP events), the per-kernel bandwidths by the LES bytes model,
+    the dominant kernel's roofline and the step's fraction of HBM peak."""
     sam = S.Sam(S.make_params(n, sgs=S.SGS_DYN, i_strat=1, grav=64.0, to=1.0, lapse=1.0, vis=1e-5, pr=1.0,
                               i_cfl=1, cfl0=0.5, dt0=0.01, precision=S.PREC_SPLIT))
     sam.set_state(ic.random_solenoidal(n, k0=8.0, seed=20261203, i_strat=1))
@@ -275,6 +367,7 @@ def les_dynamic(S, ic, n, steps, warmup):
     prof = sam.prof_read()
     sam.close()
     tot = sum(v[0] for v in prof.values()) or 1.0
+    roof, per, les_bytes = les_roofline(n, 8, 4, prof, steps)
     return {
         "workload": f"stratified LES {n}^3 with the dynamic model, k0=8, N=8, nu=1e-5, FP32 transforms / FP64 state",
         "ms_per_step": round(wall / steps * 1e3, 3),
@@ -283,6 +376,10 @@ def les_dynamic(S, ic, n, steps, warmup):
         "steps": steps,
         "warmup": warmup,
         "c_delta_sq": float(d["c_delta_sq"]),
+        "roofline": roof,
+        "bytes_per_step": int(les_bytes),
+        "step_hbm_frac": round(les_bytes / (wall / steps) / HBM_PEAK, 4),
+        "kernels": per,
         "kernel_share": {k: round(v[0] / tot, 3) for k, v in prof.items() if v[1]},
     }
 
@@ -395,6 +492,9 @@ def main():
         # launched kernels move by the same count (less with the fused z inverse)
         "step_hbm": {"bytes_per_step": sb, "frac": round(step_frac, 4),
                      "achieved_GBps": round(sb / (r64["wall"] / args.steps) / 1e9, 1),
+                     "bytes_algorithmic_per_step": step_bytes_algorithmic(n, 16),
+                     "frac_algorithmic": round(step_bytes_algorithmic(n, 16) / (r64["wall"] / args.steps) / HBM_PEAK,
+                                               4),
                      "bytes_moved_per_step": int(bm64),
                      "frac_moved": round(bm64 / (r64["wall"] / args.steps) / HBM_PEAK, 4)},
         "kernels": per_kernel,
@@ -447,8 +547,10 @@ def main():
         log("[bench] LES 128^3: learned closure vs dynamic model")
         out["les_128"] = les_comparison(S, ic, n, les_steps=args.les_steps)
         lr = out["les_128"]
-        log(f"[bench] LES dyn {lr['dynamic_ms_per_step']} ms, learned {lr['learned_ms_per_step']} ms, "
-            f"E(k) err {lr['ek_rel_err_learned_vs_dynamic']}")
+        log(f"[bench] LES dyn {lr['dynamic_ms_per_step']} ms, lagged {lr['dynamic_lagged_ms_per_step']} ms, "
+            f"learned GELU {lr['learned_gelu_ms_per_step']} ms (E(k) {lr['ek_rel_err_learned_gelu_vs_dynamic']}), "
+            f"GELU with coefficient {lr['learned_gelu_coef_ms_per_step']} ms "
+            f"(E(k) {lr['ek_rel_err_learned_gelu_coef_vs_dynamic']})")
 
     if not args.no_les512 and n >= 512:
         log("[bench] LES 512^3 with the dynamic model")

@@ -256,7 +256,16 @@ int  sam_mlp_add_samples(sam_mlp *m, sam_handle *dns, int ratio);
  *                        every sam_set_mlp_refresh steps. */
 #define SAM_LABELS_FILTERED 0
 #define SAM_LABELS_DYNAMIC  1
+/*   SAM_LABELS_DYNAMIC_COEF  the dynamic model's stress and flux with its
+ *                        coefficient: C Delta^2 of dm.f's dynamic procedure
+ *                        run on each filtered snapshot (Delta = the filtered
+ *                        grid's spacing) times the unit-coefficient form.  A
+ *                        network trained on these replaces the procedure: the
+ *                        LES uses its output directly, with no refresh. */
+#define SAM_LABELS_DYNAMIC_COEF 2
 int  sam_mlp_set_labels(sam_mlp *m, int mode, double pr_t);
+/* C Delta^2 of the last snapshot added under SAM_LABELS_DYNAMIC_COEF */
+int  sam_mlp_last_coefficient(sam_mlp *m, double *c_delta_sq);
 /* host samples: feats [n][12], labels [n][9] */
 int  sam_mlp_add_samples_host(sam_mlp *m, const float *feats, const float *labels, long long n);
 long long sam_mlp_pool_size(const sam_mlp *m);
@@ -305,8 +314,10 @@ int  sam_sgs_check(sam_handle *h, double tol, double *err, int *switched);
 #define SAM_K_SPEC   6   /* spectra                        */
 #define SAM_K_XB     7   /* x C2R to physical (LES path)   */
 #define SAM_K_XF     8   /* x R2C from physical (LES path) */
-#define SAM_K_SGS    9   /* SGS model kernels (LES path)   */
+#define SAM_K_SGS    9   /* SGS fluxes: eddy-viscosity or network (LES path) */
 #define SAM_K_MISC  10   /* clock and small kernels        */
+#define SAM_K_DMPROC 11  /* dynamic procedure and C Delta^2 (LES path) */
+#define SAM_K_STATS 12   /* vel_max statistics of physical fields (LES path) */
 #define SAM_K_COUNT 16
 /* on != 0: record a HIP event pair around every kernel the step launches */
 int  sam_prof_enable(sam_handle *h, int on);

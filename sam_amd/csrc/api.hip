@@ -534,36 +534,43 @@ template <typename T> static int les_fluxes(sam_handle *h, int stats)
         RCHK(launch_xb<T>(d, Y, d.lu, P, h->tw[0][pi], h->st));
     }
     if (stats) {
-        PROF(SAM_K_FINAL);
+        PROF(SAM_K_STATS);
         int nb = 0;
         RCHK(launch_stats_phys(d, P, sizeof(T) == sizeof(double), d.lu, h->partials, h->diag, h->st, &nb));
         RCHK(launch_finalize(d, h->k, h->partials, nb, h->diag, 1, h->st));
     }
+    // the eddy-viscosity coefficient: when the dynamic procedure runs (every
+    // substep for dm.f, at refresh steps for the lagged coefficient and the
+    // unit-coefficient network) and for Smagorinsky's fixed value
+    bool coef = false, dyn = false;
+    if (h->p.sgs == SAM_SGS_MLP) {
+        // unit-coefficient network: the dynamic procedure supplies C Delta^2
+        // at the first substep of every mlp_refresh-th step
+        if (mlp_dyn && stats && ++h->mlp_since >= h->mlp_refresh) {
+            coef = dyn = true;
+            h->mlp_since = 0;
+        }
+    } else {
+        dyn = h->p.sgs == SAM_SGS_DYN;
+        coef = true;
+        if (dyn && h->dyn_refresh > 1) {
+            // lagged coefficient: the procedure at the first substep of every
+            // dyn_refresh-th step, C Delta^2 held in between
+            coef = stats && ++h->dyn_since >= h->dyn_refresh;
+            if (coef) h->dyn_since = 0;
+        }
+    }
+    if (coef) {
+        PROF(SAM_K_DMPROC);
+        rc = sgs_coefficient<T>(h, P, h->diag, stats, dyn);
+        if (rc) return rc;
+    }
     {
         PROF(SAM_K_SGS);
         if (h->p.sgs == SAM_SGS_MLP) {
-            // unit-coefficient network: the dynamic procedure supplies C Delta^2
-            // at the first substep of every mlp_refresh-th step
-            if (mlp_dyn && stats && ++h->mlp_since >= h->mlp_refresh) {
-                rc = sgs_coefficient<T>(h, P, h->diag, stats, true);
-                if (rc) return rc;
-                h->mlp_since = 0;
-            }
             rc = mlp_flux<T>(h, P, Q);
             if (rc) return rc;
         } else {
-            const bool dyn = h->p.sgs == SAM_SGS_DYN;
-            bool procedure = dyn;
-            if (dyn && h->dyn_refresh > 1) {
-                // lagged coefficient: the procedure at the first substep of every
-                // dyn_refresh-th step, C Delta^2 held in between
-                procedure = stats && ++h->dyn_since >= h->dyn_refresh;
-                if (procedure) h->dyn_since = 0;
-            }
-            if (!dyn || procedure) {
-                rc = sgs_coefficient<T>(h, P, h->diag, stats, dyn);
-                if (rc) return rc;
-            }
             RCHK(launch_sgs_flux<T>(d, P, h->diag, Q, d.nx / h->p.xl, d.ny / h->p.yl, d.nz / h->p.zl,
                                     1.0 / h->p.pr_t, h->st));
         }
@@ -1309,6 +1316,7 @@ struct sam_mlp {
     float *feats = nullptr, *labels = nullptr;
     long long cap = 0, n = 0;
     int label_mode = SAM_LABELS_FILTERED;   // what the pool's labels are (see sam_mlp_set_labels)
+    DevBuf cdiag;                           // C Delta^2 of the last SAM_LABELS_DYNAMIC_COEF snapshot
     int activation = SAM_MLP_ACT_RELU;      // hidden activation (sam_mlp_set_activation)
     float pr_t = 1.0f;
     DevBuf act, gpart, lpart, npart, gout, tmp, lossv;
@@ -1405,7 +1413,7 @@ extern "C" int sam_mlp_destroy(sam_mlp *m)
     if (m->st) hipStreamSynchronize(m->st);
     hipFree(m->P); hipFree(m->norms); hipFree(m->m1); hipFree(m->m2); hipFree(m->img);
     hipFree(m->feats); hipFree(m->labels);
-    for (DevBuf *b : {&m->act, &m->gpart, &m->lpart, &m->npart, &m->gout, &m->tmp, &m->lossv}) hipFree(b->p);
+    for (DevBuf *b : {&m->act, &m->gpart, &m->lpart, &m->npart, &m->gout, &m->tmp, &m->lossv, &m->cdiag}) hipFree(b->p);
     if (m->st) hipStreamDestroy(m->st);
     delete m;
     return SAM_OK;
@@ -1491,8 +1499,29 @@ extern "C" int sam_mlp_add_samples(sam_mlp *m, sam_handle *dns, int r)
         RCHK(launch_box_filter<double>(d, (const double *)dns->Pb.p, r, coarse, m->labels + 16 * m->n, m->st));
     RCHK(launch_coarse_features(coarse, cx, cy, cz, d.lu, (float)(cx / dns->p.xl), (float)(cy / dns->p.yl),
                                 (float)(cz / dns->p.zl), m->feats + 16 * m->n, m->st));
-    if (m->label_mode == SAM_LABELS_DYNAMIC)
-        RCHK(launch_dyn_labels(m->feats + 16 * m->n, nc, 1.0f / m->pr_t, m->labels + 16 * m->n, m->st));
+    if (m->label_mode == SAM_LABELS_DYNAMIC || m->label_mode == SAM_LABELS_DYNAMIC_COEF) {
+        const double *cs = nullptr;
+        if (m->label_mode == SAM_LABELS_DYNAMIC_COEF) {
+            // dm.f's dynamic procedure on the filtered field itself: C Delta^2 at
+            // the coarse grid's Delta, the coefficient an LES of that grid would use
+            Dims cd = dns->d;
+            cd.nx = cx;
+            cd.ny = cy;
+            cd.nz = cz;
+            cd.pfield = nc;
+            if (cx < 16 || cy % 8 || cz < 8)
+                return set_err(SAM_ERR_VALUE, "dynamic-coefficient labels need a filtered grid of >= 16 x 8k x 8 points");
+            rc = ensure(m->npart, sizeof(double) * std::max<long long>(256 * 32, 2 * dm_fused_rows(cd)));
+            if (!rc) rc = ensure(m->cdiag, sizeof(DevDiag));
+            if (rc) return rc;
+            int nb = 0;
+            RCHK(launch_dm_fused<float>(cd, coarse, cx / dns->p.xl, cy / dns->p.yl, cz / dns->p.zl,
+                                        (double *)m->npart.p, m->st, &nb));
+            RCHK(launch_dm_coef((const double *)m->npart.p, nb, 0.0, 1, 1, (DevDiag *)m->cdiag.p, m->st));
+            cs = &((const DevDiag *)m->cdiag.p)->c_delta_sq;
+        }
+        RCHK(launch_dyn_labels(m->feats + 16 * m->n, nc, 1.0f / m->pr_t, cs, m->labels + 16 * m->n, m->st));
+    }
     HIPCHK(hipStreamSynchronize(m->st));
     m->n += nc;
     return SAM_OK;
@@ -1516,7 +1545,8 @@ extern "C" int sam_mlp_get_activation(const sam_mlp *m, int *act)
 extern "C" int sam_mlp_set_labels(sam_mlp *m, int mode, double pr_t)
 {
     if (!m) return set_err(SAM_ERR_VALUE, "null argument");
-    if (mode != SAM_LABELS_FILTERED && mode != SAM_LABELS_DYNAMIC) return set_err(SAM_ERR_VALUE, "unknown label mode %d", mode);
+    if (mode != SAM_LABELS_FILTERED && mode != SAM_LABELS_DYNAMIC && mode != SAM_LABELS_DYNAMIC_COEF)
+        return set_err(SAM_ERR_VALUE, "unknown label mode %d", mode);
     if (!(pr_t > 0)) return set_err(SAM_ERR_VALUE, "pr_t must be > 0");
     if (m->n > 0 && mode != m->label_mode) return set_err(SAM_ERR_STATE, "the sample pool holds labels of the other kind");
     m->label_mode = mode;
@@ -1525,6 +1555,18 @@ extern "C" int sam_mlp_set_labels(sam_mlp *m, int mode, double pr_t)
 }
 
 static bool mlp_unit_coefficient(const sam_mlp *m) { return m->label_mode == SAM_LABELS_DYNAMIC; }
+
+extern "C" int sam_mlp_last_coefficient(sam_mlp *m, double *c_delta_sq)
+{
+    if (!m || !c_delta_sq) return set_err(SAM_ERR_VALUE, "null argument");
+    if (m->label_mode != SAM_LABELS_DYNAMIC_COEF || !m->cdiag.p)
+        return set_err(SAM_ERR_STATE, "no dynamic-coefficient snapshot has been added");
+    DevDiag dg;
+    HIPCHK(hipStreamSynchronize(m->st));
+    HIPCHK(hipMemcpy(&dg, m->cdiag.p, sizeof dg, hipMemcpyDeviceToHost));
+    *c_delta_sq = dg.c_delta_sq;
+    return SAM_OK;
+}
 
 extern "C" int sam_set_dyn_refresh(sam_handle *h, int nsteps)
 {

@@ -156,7 +156,10 @@ int launch_mlp_infer(const float *P, const void *img, const float *norms, const 
 template <typename T>
 int launch_les_mlp_flux(const Dims &d, const void *img, const float *norms, const T *U, T *Q, float dxi, float dyi,
                         float dzi, const double *cscale, int act, hipStream_t st);
-int launch_dyn_labels(const float *feats, long long n, float pr_t_inv, float *labels, hipStream_t st);
+// the dynamic model's stress and flux at unit coefficient of the features
+// gradients, times *cscale (device) when cscale is not null
+int launch_dyn_labels(const float *feats, long long n, float pr_t_inv, const double *cscale, float *labels,
+                      hipStream_t st);
 template <typename T> int launch_box_filter(const Dims &d, const T *U, int r, float *coarse, float *labels, hipStream_t st);
 int launch_coarse_features(const float *C, int nx, int ny, int nz, int lu, float dxi, float dyi, float dzi,
                            float *feats, hipStream_t st);

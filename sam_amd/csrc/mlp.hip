@@ -445,13 +445,19 @@ __global__ __launch_bounds__(256) void k_mlp_adam(const float *__restrict__ part
     P[j] -= lr * mh / (sqrtf(vh) + eps);
 }
 
-__global__ void k_sum_loss(const double *__restrict__ part, int n, double *__restrict__ out)
+// the per-block loss partials in a fixed order (strided per thread, then a tree)
+__global__ __launch_bounds__(256) void k_sum_loss(const double *__restrict__ part, int n, double *__restrict__ out)
 {
-    if (threadIdx.x == 0) {
-        double s = 0.0;
-        for (int i = 0; i < n; i++) s += part[i];
-        *out = s;
+    __shared__ double r[256];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) s += part[i];
+    r[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) r[threadIdx.x] += r[threadIdx.x + o];
+        __syncthreads();
     }
+    if (threadIdx.x == 0) *out = r[0];
 }
 
 // ---------------------------------------------------------------------
@@ -670,8 +676,9 @@ __global__ __launch_bounds__(256) void k_box_filter(const T *__restrict__ U, int
 // -(|S|/Pr_t) dT/dx_j, with S_ij = (du_i/dx_j + du_j/dx_i)/2 and |S| =
 // sqrt(2 S_ij S_ij) from the same centred differences as the features.  The
 // dynamic procedure supplies C Delta^2 at run time.
+// cscale (device, may be null): C Delta^2 the labels are multiplied by
 __global__ __launch_bounds__(256) void k_dyn_labels(const float *__restrict__ feats, long long n, float pr_t_inv,
-                                                     float *__restrict__ labels)
+                                                     const double *__restrict__ cscale, float *__restrict__ labels)
 {
     const long long p = (long long)blockIdx.x * 256 + threadIdx.x;
     if (p >= n) return;
@@ -685,7 +692,7 @@ __global__ __launch_bounds__(256) void k_dyn_labels(const float *__restrict__ fe
             S[i][j] = 0.5 * (g[i][j] + g[j][i]);
             s2 += S[i][j] * S[i][j];
         }
-    const double smag = sqrt(2.0 * s2);
+    const double smag = sqrt(2.0 * s2) * (cscale ? *cscale : 1.0);
     const double tr = (S[0][0] + S[1][1] + S[2][2]) / 3.0;
     float *L = labels + p * 16;
     L[0] = (float)(-2.0 * smag * (S[0][0] - tr));
@@ -793,7 +800,7 @@ int launch_mlp_adam(const float *part, int nch, float *P, float *m1, float *m2, 
 
 int launch_sum_loss(const double *part, int n, double *out, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_sum_loss, dim3(1), dim3(64), 0, st, part, n, out);
+    hipLaunchKernelGGL(k_sum_loss, dim3(1), dim3(256), 0, st, part, n, out);
     return launched_mlp();
 }
 
@@ -812,9 +819,11 @@ int launch_mlp_infer(const float *P, const void *img, const float *norms, const 
     return launched_mlp();
 }
 
-int launch_dyn_labels(const float *feats, long long n, float pr_t_inv, float *labels, hipStream_t st)
+int launch_dyn_labels(const float *feats, long long n, float pr_t_inv, const double *cscale, float *labels,
+                      hipStream_t st)
 {
-    hipLaunchKernelGGL(k_dyn_labels, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, feats, n, pr_t_inv, labels);
+    hipLaunchKernelGGL(k_dyn_labels, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, feats, n, pr_t_inv, cscale,
+                       labels);
     return launched_mlp();
 }
 

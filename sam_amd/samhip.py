@@ -39,19 +39,19 @@ EXPORTS = (
     "sam_prof_enable", "sam_prof_read", "sam_sync", "sam_device_copy",
     "sam_mlp_create", "sam_mlp_destroy", "sam_mlp_get_params", "sam_mlp_set_params",
     "sam_mlp_add_samples", "sam_mlp_add_samples_host", "sam_mlp_pool_size", "sam_mlp_get_samples",
-    "sam_mlp_clear_pool", "sam_mlp_set_labels", "sam_mlp_set_activation", "sam_mlp_get_activation",
+    "sam_mlp_clear_pool", "sam_mlp_set_labels", "sam_mlp_last_coefficient", "sam_mlp_set_activation", "sam_mlp_get_activation",
     "sam_mlp_fit_norms", "sam_mlp_train", "sam_mlp_grad", "sam_mlp_infer",
     "sam_attach_mlp", "sam_set_sgs", "sam_sgs_check", "sam_set_mlp_refresh",
     "sam_set_forcing", "sam_set_cl", "sam_set_planes", "sam_planes", "sam_set_dyn_refresh",
 )
 
 MLP_NPARAM = 10448
-LABELS_FILTERED, LABELS_DYNAMIC = 0, 1
+LABELS_FILTERED, LABELS_DYNAMIC, LABELS_DYNAMIC_COEF = 0, 1, 2
 # hidden activation of the learned closure (SAM_MLP_ACT_*)
 ACT_RELU, ACT_GELU = 0, 1
 
 # kernel classes of sam_prof_read (SAM_K_* in include/samhip.h)
-KERNEL_CLASSES = ("zb", "yb", "xrow", "final", "yf", "zrhs", "spec", "xb", "xf", "sgs", "misc")
+KERNEL_CLASSES = ("zb", "yb", "xrow", "final", "yf", "zrhs", "spec", "xb", "xf", "sgs", "misc", "dmproc", "stats")
 K_COUNT = 16
 
 
@@ -154,6 +154,7 @@ def load():
     L.sam_mlp_get_samples.argtypes = [_hp, ctypes.c_longlong, ctypes.c_longlong, _fp, _fp]
     L.sam_mlp_fit_norms.argtypes = [_hp]
     L.sam_mlp_set_labels.argtypes = [_hp, ctypes.c_int, ctypes.c_double]
+    L.sam_mlp_last_coefficient.argtypes = [_hp, _dp]
     L.sam_mlp_set_activation.argtypes = [_hp, ctypes.c_int]
     L.sam_mlp_get_activation.argtypes = [_hp, ctypes.POINTER(ctypes.c_int)]
     L.sam_mlp_train.argtypes = [_hp, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_uint, _fp]
@@ -442,9 +443,17 @@ class Mlp:
         _chk(self.L.sam_mlp_clear_pool(self.h))
 
     def set_labels(self, mode, pr_t=1.0):
-        """LABELS_FILTERED (subfilter stress of the box filter) or LABELS_DYNAMIC
-        (dynamic-model stress at unit coefficient); set before adding samples"""
+        """LABELS_FILTERED (subfilter stress of the box filter), LABELS_DYNAMIC
+        (dynamic-model stress at unit coefficient) or LABELS_DYNAMIC_COEF (with
+        the coefficient of the dynamic procedure on each filtered snapshot);
+        set before adding samples"""
         _chk(self.L.sam_mlp_set_labels(self.h, int(mode), ctypes.c_double(pr_t)))
+
+    def last_coefficient(self):
+        """C Delta^2 of the last LABELS_DYNAMIC_COEF snapshot"""
+        c = ctypes.c_double()
+        _chk(self.L.sam_mlp_last_coefficient(self.h, ctypes.byref(c)))
+        return c.value
 
     def set_activation(self, act):
         """ACT_RELU (default) or ACT_GELU; part of the network, set before training"""
