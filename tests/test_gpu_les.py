@@ -163,3 +163,35 @@ def test_lagged_dynamic_coefficient_holds_between_refreshes():
     every.set_state(u0)
     every.step(4)
     assert np.linalg.norm(every.get_state() - b) > 1e-10 * np.linalg.norm(b)
+
+
+def test_dynamic_coefficient_labels_match_oracle(oracle_mod):
+    """SAM_LABELS_DYNAMIC_COEF: the labels are the unit-coefficient labels
+    times C Delta^2 of dm.f's dynamic procedure run on the filtered field;
+    that coefficient equals the oracle's dm.f coefficient of the same box
+    filter of the same DNS field (FP32 filtered field: 2e-3)"""
+    O = oracle_mod
+    n, r = 64, 2
+    kw = dict(KW, precision=S.PREC_FP64)
+    dns = S.Sam(S.make_params(n, **kw))
+    dns.set_state(ic.random_solenoidal(n, k0=6, seed=4, i_strat=1))
+    dns.step(5)
+    nets = {}
+    for mode in (S.LABELS_DYNAMIC, S.LABELS_DYNAMIC_COEF):
+        m = S.Mlp(seed=2, pool_capacity=(n // r) ** 3)
+        m.set_labels(mode, 1.0)
+        m.add_samples(dns, r)
+        nets[mode] = m
+    c = nets[S.LABELS_DYNAMIC_COEF].last_coefficient()
+    _, unit = nets[S.LABELS_DYNAMIC].get_samples(0, (n // r) ** 3)
+    _, coef = nets[S.LABELS_DYNAMIC_COEF].get_samples(0, (n // r) ** 3)
+    assert np.linalg.norm(coef - c * unit) <= 1e-6 * np.linalg.norm(coef)
+    # the oracle: box filter of the physical field, dm.f on the coarse grid
+    o = O.Oracle(O.make_cfg(n, i_strat=1))
+    phys = o.to_physical(dns.get_state())
+    cx = n // r
+    coarse = phys[..., :3].reshape(cx, r, cx, r, cx, r, 3).mean(axis=(1, 3, 5)).astype(np.float32)
+    oc = O.Oracle(O.make_cfg(cx, sgs=2))
+    _, c_ref = oc.eddy_visc(np.asfortranarray(coarse.astype(np.float64)))
+    print(f"C Delta^2 of the filtered field: GPU {c:.6e}, oracle {c_ref:.6e}")
+    assert abs(c - c_ref) <= 2e-3 * abs(c_ref)
