@@ -404,34 +404,112 @@ int launch_dm_coef(const double *partials, int nblocks, double c_fixed, int dyna
     return launched_sgs();
 }
 
-// fluxes with the eddy viscosity nu_t = C Delta^2 |S| (C Delta^2 in diag)
-template <typename T>
-__global__ __launch_bounds__(256) void k_sgs_flux(const T *__restrict__ P, const DevDiag *__restrict__ diag,
-                                                   T *__restrict__ Q, Grid3 g, double dxi, double dyi, double dzi,
-                                                   double prt_inv, int lu)
+// fluxes with the eddy viscosity nu_t = C Delta^2 |S| (C Delta^2 in diag):
+// F_ij = u_i u_j - 2 nu_t S_ij, F_Tj = u_j T - (nu_t / Pr_t) dT/dx_j, centred
+// differences as strain_c.  A block owns a DMX x FY tile in x, y and ZC
+// planes in z and marches z: plane z of u, v, w, T with a 1-point x-y halo is
+// in LDS (a ring of two planes; plane z+2 is loaded into registers during
+// plane z), each thread keeps its own column's values of planes z-1, z, z+1
+// in registers for the z differences.  The fields are read from HBM once
+// per block plus the x-y halo, instead of once per stencil point.
+constexpr int FY = 8;
+
+template <typename T, int DMX>
+__global__ __launch_bounds__(DMX * FY) void k_sgs_flux(const T *__restrict__ P, const DevDiag *__restrict__ diag,
+                                                        T *__restrict__ Q, Grid3 g, int zc, double dxi, double dyi,
+                                                        double dzi, double prt_inv, int lu)
 {
-    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (i >= g.np) return;
-    const int lx = __builtin_ctz(g.nx), ly = __builtin_ctz(g.ny);   // power-of-two sizes: no division
-    const int x = (int)(i & (g.nx - 1)), y = (int)((i >> lx) & (g.ny - 1)), z = (int)(i >> (lx + ly));
-    double s[6];
-    strain_c(P, g, x, y, z, dxi, dyi, dzi, s);
-    const double nut = diag->c_delta_sq * smag6(s);
-    const double u = (double)P[i], v = (double)P[g.np + i], w = (double)P[2 * g.np + i];
-    const long long np = g.np;
-    Q[i] = (T)(u * u - 2.0 * nut * s[0]);
-    Q[np + i] = (T)(u * v - 2.0 * nut * s[3]);
-    Q[2 * np + i] = (T)(v * v - 2.0 * nut * s[1]);
-    Q[3 * np + i] = (T)(u * w - 2.0 * nut * s[5]);
-    Q[4 * np + i] = (T)(v * w - 2.0 * nut * s[4]);
-    Q[5 * np + i] = (T)(w * w - 2.0 * nut * s[2]);
-    if (lu == 4) {
-        const T *t = P + 3 * np;
-        const double tt = (double)t[i];
-        const double k = nut * prt_inv * 0.5;
-        Q[6 * np + i] = (T)(u * tt - k * ((double)t[g.at(x + 1, y, z)] - (double)t[g.at(x - 1, y, z)]) * dxi);
-        Q[7 * np + i] = (T)(v * tt - k * ((double)t[g.at(x, y + 1, z)] - (double)t[g.at(x, y - 1, z)]) * dyi);
-        Q[8 * np + i] = (T)(w * tt - k * ((double)t[g.at(x, y, z + 1)] - (double)t[g.at(x, y, z - 1)]) * dzi);
+    constexpr int NT = DMX * FY, HX = DMX + 2, HY = FY + 2;
+    constexpr int NH = 4 * HX * HY;               // values of one plane (4 fields, halo 1)
+    constexpr int NPRE = (NH + NT - 1) / NT;
+    __shared__ T pl[2][4][HY][HX];
+    const int tid = threadIdx.x, tx = tid % DMX, ty = tid / DMX;
+    const int x0 = blockIdx.x * DMX, y0 = blockIdx.y * FY, z0 = blockIdx.z * zc;
+    const int nf = lu;                            // fields read: u, v, w (, T)
+    const int nh = nf * HX * HY;
+    auto fetch = [&](int zz, T (&v)[NPRE]) {
+#pragma unroll
+        for (int k = 0; k < NPRE; k++) {
+            const int e = tid + k * NT;
+            T val = (T)0;
+            if (e < nh) {
+                const int f = e / (HX * HY), r = e % (HX * HY);
+                val = P[f * g.np + g.at(x0 - 1 + r % HX, y0 - 1 + r / HX, zz)];
+            }
+            v[k] = val;
+        }
+    };
+    auto store = [&](int slot, const T (&v)[NPRE]) {
+#pragma unroll
+        for (int k = 0; k < NPRE; k++) {
+            const int e = tid + k * NT;
+            if (e < nh) {
+                const int f = e / (HX * HY), r = e % (HX * HY);
+                pl[slot][f][r / HX][r % HX] = v[k];
+            }
+        }
+    };
+    // own column at plane z0 - 1 (registers), planes z0, z0 + 1 (LDS slots 0, 1)
+    double below[4];
+#pragma unroll
+    for (int f = 0; f < 4; f++)
+        below[f] = f < nf ? (double)P[f * g.np + g.at(x0 + tx, y0 + ty, z0 - 1)] : 0.0;
+    {
+        T v[NPRE];
+        fetch(z0, v);
+        store(0, v);
+        fetch(z0 + 1, v);
+        store(1, v);
+    }
+    __syncthreads();
+    const double cds = diag->c_delta_sq;
+    const int lx = __builtin_ctz(g.nx), ly = __builtin_ctz(g.ny);
+    for (int z = z0; z < z0 + zc; z++) {
+        const int sc = (z - z0) & 1, sn = sc ^ 1;       // slots of planes z and z + 1
+        T nxt[NPRE];
+        const bool more = z + 2 <= z0 + zc;
+        if (more) fetch(z + 2, nxt);
+        const int hx = tx + 1, hy = ty + 1;
+        double c[4], ax[4], ay[4], az[4];
+#pragma unroll
+        for (int f = 0; f < 4; f++) {
+            if (f < nf) {
+                c[f] = (double)pl[sc][f][hy][hx];
+                ax[f] = ((double)pl[sc][f][hy][hx + 1] - (double)pl[sc][f][hy][hx - 1]) * dxi;
+                ay[f] = ((double)pl[sc][f][hy + 1][hx] - (double)pl[sc][f][hy - 1][hx]) * dyi;
+                az[f] = ((double)pl[sn][f][hy][hx] - below[f]) * dzi;
+            } else {
+                c[f] = ax[f] = ay[f] = az[f] = 0.0;
+            }
+        }
+        double s[6];
+        s[0] = 0.5 * ax[0];
+        s[1] = 0.5 * ay[1];
+        s[2] = 0.5 * az[2];
+        s[3] = 0.25 * (ay[0] + ax[1]);
+        s[4] = 0.25 * (az[1] + ay[2]);
+        s[5] = 0.25 * (ax[2] + az[0]);
+        const double nut = cds * smag6(s);
+        const long long i = ((long long)z << (lx + ly)) + ((long long)(y0 + ty) << lx) + (x0 + tx);
+        const long long np = g.np;
+        const double u = c[0], v = c[1], w = c[2];
+        Q[i] = (T)(u * u - 2.0 * nut * s[0]);
+        Q[np + i] = (T)(u * v - 2.0 * nut * s[3]);
+        Q[2 * np + i] = (T)(v * v - 2.0 * nut * s[1]);
+        Q[3 * np + i] = (T)(u * w - 2.0 * nut * s[5]);
+        Q[4 * np + i] = (T)(v * w - 2.0 * nut * s[4]);
+        Q[5 * np + i] = (T)(w * w - 2.0 * nut * s[2]);
+        if (lu == 4) {
+            const double tt = c[3], k = nut * prt_inv * 0.5;
+            Q[6 * np + i] = (T)(u * tt - k * ax[3]);
+            Q[7 * np + i] = (T)(v * tt - k * ay[3]);
+            Q[8 * np + i] = (T)(w * tt - k * az[3]);
+        }
+#pragma unroll
+        for (int f = 0; f < 4; f++) below[f] = c[f];
+        __syncthreads();                  // plane z is read by every thread
+        if (more) store(sc, nxt);         // plane z + 2 into the slot of plane z
+        __syncthreads();
     }
 }
 
@@ -440,8 +518,18 @@ int launch_sgs_flux(const Dims &d, const T *P, const DevDiag *diag, T *Q, double
                     double pr_t_inv, hipStream_t st)
 {
     Grid3 g{d.nx, d.ny, d.nz, d.pfield};
-    hipLaunchKernelGGL((k_sgs_flux<T>), dim3((unsigned)((g.np + 255) / 256)), dim3(256), 0, st, P, diag, Q, g, dxi,
-                       dyi, dzi, pr_t_inv, d.lu);
+    if (d.ny % FY) return SAM_ERR_UNSUPPORTED;
+    const int dmx = d.nx >= 32 ? 32 : 16;
+    const long long cols = (long long)(d.nx / dmx) * (d.ny / FY);
+    int zc = d.nz;                                 // z chunks until about 8192 blocks, at least 8 planes
+    while (zc > 8 && cols * (d.nz / zc) < 8192) zc >>= 1;
+    const dim3 grid((unsigned)(d.nx / dmx), (unsigned)(d.ny / FY), (unsigned)(d.nz / zc));
+    if (dmx == 32)
+        hipLaunchKernelGGL((k_sgs_flux<T, 32>), grid, dim3(32 * FY), 0, st, P, diag, Q, g, zc, dxi, dyi, dzi,
+                           pr_t_inv, d.lu);
+    else
+        hipLaunchKernelGGL((k_sgs_flux<T, 16>), grid, dim3(16 * FY), 0, st, P, diag, Q, g, zc, dxi, dyi, dzi,
+                           pr_t_inv, d.lu);
     return launched_sgs();
 }
 
