@@ -85,11 +85,12 @@ def step_bytes_algorithmic(n, bc, i_strat=1, nrk=3):
     return step_bytes(n, bc, i_strat, nrk) - g["lu"] * g["S"] * 16
 
 
-def les_kernel_bytes(n, bc, bp, i_strat=1, nrk=3):
+def les_kernel_bytes(n, bc, bp, i_strat=1, nrk=3, zrhs_fused=False):
     """Algorithmic HBM bytes per launch of the LES step's kernels (the stage
     list of the step with an SGS model; physical fields of bp = 8 or 4 bytes
     per value, P points per field):
-      zb, yb      as in the DNS step (k_zb runs every substep: no fused z inverse)
+      zb, yb      as in the DNS step (k_zb runs every substep unless k_zrhs
+                  writes the next z inverse: the persistent form, Nz = 512)
       xb          Lu fields Y -> P                      (x inverse to physical)
       stats       reads Lu P fields                     (vel_max, first substep)
       dmproc      reads u, v, w once                    (fused dynamic procedure)
@@ -97,10 +98,10 @@ def les_kernel_bytes(n, bc, bp, i_strat=1, nrk=3):
                   network fluxes)
       xf          Lr fields P -> Y                      (x forward)
       yf          as in the DNS step
-      zrhs        as in the DNS step, without the fused z inverse"""
+      zrhs        as in the DNS step, with the fused z inverse where it runs"""
     g = grid_counts(n, i_strat)
     S, Z, Y, P, lu, lr = g["S"], g["Z"], g["Y"], g["P"], g["lu"], g["lr"]
-    kb = kernel_bytes(n, bc, i_strat, nrk=nrk)
+    kb = kernel_bytes(n, bc, i_strat, zrhs_fused=zrhs_fused, nrk=nrk)
     return {
         "zb": kb["zb"], "yb": kb["yb"],
         "xb": lu * Y * bc + lu * P * bp,
@@ -115,7 +116,7 @@ def les_kernel_bytes(n, bc, bp, i_strat=1, nrk=3):
 def les_roofline(n, bc, bp, prof, steps):
     """per-kernel achieved bandwidth of an LES run, the dominant kernel's
     roofline and the step's algorithmic bytes (launch counts from the run)"""
-    kb = les_kernel_bytes(n, bc, bp)
+    kb = les_kernel_bytes(n, bc, bp, zrhs_fused=zrhs_fused(prof))
     per, tot_bytes = {}, 0.0
     tot_ms = sum(v[0] for v in prof.values()) or 1.0
     best = None

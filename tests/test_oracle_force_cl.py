@@ -1,9 +1,11 @@
 """Pin the oracle's spectral forcing (force.f) and Chollet-Lesieur model
 (cl_energy.f, spectra.f:39-43) -- CPU only.
 
-The reference routines call MPI (mpi_allreduce), which this image does not
-provide, so they are not compiled into oracle/_ref.  They are pinned here by
-independent evaluations:
+force.f itself is pinned through the reference's own program: the force16
+run of tests/golden/sam_runs.npz (the reference's `sam` built with MPICH by
+oracle/Makefile, run with force.f after every step) is matched by the oracle
+in test_oracle_refrun.py.  No reference run covers cl_energy.f (i_les = 1).
+Both are also pinned here by independent evaluations:
 - forcing: after force.f the shell energies 2..k_force+1 of u, v, w equal
   the target exactly, every other shell, the mean and T' are untouched, and
   the divergence is unchanged (all components of a mode share one factor);
