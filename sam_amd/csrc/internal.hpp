@@ -48,6 +48,21 @@ struct StepConsts {
     double cl_a, cl_b, cl_c, k_cl, pr_t_inv;
 };
 
+// XCD-aware tile order.  The dispatcher deals the blocks of a launch out to
+// the 8 XCDs round robin (block i runs on XCD i % 8), and each XCD has its own
+// L2.  Renumbering so that XCD x takes the contiguous tile range
+// [x q + min(x, r), ...) (q = n / 8, r = n % 8, a bijection of [0, n)) keeps
+// the neighbouring tiles of one XCD on neighbouring HBM rows.  At 512^3 FP64
+// this made k_yb 10% faster (992 -> 896 us), k_zb, k_xrow and k_yf 2-3%.
+// The FP32 y-forward transform and k_zrhs ran no faster, and they keep the
+// launch order.  The SGS stencil kernels (sgs.hip) use it so that tiles that
+// share halo lines run on one XCD, whose L2 then serves the halo.
+__device__ __forceinline__ int xcd_tile(int i, int n)
+{
+    const int q = n / 8, r = n % 8, x = i % 8;
+    return x * q + (x < r ? x : r) + i / 8;
+}
+
 /* Device buffer layouts (complex elements, first index fastest):
  *   S : (Nz_min, Ny_min, Nx_min, F)   spectral state / RHS, FP64
  *   Z : (Nz, Ny_min, Nx_min, F)       z in physical space, x,y spectral

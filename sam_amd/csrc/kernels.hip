@@ -65,19 +65,6 @@ static unsigned persistent_grid(const void *kernel, int threads, long long nwork
     return (unsigned)(nwork < nres ? nwork : nres);
 }
 
-// XCD-aware tile order.  The dispatcher deals the blocks of a launch out to
-// the 8 XCDs round robin (block i runs on XCD i % 8), and each XCD has its own
-// L2.  Renumbering so that XCD x takes the contiguous tile range
-// [x q + min(x, r), ...) (q = n / 8, r = n % 8, a bijection of [0, n)) keeps
-// the neighbouring tiles of one XCD on neighbouring HBM rows.  At 512^3 FP64
-// this made k_yb 10% faster (992 -> 896 us), k_zb, k_xrow and k_yf 2-3%.
-// The FP32 y-forward transform and k_zrhs ran no faster, and they keep the
-// launch order.
-__device__ __forceinline__ int xcd_tile(int i, int n)
-{
-    const int q = n / 8, r = n % 8, x = i % 8;
-    return x * q + (x < r ? x : r) + i / 8;
-}
 
 // columns whose strided side is read/written in 128-byte rows
 template <typename T> struct Row128 { static constexpr int C = 128 / (int)sizeof(cpx<T>); };

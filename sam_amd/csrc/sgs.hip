@@ -183,7 +183,9 @@ __global__ __launch_bounds__(DMX * DMY) void k_dm_fused(const T *__restrict__ P,
     __shared__ T xf[DMF][PY][DMX];
     __shared__ double red[2][NT / 64];
     const int tid = threadIdx.x;
-    const int x0 = blockIdx.x * DMX, y0 = blockIdx.y * DMY, z0 = blockIdx.z * zc;
+    // tile of this block, dealt to the XCDs as in k_sgs_flux
+    const int ntx = g.nx / DMX, nty = g.ny / DMY, t = xcd_tile(blockIdx.x, gridDim.x);
+    const int x0 = (t % ntx) * DMX, y0 = (t / ntx % nty) * DMY, z0 = t / (ntx * nty) * zc;
     const int tx = tid % DMX, ty = tid / DMX;     // stage B column
     const int arow = tid / NSEG, aseg = tid % NSEG;   // stage A item
     const C cdx = (C)dxi, cdy = (C)dyi, cdz = (C)dzi;
@@ -333,9 +335,8 @@ __global__ __launch_bounds__(DMX * DMY) void k_dm_fused(const T *__restrict__ P,
             a += red[0][w];
             b += red[1][w];
         }
-        const long long blk = ((long long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-        partials[2 * blk] = a;
-        partials[2 * blk + 1] = b;
+        partials[2 * t] = a;      // in tile order: the coefficient sums them in that order
+        partials[2 * t + 1] = b;
     }
 }
 
@@ -363,8 +364,8 @@ int launch_dm_fused(const Dims &d, const T *P, double dxi, double dyi, double dz
     const int dmx = d.nx >= 32 ? 32 : 16;
     if (d.ny % DMY) return SAM_ERR_UNSUPPORTED;
     const int zc = dm_planes(d, dmx);
-    const dim3 grid((unsigned)(d.nx / dmx), (unsigned)(d.ny / DMY), (unsigned)(d.nz / zc));
-    *nblocks_out = (int)(grid.x * grid.y * grid.z);
+    const dim3 grid((unsigned)((d.nx / dmx) * (d.ny / DMY) * (d.nz / zc)));
+    *nblocks_out = (int)grid.x;
     if (dmx == 32)
         hipLaunchKernelGGL((k_dm_fused<T, 32>), grid, dim3(32 * DMY), 0, st, P, g, zc, dxi, dyi, dzi, partials);
     else
@@ -460,7 +461,10 @@ __global__ __launch_bounds__(DMX * FY) void k_sgs_flux(const T *__restrict__ P, 
     constexpr int NPRE = (NH + NT - 1) / NT;
     __shared__ T pl[2][4][HY][HX];
     const int tid = threadIdx.x, tx = tid % DMX, ty = tid / DMX;
-    const int x0 = blockIdx.x * DMX, y0 = blockIdx.y * FY, z0 = blockIdx.z * zc;
+    // tile of this block: x fastest, then y, then the z chunk, dealt to the
+    // XCDs in contiguous ranges (neighbouring tiles share their halo lines)
+    const int ntx = g.nx / DMX, nty = g.ny / FY, t = xcd_tile(blockIdx.x, gridDim.x);
+    const int x0 = (t % ntx) * DMX, y0 = (t / ntx % nty) * FY, z0 = t / (ntx * nty) * zc;
     const int nf = lu;                            // fields read: u, v, w (, T)
     const int nh = nf * HX * HY;
     auto fetch = [&](int zz, T (&v)[NPRE]) {
@@ -563,7 +567,7 @@ int launch_sgs_flux(const Dims &d, const T *P, const DevDiag *diag, T *Q, double
     const long long cols = (long long)(d.nx / dmx) * (d.ny / FY);
     int zc = d.nz;                                 // z chunks until about 8192 blocks, at least 8 planes
     while (zc > 8 && cols * (d.nz / zc) < 8192) zc >>= 1;
-    const dim3 grid((unsigned)(d.nx / dmx), (unsigned)(d.ny / FY), (unsigned)(d.nz / zc));
+    const dim3 grid((unsigned)(cols * (d.nz / zc)));
     if (dmx == 32)
         hipLaunchKernelGGL((k_sgs_flux<T, 32>), grid, dim3(32 * FY), 0, st, P, diag, Q, g, zc, dxi, dyi, dzi,
                            pr_t_inv, d.lu);
