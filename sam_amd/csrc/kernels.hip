@@ -117,6 +117,13 @@ int launch_zb(const Dims &d, const double *S, int ldz, int nvar, T *Z, const voi
     return launched();
 }
 
+// z columns per y-inverse block: 128-byte rows, except FP32 at Ny = 1024,
+// where 16 columns (128 KB of LDS, 1024 threads) leave one block per CU; 8
+// columns (64-byte rows) let two blocks overlap: 4.57 -> 4.24 ms per launch
+// at 1024^3.  The forward y kernel ran slower with 8 (9.70 -> 10.10 ms) and
+// keeps 16.
+template <typename T, int N> constexpr int yb_cols() { return (sizeof(T) == 4 && N >= 1024) ? 8 : Row128<T>::C; }
+
 // ---------------------------------------------------------------------
 // k_yb: y zero-pad + y inverse transform, Z -> Y.  Tile = C consecutive z
 // of one (field, kx) slab; loads 128-byte rows, stores contiguous columns.
@@ -163,8 +170,8 @@ __global__ __launch_bounds__(C * N / EPT) void k_yb(const T *__restrict__ Zr, T 
 template <typename T>
 int launch_yb(const Dims &d, const T *Z, int nvar, T *Y, const void *twy, hipStream_t st)
 {
-    constexpr int C = Row128<T>::C;
     SAM_SWITCH_N(d.ny, {
+        constexpr int C = yb_cols<T, N>();
         const unsigned nb = (unsigned)((long long)nvar * d.nxm * (d.nz / C));
         if constexpr (C * N / 8 <= 512)
             hipLaunchKernelGGL((k_yb<T, N, C, 8>), dim3(nb), dim3(C * N / 8), 0, st, Z, Y, (const cpx<T> *)twy, d);
