@@ -1321,41 +1321,16 @@ struct sam_mlp {
     DevBuf cdiag;                           // C Delta^2 of the last SAM_LABELS_DYNAMIC_COEF snapshot
     int activation = SAM_MLP_ACT_RELU;      // hidden activation (sam_mlp_set_activation)
     float pr_t = 1.0f;
-    DevBuf act, gpart, lpart, npart, gout, tmp, lossv;
+    DevBuf gpart, lpart, npart, gout, tmp, lossv;
 };
 
-static int mlp_alloc_train(sam_mlp *m, int batch, int nch)
+static int mlp_alloc_train(sam_mlp *m, int nch)
 {
-    const size_t per = (size_t)(16 + 3 * 64 + 3 * 64 + 16) * sizeof(short);
-    int rc = ensure(m->act, per * batch);
-    if (rc) return rc;
-    rc = ensure(m->gpart, sizeof(float) * (size_t)MLP_NPARAM * nch);
+    int rc = ensure(m->gpart, sizeof(float) * (size_t)MLP_NPARAM * nch);
     if (rc) return rc;
     rc = ensure(m->lpart, sizeof(double) * 1024);
     if (rc) return rc;
     return ensure(m->gout, sizeof(float) * MLP_NPARAM);
-}
-
-static MlpTrainBufs mlp_bufs(sam_mlp *m, int batch)
-{
-    short *b = (short *)m->act.p;
-    MlpTrainBufs t;
-    t.a0 = b;
-    t.a1 = t.a0 + (size_t)16 * batch;
-    t.a2 = t.a1 + (size_t)64 * batch;
-    t.a3 = t.a2 + (size_t)64 * batch;
-    t.dz1 = t.a3 + (size_t)64 * batch;
-    t.dz2 = t.dz1 + (size_t)64 * batch;
-    t.dz3 = t.dz2 + (size_t)64 * batch;
-    t.dout = t.dz3 + (size_t)64 * batch;
-    return t;
-}
-
-static int wgrad_chunks(int batch)
-{
-    int nch = 256;
-    while (nch > 1 && (batch / 16) % nch) nch >>= 1;
-    return nch;
 }
 
 extern "C" int sam_mlp_create(sam_mlp **out, unsigned seed, long long cap)
@@ -1415,7 +1390,7 @@ extern "C" int sam_mlp_destroy(sam_mlp *m)
     if (m->st) hipStreamSynchronize(m->st);
     hipFree(m->P); hipFree(m->norms); hipFree(m->m1); hipFree(m->m2); hipFree(m->img);
     hipFree(m->feats); hipFree(m->labels);
-    for (DevBuf *b : {&m->act, &m->gpart, &m->lpart, &m->npart, &m->gout, &m->tmp, &m->lossv, &m->cdiag}) hipFree(b->p);
+    for (DevBuf *b : {&m->gpart, &m->lpart, &m->npart, &m->gout, &m->tmp, &m->lossv, &m->cdiag}) hipFree(b->p);
     if (m->st) hipStreamDestroy(m->st);
     delete m;
     return SAM_OK;
@@ -1626,17 +1601,12 @@ extern "C" int sam_mlp_fit_norms(sam_mlp *m)
 static int mlp_pass(sam_mlp *m, const float *feats, const float *labels, long long npool, long long base,
                     long long pa, long long pb, int batch, double *loss_dst, int apply, float lr)
 {
-    const int nch = wgrad_chunks(batch);
-    int rc = mlp_alloc_train(m, batch, nch);
+    const int nblk = mlp_step_blocks(batch), nch = nblk;
+    int rc = mlp_alloc_train(m, nch);
     if (rc) return rc;
-    MlpTrainBufs tb = mlp_bufs(m, batch);
-    int nblk = batch / 64;
-    if (nblk > 1024) nblk = 1024;
-    if (nblk < 1) nblk = 1;
-    RCHK(launch_mlp_fb(m->P, m->img, m->norms, feats, labels, npool, base, pa, pb, batch, tb, (double *)m->lpart.p,
-                       nblk, m->activation, m->st));
+    RCHK(launch_mlp_step(m->img, m->norms, feats, labels, npool, base, pa, pb, batch, (float *)m->gpart.p,
+                         (double *)m->lpart.p, m->activation, m->st));
     RCHK(launch_sum_loss((const double *)m->lpart.p, nblk, loss_dst, m->st));
-    RCHK(launch_mlp_wgrad(tb, batch, (float *)m->gpart.p, nch, m->st));
     if (apply) m->step++;
     RCHK(launch_mlp_adam((const float *)m->gpart.p, nch, m->P, m->m1, m->m2, (float *)m->gout.p, lr, m->step > 0 ? m->step : 1,
                          apply, m->st));

@@ -153,15 +153,15 @@ constexpr int MLP_OFF_B3 = MLP_OFF_W3 + 64 * 64;
 constexpr int MLP_OFF_W4 = MLP_OFF_B3 + 64;
 constexpr int MLP_OFF_B4 = MLP_OFF_W4 + 16 * 64;
 constexpr int MLP_NPARAM = MLP_OFF_B4 + 16;
-struct MlpTrainBufs {
-    short *a0, *a1, *a2, *a3, *dz1, *dz2, *dz3, *dout;
-};
 constexpr int MLP_TILE_BYTES = 76 * 64 * 8 + 208 * 4;   // packed BF16 weight tiles + FP32 biases
 int launch_mlp_pack(const float *P, void *img, hipStream_t st);
-int launch_mlp_fb(const float *P, const void *img, const float *norms, const float *feats, const float *labels,
-                  long long npool, long long base, long long pa, long long pb, int batch, const MlpTrainBufs &b,
-                  double *loss_part, int nblk, int act, hipStream_t st);
-int launch_mlp_wgrad(const MlpTrainBufs &b, int batch, float *part, int nchunks, hipStream_t st);
+// one fused training pass (mlp_train.hip): forward, loss, backward and the
+// weight gradients of `batch` samples; writes mlp_step_blocks(batch)
+// gradient rows of MLP_NPARAM floats to part and as many loss partials
+int mlp_step_blocks(int batch);
+int launch_mlp_step(const void *img, const float *norms, const float *feats, const float *labels, long long npool,
+                    long long base, long long pa, long long pb, int batch, float *part, double *loss_part, int act,
+                    hipStream_t st);
 int launch_mlp_adam(const float *part, int nch, float *P, float *m1, float *m2, float *gout, float lr, int step,
                     int apply, hipStream_t st);
 int launch_sum_loss(const double *part, int n, double *out, hipStream_t st);

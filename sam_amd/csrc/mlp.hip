@@ -21,109 +21,15 @@
 // output is multiplied by |x|^2, so the closure is homogeneous of degree 2
 // in the resolved gradients (as gradient-model and eddy-viscosity stresses
 // are) and holds outside the gradient range it was trained on.
-#include "internal.hpp"
+#include "mlp_core.hpp"
 
 namespace samhip {
-
-typedef short v4s __attribute__((ext_vector_type(4)));
-typedef short v8s __attribute__((ext_vector_type(8)));
-typedef float v4f __attribute__((ext_vector_type(4)));
 
 static inline int launched_mlp()
 {
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? SAM_OK : SAM_ERR_HIP;
 }
-
-__device__ __forceinline__ float bf2f(short h) { return __uint_as_float(((unsigned)(unsigned short)h) << 16); }
-
-// a zero the compiler cannot see through: re-basing the LDS weight tiles and
-// bias pointers on it each iteration keeps them from being hoisted out of the
-// per-tile loops into registers
-__device__ __forceinline__ int opaque_zero()
-{
-    int z = 0;
-    asm volatile("" : "+v"(z));
-    return z;
-}
-
-typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ unsigned pk2(float a, float b) { return __builtin_bit_cast(unsigned, v2bf{(__bf16)a, (__bf16)b}); }
-__device__ __forceinline__ v4s bf4(v4f x)
-{
-    typedef unsigned v2u __attribute__((ext_vector_type(2)));
-    return __builtin_bit_cast(v4s, v2u{pk2(x[0], x[1]), pk2(x[2], x[3])});
-}
-
-__device__ __forceinline__ v4f mfma(v4s a, v4s b, v4f c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
-// Two K blocks of 16 in one gfx950 v_mfma_f32_16x16x32_bf16 (twice the work
-// per instruction of the 16x16x16 form).  Lane (i, g) supplies K slots
-// 8g..8g+7: the 4 values it holds for block 0 and then the 4 it holds for
-// block 1.  The K order inside one MFMA is free as long as A and B agree, so
-// the operands are the 16x16x16 lane tiles of the two blocks side by side.
-typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
-__device__ __forceinline__ v4f mfma2(v4s a0, v4s a1, v4s b0, v4s b1, v4f c)
-{
-    const v8s a = __builtin_shufflevector(a0, a1, 0, 1, 2, 3, 4, 5, 6, 7);
-    const v8s b = __builtin_shufflevector(b0, b1, 0, 1, 2, 3, 4, 5, 6, 7);
-    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8bf, a), __builtin_bit_cast(v8bf, b), c, 0, 0, 0);
-}
-
-// |x|^2 of a point's scaled features, held 4 per lane in lanes i, i+16, i+32
-// (lane i+48 holds zeros); every lane of the point gets the sum
-__device__ __forceinline__ float point_r2(v4f x)
-{
-    float r = x[0] * x[0] + x[1] * x[1] + x[2] * x[2] + x[3] * x[3];
-    r += __shfl_xor(r, 16, 64);
-    r += __shfl_xor(r, 32, 64);
-    return r;
-}
-__device__ __forceinline__ float inv_radius(float r2) { return r2 > 0.f ? 1.0f / sqrtf(r2) : 0.f; }
-
-// Hidden activation, ACT = SAM_MLP_ACT_RELU or SAM_MLP_ACT_GELU.
-//
-// ReLU: one instruction per value, a signed max on the bits (a negative
-// float, sign bit set, is a negative integer; fmaxf would also quiet NaNs
-// first).  Derivative 1 for x > 0, else 0.
-//
-// GELU, tanh form: 0.5 x (1 + tanh u) with u = c (x + a x^3), evaluated as
-// x * sigmoid(2u) = x / (1 + 2^(-2u log2 e)): one exp2 and one reciprocal,
-// quarter-rate instructions.  With 48 activations per lane per 16-point tile
-// they take about 40% of the inference kernel's time.  e -> inf for large
-// negative x gives exactly -0; e -> 0 gives x.
-constexpr float GELU_C = 0.7978845608028654f, GELU_A = 0.044715f;
-constexpr float GELU_K1 = -2.0f * GELU_C * 1.4426950408889634f, GELU_K2 = GELU_K1 * GELU_A;
-__device__ __forceinline__ float gelu_sig(float x)
-{
-    const float e = __builtin_amdgcn_exp2f(x * __builtin_fmaf(GELU_K2, x * x, GELU_K1));
-    return __builtin_amdgcn_rcpf(1.0f + e);
-}
-template <int ACT> __device__ __forceinline__ float act(float x)
-{
-    if constexpr (ACT == SAM_MLP_ACT_GELU) return x * gelu_sig(x);
-    else return __int_as_float(max(__float_as_int(x), 0));
-}
-// GELU derivative: s + g (1 - s) 2c (1 + 3 a x^2), s = sigmoid(2u), g = x s
-template <int ACT> __device__ __forceinline__ float act_deriv(float x)
-{
-    if constexpr (ACT == SAM_MLP_ACT_GELU) {
-        const float sg = gelu_sig(x), g = x * sg;
-        return __builtin_fmaf(g * (1.0f - sg), __builtin_fmaf(6.0f * GELU_C * GELU_A, x * x, 2.0f * GELU_C), sg);
-    } else {
-        return x > 0.0f ? 1.0f : 0.0f;
-    }
-}
-
-// ---- weight tiles in LDS ---------------------------------------------
-// tile t (64 lanes x 4 bf16): forward tiles  0..3  W1 (mb),  4..19 W2 (mb*4+kb),
-// 20..35 W3, 36..39 W4 (kb); transposed tiles 40..43 W4^T (mb), 44..59 W3^T,
-// 60..75 W2^T.
-constexpr int T_W1 = 0, T_W2 = 4, T_W3 = 20, T_W4 = 36, T_W4T = 40, T_W3T = 44, T_W2T = 60, T_COUNT = 76;
-
-__device__ __forceinline__ float wparam(const float *P, int off, int ld, int row, int col) { return P[off + row * ld + col]; }
-
-// biases in the packed image, after the tiles (FP32): b1, b2, b3, b4
-constexpr int NB_BIAS = 208, BO_1 = 0, BO_2 = 64, BO_3 = 128, BO_4 = 192;
 
 // the packed image: tile t, lane l at index t*64 + l, then the 208 biases;
 // built once per parameter update (k_pack_tiles) and copied into LDS by
@@ -153,282 +59,23 @@ __global__ __launch_bounds__(256) void k_pack_tiles(const float *__restrict__ P,
         bimg[e] = e < BO_4 ? P[boff[e / 64] + e % 64] : P[MLP_OFF_B4 + e - BO_4];
 }
 
-// copy the first `ntiles` tiles of the packed image and the biases into LDS
-// (16 B per thread)
-__device__ __forceinline__ void copy_consts(const v4s *__restrict__ img, v4s *tiles, int ntiles, v4f *bias)
-{
-    const uint4 *src = reinterpret_cast<const uint4 *>(img);
-    uint4 *dst = reinterpret_cast<uint4 *>(tiles);
-    for (int e = threadIdx.x; e < ntiles * 32; e += blockDim.x) dst[e] = src[e];
-    const uint4 *bs = reinterpret_cast<const uint4 *>(img + T_COUNT * 64);
-    uint4 *bd = reinterpret_cast<uint4 *>(bias);
-    for (int e = threadIdx.x; e < NB_BIAS / 4; e += blockDim.x) bd[e] = bs[e];
-}
-
-// biases u0..u0+3 of the bias block at `off` (LDS)
-__device__ __forceinline__ v4f bias4(const float *bias, int off, int u0)
-{
-    return *reinterpret_cast<const v4f *>(bias + off + u0);
-}
-
-struct Acts {
-    v4s a0;          // inputs (bf16)
-    v4f z[3][4];     // pre-activations of the hidden layers
-    v4s a[3][4];     // hidden activations (bf16)
-    v4f out;         // outputs 4g..4g+3
-};
-
-template <int ACT> __device__ __forceinline__ v4s act_bf4(v4f z)
-{
-    return bf4(v4f{act<ACT>(z[0]), act<ACT>(z[1]), act<ACT>(z[2]), act<ACT>(z[3])});
-}
-
-// Weight-tile sources of the forward pass: tile k of this lane from the LDS
-// image, or from registers loaded once per wavefront (forward tiles only).
-struct LdsTiles {
-    const v4s *t;
-    int lane;
-    __device__ __forceinline__ v4s operator()(int k) const { return t[k * 64 + lane]; }
-};
-struct RegTiles {
-    v4s w[T_W4T];
-    __device__ __forceinline__ v4s operator()(int k) const { return w[k]; }
-};
-
-// one hidden layer 64 -> 64: z = W a + b, tiles tb..tb+15 (mb*4+kb); the
-// four output blocks' MFMA chains are interleaved (kb outer)
-template <int ACT, class TS>
-__device__ __forceinline__ void fw_hidden(const float *bias, const TS &tiles, int tb, int boff, int lane,
-                                          const v4s *ain, v4f *z, v4s *aout)
-{
-    const int g = lane >> 4;
-    v4f acc[4];
-#pragma unroll
-    for (int mb = 0; mb < 4; mb++) acc[mb] = bias4(bias, boff, 16 * mb + 4 * g);
-#pragma unroll
-    for (int kb = 0; kb < 4; kb += 2)
-#pragma unroll
-        for (int mb = 0; mb < 4; mb++)
-            acc[mb] = mfma2(tiles(tb + mb * 4 + kb), tiles(tb + mb * 4 + kb + 1), ain[kb], ain[kb + 1], acc[mb]);
-#pragma unroll
-    for (int mb = 0; mb < 4; mb++) {
-        z[mb] = acc[mb];
-        aout[mb] = act_bf4<ACT>(acc[mb]);
-    }
-}
-
-// forward pass of one 16-point tile; x: this lane's 4 normalised inputs;
-// bias in LDS, weight tiles from `tiles`
-template <int ACT, class TS>
-__device__ __forceinline__ void mlp_forward(const float *bias, const TS &tiles, int lane, v4f x, Acts &A)
-{
-    const int g = lane >> 4;
-    A.a0 = bf4(x);
-    v4f z[4];
-#pragma unroll
-    for (int mb = 0; mb < 4; mb++) z[mb] = mfma(tiles(T_W1 + mb), A.a0, bias4(bias, BO_1, 16 * mb + 4 * g));
-#pragma unroll
-    for (int mb = 0; mb < 4; mb++) {
-        A.z[0][mb] = z[mb];
-        A.a[0][mb] = act_bf4<ACT>(z[mb]);
-    }
-    fw_hidden<ACT, TS>(bias, tiles, T_W2, BO_2, lane, A.a[0], A.z[1], A.a[1]);
-    fw_hidden<ACT, TS>(bias, tiles, T_W3, BO_3, lane, A.a[1], A.z[2], A.a[2]);
-    v4f o = bias4(bias, BO_4, 4 * g);
-#pragma unroll
-    for (int kb = 0; kb < 4; kb += 2)
-        o = mfma2(tiles(T_W4 + kb), tiles(T_W4 + kb + 1), A.a[2][kb], A.a[2][kb + 1], o);
-    A.out = o;
-}
-
-template <int ACT> __device__ __forceinline__ v4f act_grad4(v4f z)
-{
-    return v4f{act_deriv<ACT>(z[0]), act_deriv<ACT>(z[1]), act_deriv<ACT>(z[2]), act_deriv<ACT>(z[3])};
-}
-
-template <int ACT> __device__ __forceinline__ void bw_act(const v4f *z, const v4f *da, v4s *dz)
-{
-#pragma unroll
-    for (int mb = 0; mb < 4; mb++) {
-        const v4f gd = act_grad4<ACT>(z[mb]);
-        dz[mb] = bf4(v4f{da[mb][0] * gd[0], da[mb][1] * gd[1], da[mb][2] * gd[2], da[mb][3] * gd[3]});
-    }
-}
-
-__device__ __forceinline__ void bw_matvec(const v4s *tiles, int tb, int lane, const v4s *dz, v4f *da)
-{
-#pragma unroll
-    for (int mb = 0; mb < 4; mb++) {
-        v4f s = {0, 0, 0, 0};
-#pragma unroll
-        for (int kb = 0; kb < 4; kb += 2)
-            s = mfma2(tiles[(tb + mb * 4 + kb) * 64 + lane], tiles[(tb + mb * 4 + kb + 1) * 64 + lane], dz[kb], dz[kb + 1], s);
-        da[mb] = s;
-    }
-}
-
-// backward of one tile from dL/d(out); returns dz of each hidden layer (bf16)
-template <int ACT>
-__device__ __forceinline__ void mlp_backward(const v4s *tiles, int lane, const Acts &A, v4f dout, v4s dz[3][4],
-                                             v4s &dout_b)
-{
-    dout_b = bf4(dout);
-    v4f da[4];
-#pragma unroll
-    for (int mb = 0; mb < 4; mb++) da[mb] = mfma(tiles[(T_W4T + mb) * 64 + lane], dout_b, v4f{0, 0, 0, 0});
-    bw_act<ACT>(A.z[2], da, dz[2]);
-    bw_matvec(tiles, T_W3T, lane, dz[2], da);
-    bw_act<ACT>(A.z[1], da, dz[1]);
-    bw_matvec(tiles, T_W2T, lane, dz[1], da);
-    bw_act<ACT>(A.z[0], da, dz[0]);
-}
-
-// ---------------------------------------------------------------------
-// training forward+backward over a minibatch of the sample pool
-// pool: feats [p][16] raw (12 used), labels [p][16] raw (9 used)
-// stores (point-major, bf16): a0 [p][16], a1..a3 [p][64], dz1..dz3 [p][64], dout [p][16]
-// ---------------------------------------------------------------------
-struct TrainBufs {
-    short *a0, *a1, *a2, *a3, *dz1, *dz2, *dz3, *dout;
-};
-
-__device__ __forceinline__ void store4(short *base, int ld, int p, int u0, v4s v)
-{
-    *reinterpret_cast<v4s *>(base + (long long)p * ld + u0) = v;
-}
-
-template <int ACT>
-__global__ __launch_bounds__(256) void k_mlp_fb(const float *__restrict__ P, const v4s *__restrict__ img,
-                                                 const float *__restrict__ norms,
-                                                 const float *__restrict__ feats, const float *__restrict__ labels,
-                                                 long long npool, long long base, long long pa, long long pb,
-                                                 int batch, TrainBufs tb, double *__restrict__ loss_part)
-{
-    __shared__ v4s tiles[T_COUNT * 64];
-    __shared__ v4f biasv[NB_BIAS / 4];
-    __shared__ double lred[4];
-    copy_consts(img, tiles, T_COUNT, biasv);
-    __syncthreads();
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, i = lane & 15, g = lane >> 4;
-    const int ntiles = batch / 16;
-    double lsum = 0.0;
-    for (int t = blockIdx.x * 4 + w; t < ntiles; t += gridDim.x * 4) {
-        const int p = t * 16 + i;                                  // position in the batch
-        // sample index: pa, base, pb < npool < 2^31, so the product stays below 2^62
-        const long long s = (pa * ((base + p) % npool) + pb) % npool;
-        v4f x = {0, 0, 0, 0};
-        if (g < 3) {
-#pragma unroll
-            for (int kk = 0; kk < 4; kk++) x[kk] = feats[s * 16 + 4 * g + kk] * norms[4 * g + kk];
-        }
-        const float r2 = point_r2(x), ir = inv_radius(r2), ir2 = ir * ir;
-        x *= ir;
-        const int zo = opaque_zero();
-        const v4s *tl = tiles + zo;
-        Acts A;
-        mlp_forward<ACT>(reinterpret_cast<const float *>(biasv) + zo, LdsTiles{tl, lane}, lane, x, A);
-        // MSE over the 9 normalised outputs: L = mean over batch*9 of (out - y/os)^2
-        v4f dout;
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int o = 4 * g + r;
-            float d = 0.f;
-            if (o < MLP_OUT) {
-                // scale-free target; a column that is zero over the pool (the scalar
-                // flux of a flow without temperature) has scale 0 and target 0
-                const float os = norms[16 + o];
-                const float y = os > 0.f ? labels[s * 16 + o] / os * ir2 : 0.f;
-                d = A.out[r] - y;
-                lsum += (double)d * d;
-            }
-            dout[r] = 2.0f * d / (float)(batch * MLP_OUT);
-        }
-        v4s dz[3][4], dob;
-        mlp_backward<ACT>(tl, lane, A, dout, dz, dob);
-        store4(tb.a0, 16, p, 4 * g, A.a0);
-        store4(tb.dout, 16, p, 4 * g, dob);
-#pragma unroll
-        for (int mb = 0; mb < 4; mb++) {
-            store4(tb.a1, 64, p, 16 * mb + 4 * g, A.a[0][mb]);
-            store4(tb.a2, 64, p, 16 * mb + 4 * g, A.a[1][mb]);
-            store4(tb.a3, 64, p, 16 * mb + 4 * g, A.a[2][mb]);
-            store4(tb.dz1, 64, p, 16 * mb + 4 * g, dz[0][mb]);
-            store4(tb.dz2, 64, p, 16 * mb + 4 * g, dz[1][mb]);
-            store4(tb.dz3, 64, p, 16 * mb + 4 * g, dz[2][mb]);
-        }
-    }
-    for (int o = 32; o > 0; o >>= 1) lsum += __shfl_xor(lsum, o, 64);
-    if (lane == 0) lred[w] = lsum;
-    __syncthreads();
-    if (threadIdx.x == 0) loss_part[blockIdx.x] = lred[0] + lred[1] + lred[2] + lred[3];
-}
-
-// ---------------------------------------------------------------------
-// weight gradients: dW[i][j] = sum_p dz[p][i] a[p][j], db[i] = sum_p dz[p][i]
-// (the bias as one more N block of ones).  One wavefront per chunk of points.
-// ---------------------------------------------------------------------
-template <int MB, int NB>
-__device__ void wgrad_layer(const short *__restrict__ dz, int ldz, const short *__restrict__ a, int lda, int p0, int np,
-                            float *__restrict__ outW, float *__restrict__ outB, int ldw)
-{
-    const int lane = threadIdx.x & 63, i = lane & 15, g = lane >> 4;
-    v4f acc[MB][NB + 1];
-#pragma unroll
-    for (int mb = 0; mb < MB; mb++)
-#pragma unroll
-        for (int nb = 0; nb <= NB; nb++) acc[mb][nb] = v4f{0, 0, 0, 0};
-    const short one = (short)0x3f80;
-    const v4s ones = {one, one, one, one};
-    for (int k0 = p0; k0 < p0 + np; k0 += 16) {
-        v4s A[MB], B[NB];
-#pragma unroll
-        for (int mb = 0; mb < MB; mb++)
-            for (int kk = 0; kk < 4; kk++) A[mb][kk] = dz[(long long)(k0 + 4 * g + kk) * ldz + 16 * mb + i];
-#pragma unroll
-        for (int nb = 0; nb < NB; nb++)
-            for (int kk = 0; kk < 4; kk++) B[nb][kk] = a[(long long)(k0 + 4 * g + kk) * lda + 16 * nb + i];
-#pragma unroll
-        for (int mb = 0; mb < MB; mb++) {
-#pragma unroll
-            for (int nb = 0; nb < NB; nb++) acc[mb][nb] = mfma(A[mb], B[nb], acc[mb][nb]);
-            acc[mb][NB] = mfma(A[mb], ones, acc[mb][NB]);
-        }
-    }
-#pragma unroll
-    for (int mb = 0; mb < MB; mb++) {
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int row = 16 * mb + 4 * g + r;
-#pragma unroll
-            for (int nb = 0; nb < NB; nb++) outW[row * ldw + 16 * nb + i] = acc[mb][nb][r];
-            if (i == 0) outB[row] = acc[mb][NB][r];
-        }
-    }
-}
-
-// grid: nchunks blocks x 4 waves; wave w handles layer w of chunk blockIdx.x
-__global__ __launch_bounds__(256) void k_mlp_wgrad(TrainBufs tb, int batch, float *__restrict__ part)
-{
-    const int nch = gridDim.x;
-    const int np = batch / nch;
-    const int p0 = blockIdx.x * np;
-    float *out = part + (long long)blockIdx.x * MLP_NPARAM;
-    const int w = threadIdx.x >> 6;
-    if (w == 0) wgrad_layer<4, 1>(tb.dz1, 64, tb.a0, 16, p0, np, out + MLP_OFF_W1, out + MLP_OFF_B1, 16);
-    if (w == 1) wgrad_layer<4, 4>(tb.dz2, 64, tb.a1, 64, p0, np, out + MLP_OFF_W2, out + MLP_OFF_B2, 64);
-    if (w == 2) wgrad_layer<4, 4>(tb.dz3, 64, tb.a2, 64, p0, np, out + MLP_OFF_W3, out + MLP_OFF_B3, 64);
-    if (w == 3) wgrad_layer<1, 4>(tb.dout, 16, tb.a3, 64, p0, np, out + MLP_OFF_W4, out + MLP_OFF_B4, 64);
-}
-
 // sum the per-chunk gradients (fixed order) and either return them or take an Adam step
+// grid: ceil(MLP_NPARAM / 64) blocks x 4 wavefronts; lane -> parameter,
+// wavefront w sums rows w, w + 4, ... and the four sums are added in order
 __global__ __launch_bounds__(256) void k_mlp_adam(const float *__restrict__ part, int nch, float *__restrict__ P,
                                                    float *__restrict__ m1, float *__restrict__ m2, float *__restrict__ gout,
                                                    float lr, float b1, float b2, float eps, int step, int apply)
 {
-    const int j = blockIdx.x * 256 + threadIdx.x;
-    if (j >= MLP_NPARAM) return;
-    float gsum = 0.f;
-    for (int c = 0; c < nch; c++) gsum += part[(long long)c * MLP_NPARAM + j];
+    __shared__ float ws[4][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int j = blockIdx.x * 64 + lane;
+    float ps = 0.f;
+    if (j < MLP_NPARAM)
+        for (int c = w; c < nch; c += 4) ps += part[(long long)c * MLP_NPARAM + j];
+    ws[w][lane] = ps;
+    __syncthreads();
+    if (w != 0 || j >= MLP_NPARAM) return;
+    float gsum = ((ws[0][lane] + ws[1][lane]) + ws[2][lane]) + ws[3][lane];
     // padded rows/columns of W1 (inputs 12..15) and W4/b4 (outputs 9..15) stay zero
     const bool pad = (j >= MLP_OFF_W1 && j < MLP_OFF_B1 && (j - MLP_OFF_W1) % 16 >= MLP_IN) ||
                      (j >= MLP_OFF_W4 && j < MLP_OFF_B4 && (j - MLP_OFF_W4) / 64 >= MLP_OUT) ||
@@ -769,31 +416,10 @@ int launch_mlp_pack(const float *P, void *img, hipStream_t st)
     return launched_mlp();
 }
 
-int launch_mlp_fb(const float *P, const void *img, const float *norms, const float *feats, const float *labels,
-                  long long npool, long long base, long long pa, long long pb, int batch, const MlpTrainBufs &b,
-                  double *loss_part, int nblk, int act, hipStream_t st)
-{
-    TrainBufs tb{b.a0, b.a1, b.a2, b.a3, b.dz1, b.dz2, b.dz3, b.dout};
-    if (act == SAM_MLP_ACT_GELU)
-        hipLaunchKernelGGL(k_mlp_fb<SAM_MLP_ACT_GELU>, dim3(nblk), dim3(256), 0, st, P, (const v4s *)img, norms, feats,
-                           labels, npool, base, pa, pb, batch, tb, loss_part);
-    else
-        hipLaunchKernelGGL(k_mlp_fb<SAM_MLP_ACT_RELU>, dim3(nblk), dim3(256), 0, st, P, (const v4s *)img, norms, feats,
-                           labels, npool, base, pa, pb, batch, tb, loss_part);
-    return launched_mlp();
-}
-
-int launch_mlp_wgrad(const MlpTrainBufs &b, int batch, float *part, int nchunks, hipStream_t st)
-{
-    TrainBufs tb{b.a0, b.a1, b.a2, b.a3, b.dz1, b.dz2, b.dz3, b.dout};
-    hipLaunchKernelGGL(k_mlp_wgrad, dim3(nchunks), dim3(256), 0, st, tb, batch, part);
-    return launched_mlp();
-}
-
 int launch_mlp_adam(const float *part, int nch, float *P, float *m1, float *m2, float *gout, float lr, int step,
                     int apply, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_mlp_adam, dim3((MLP_NPARAM + 255) / 256), dim3(256), 0, st, part, nch, P, m1, m2, gout, lr,
+    hipLaunchKernelGGL(k_mlp_adam, dim3((MLP_NPARAM + 63) / 64), dim3(256), 0, st, part, nch, P, m1, m2, gout, lr,
                        0.9f, 0.999f, 1e-8f, step, apply);
     return launched_mlp();
 }
