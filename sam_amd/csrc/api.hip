@@ -181,23 +181,30 @@ static int ensure(DevBuf &b, size_t bytes)
     return SAM_OK;
 }
 
+// exp(-2 pi i k / n), k < n, followed by the half-length table exp(-2 pi i k
+// / (n/2)), k < n/2 (the x rows of a 1024 grid run as 512-point transforms)
 static int make_twiddles(sam_handle *h, int dir, int n)
 {
-    std::vector<double> td(2 * (size_t)n);
-    std::vector<float> tf(2 * (size_t)n);
+    const size_t tot = (size_t)n + (size_t)n / 2;
+    std::vector<double> td(2 * tot);
+    std::vector<float> tf(2 * tot);
     const long double pi = 3.14159265358979323846264338327950288L;
-    for (int k = 0; k < n; k++) {
-        const long double a = -2.0L * pi * (long double)k / (long double)n;
-        const long double c = cosl(a), s = sinl(a);
-        td[2 * k] = (double)c;
-        td[2 * k + 1] = (double)s;
-        tf[2 * k] = (float)c;
-        tf[2 * k + 1] = (float)s;
-    }
-    HIPCHK(hipMalloc(&h->tw[dir][0], sizeof(double) * 2 * n));
-    HIPCHK(hipMalloc(&h->tw[dir][1], sizeof(float) * 2 * n));
-    HIPCHK(hipMemcpy(h->tw[dir][0], td.data(), sizeof(double) * 2 * n, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(h->tw[dir][1], tf.data(), sizeof(float) * 2 * n, hipMemcpyHostToDevice));
+    auto fill = [&](size_t off, int len) {
+        for (int k = 0; k < len; k++) {
+            const long double a = -2.0L * pi * (long double)k / (long double)len;
+            const long double c = cosl(a), s = sinl(a);
+            td[2 * (off + k)] = (double)c;
+            td[2 * (off + k) + 1] = (double)s;
+            tf[2 * (off + k)] = (float)c;
+            tf[2 * (off + k) + 1] = (float)s;
+        }
+    };
+    fill(0, n);
+    fill(n, n / 2);
+    HIPCHK(hipMalloc(&h->tw[dir][0], sizeof(double) * 2 * tot));
+    HIPCHK(hipMalloc(&h->tw[dir][1], sizeof(float) * 2 * tot));
+    HIPCHK(hipMemcpy(h->tw[dir][0], td.data(), sizeof(double) * 2 * tot, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->tw[dir][1], tf.data(), sizeof(float) * 2 * tot, hipMemcpyHostToDevice));
     return SAM_OK;
 }
 

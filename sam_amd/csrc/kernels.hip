@@ -142,6 +142,9 @@ __global__ __launch_bounds__(C * N / EPT) void k_yb(const T *__restrict__ Zr, T 
     const int ix = b % d.nxm, f = b / d.nxm;
     const int z0 = zt * C;
     const cpx<T> *Z = reinterpret_cast<const cpx<T> *>(Zr) + ((long long)f * d.nxm + ix) * d.nym * d.nz + z0;
+    // half-line rows (64 bytes): default-policy loads keep the line for the
+    // neighbouring tile, which the XCD order runs on the same XCD
+    constexpr bool HALF = C * (int)sizeof(cpx<T>) < 128;
     // EPT items per thread: all loads issued before the first LDS write
     cpx<T> x[EPT];
 #pragma unroll
@@ -149,7 +152,7 @@ __global__ __launch_bounds__(C * N / EPT) void k_yb(const T *__restrict__ Zr, T 
         const int e = threadIdx.x + s * NT, p = e / C, c = e % C;
         const int iy = (p <= d.kym) ? p : ((p >= N - d.kym) ? p - N + d.nym : -1);
         x[s] = {(T)0, (T)0};
-        if (iy >= 0) x[s] = ld_nt(Z + (long long)iy * d.nz + c);
+        if (iy >= 0) x[s] = HALF ? Z[(long long)iy * d.nz + c] : ld_nt(Z + (long long)iy * d.nz + c);
     }
 #pragma unroll
     for (int s = 0; s < EPT; s++) {
@@ -192,7 +195,11 @@ __global__ __launch_bounds__(C * N / EPT) void k_yf(const T *__restrict__ Yr, T 
     constexpr int NT = C * N / EPT;
     __shared__ cpx<T> lds[C * N];
     const int nzt = d.nz / C;
-    int b = sizeof(T) == 8 ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+    // half-line rows (64 bytes): the XCD order puts neighbouring tiles, which
+    // write the two halves of each line, on one XCD, and default-policy stores
+    // let the halves merge in its L2
+    constexpr bool HALF = C * (int)sizeof(cpx<T>) < 128;
+    int b = (sizeof(T) == 8 || HALF) ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x;
     const int zt = b % nzt;
     b /= nzt;
     const int ix = b % d.nxm, q = b / d.nxm;
@@ -218,15 +225,19 @@ __global__ __launch_bounds__(C * N / EPT) void k_yf(const T *__restrict__ Yr, T 
 #pragma unroll
     for (int s = 0; s < EPT; s++) {
         const int e = threadIdx.x + s * NT;
-        if (e < C * d.nym) st_nt(Z + (long long)(e / C) * d.nz + e % C, o[s]);
+        if (e < C * d.nym) {
+            cpx<T> *dst = Z + (long long)(e / C) * d.nz + e % C;
+            if constexpr (HALF) *dst = o[s];
+            else st_nt(dst, o[s]);
+        }
     }
 }
 
 template <typename T>
 int launch_yf(const Dims &d, const T *Y, int nvar, T *Z, const void *twy, hipStream_t st)
 {
-    constexpr int C = Row128<T>::C;
     SAM_SWITCH_N(d.ny, {
+        constexpr int C = yb_cols<T, N>();
         const unsigned nb = (unsigned)((long long)nvar * d.nxm * (d.nz / C));
         if constexpr (C * N / 8 <= 512)
             hipLaunchKernelGGL((k_yf<T, N, C, 8>), dim3(nb), dim3(C * N / 8), 0, st, Y, Z, (const cpx<T> *)twy, d);
@@ -329,6 +340,14 @@ __device__ __forceinline__ void lds_dma16(const void *gsrc, const void *lds_dst)
 {
     const unsigned m0 = __builtin_amdgcn_readfirstlane((unsigned)(uintptr_t)(const lds_void_t *)lds_dst);
     asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off nt" ::"v"(gsrc), "s"(m0)
+                 : "memory", "m0");
+}
+// the same with the default cache policy: for data whose other half-line a
+// neighbouring block reads soon after (64-byte rows)
+__device__ __forceinline__ void lds_dma16_keep(const void *gsrc, const void *lds_dst)
+{
+    const unsigned m0 = __builtin_amdgcn_readfirstlane((unsigned)(uintptr_t)(const lds_void_t *)lds_dst);
+    asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(gsrc), "s"(m0)
                  : "memory", "m0");
 }
 
@@ -766,10 +785,203 @@ __global__ __launch_bounds__(CP * N / 8, xrow_waves<T>()) void k_xrow(T *__restr
     xrow_products<T, N, CP, LU, LR, NT >= 64, DS>(lds, stg, tw, uf, Y + zoff, fstride, d, y0, c, j0);
 }
 
+// ---------------------------------------------------------------------
+// x rows of length N as M = N/2-point complex transforms, one real row each
+// (FP32 at N = 1024).  The pair form above transforms two rows per N-point
+// column; at N = 1024 that column spans two wavefronts (workgroup barriers
+// in every exchange, four passes) and its FP32 16-element form would not fit
+// the registers.  Here a row is one M-point column of one wavefront (three
+// passes, wavefront-level exchanges), with w = exp(-2 pi i / N):
+//   inverse (X_k, k <= kx_max, X_0 real, zero above):
+//     Z_k = E_k + i O_k,  E_k = X_k + conj X_{M-k},  O_k = (X_k - conj X_{M-k}) conj(w^k),
+//     z_n = sum_k Z_k exp(+2 pi i k n / M) = x_{2n} + i x_{2n+1};
+//   forward (Z = the M-point transform of z_n = x_{2n} + i x_{2n+1}, Z_M = Z_0):
+//     X_k = (Z_k + conj Z_{M-k})/2 - i w^k (Z_k - conj Z_{M-k})/2.
+// Products act on the real values, so the interleaved even/odd samples in
+// (re, im) need no reordering.  A block takes R = 8 rows (64-byte k-rows).
+// ---------------------------------------------------------------------
+template <int N> constexpr int xrowh_srows() { return ((N / 3 + 1) + 15) / 16 * 16; }   // staged k-rows, whole DMA blocks
+
+// Stage k-rows 0..kx_max of R = 8 rows (64 bytes each) into LDS.  One DMA
+// instruction moves 16 k-rows; lane l takes 16-byte chunk q = l/16 (rows
+// 2q, 2q+1) of k-row l%16, so within the 1 KB block chunk q of all 16 k-rows
+// is contiguous: a wavefront then reads 16 consecutive k-rows of one row at
+// a 16-byte stride (two cycles per 16 lanes, no further conflict).
+template <typename T, int N, int NT>
+__device__ __forceinline__ void xrowh_issue_stage(cpx<T> *stg, const cpx<T> *__restrict__ Yf, int ny, int y0, int kxm)
+{
+    static_assert(sizeof(cpx<T>) == 8, "64-byte k-rows of 8 FP32 complex values");
+    constexpr int NI = xrowh_srows<N>() / 16, NW = NT / 64;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int i = w; i < NI; i += NW) {
+        int k = 16 * i + (lane & 15);
+        if (k > kxm) k = kxm;      // rows past kx_max: re-read a valid row, ignored
+        const char *src = reinterpret_cast<const char *>(Yf + (long long)k * ny + y0) + (lane >> 4) * 16;
+        lds_dma16_keep(src, stg + i * 128);
+    }
+}
+template <int N, int NT> __device__ __forceinline__ int xrowh_stage_count(int w)
+{
+    constexpr int NI = xrowh_srows<N>() / 16, NW = NT / 64;
+    return w < NI ? (NI - w + NW - 1) / NW : 0;
+}
+// element (k-row r, row c) of a staged field
+template <typename T> __device__ __forceinline__ cpx<T> xrowh_at(const cpx<T> *stg, int r, int c)
+{
+    return stg[(r >> 4) * 128 + (c >> 1) * 32 + (r & 15) * 2 + (c & 1)];
+}
+
+// pass-0 inputs Z_k of row c (slot s: k = j0 + TPC kin(0, s)); wk[m] = w^{j0 + TPC m}
+template <typename T, int N>
+__device__ __forceinline__ void xrowh_pass0(const cpx<T> *stg, int c, int j0, int kxm, const cpx<T> (&wk)[8], cpx<T> *v)
+{
+    constexpr int M = N / 2;
+    using TL = Tile<T, M, 8>;
+    constexpr int TPC = TL::TPC, KXM = N / 3;
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+        const int lo = TPC * TL::kin(0, s), k = j0 + lo;
+        // compile-time bands: X_k present when k <= kx_max, X_{M-k} when M - k <= kx_max
+        const bool xk_all = lo + TPC - 1 <= KXM, xk_none = lo > KXM;
+        const bool xm_all = lo >= M - KXM && lo > 0, xm_none = lo + TPC - 1 < M - KXM;
+        cpx<T> X = {(T)0, (T)0}, Y = {(T)0, (T)0};
+        if (!xk_none && (xk_all || k <= kxm)) {
+            X = xrowh_at(stg, k, c);
+            if (lo == 0 && k == 0) X.y = (T)0;     // kx = 0: real part only (xy_trans_b.f:28-31)
+        }
+        if (!xm_none && (xm_all || (k > 0 && M - k <= kxm))) Y = xrowh_at(stg, M - k, c);
+        const cpx<T> E = {X.x + Y.x, X.y - Y.y}, D = {X.x - Y.x, X.y + Y.y};
+        const cpx<T> w = wk[TL::kin(0, s)];
+        const cpx<T> O = {D.x * w.x + D.y * w.y, D.y * w.x - D.x * w.y};   // D conj(w)
+        v[s] = {E.x - O.y, E.y + O.x};
+    }
+}
+
+// forward outputs of the R rows (in the tile, natural order) -> X_k, k <= kx_max,
+// scaled by h, stored as 64-byte k-rows; twl: w^k, k < M, in LDS
+template <typename T, int N, int R, int NT>
+__device__ __forceinline__ void xrowh_store(const cpx<T> *lds, const cpx<T> *twl, cpx<T> *__restrict__ Yq, int ny,
+                                            int y0, int kxm, T h)
+{
+    constexpr int M = N / 2, RP = R / 2;
+    const int nitem = RP * (kxm + 1);
+    // item: k-row k, rows 2cp and 2cp + 1 (one 16-byte store)
+    for (int e = threadIdx.x; e < nitem; e += NT) {
+        const int k = e / RP, c0 = 2 * (e % RP);
+        const cpx<T> w = twl[k];
+        T o[4];
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            const cpx<T> zk = lds[lds_idx<T, M>(c0 + j, k)], zm = lds[lds_idx<T, M>(c0 + j, (M - k) & (M - 1))];
+            const cpx<T> A = {zk.x + zm.x, zk.y - zm.y}, D = {zk.x - zm.x, zk.y + zm.y};   // Z_k + conj Z_{M-k}, Z_k - conj Z_{M-k}
+            const cpx<T> wd = cmul(w, D);
+            o[2 * j] = (A.x + wd.y) * h;
+            o[2 * j + 1] = (A.y - wd.x) * h;
+        }
+        // default policy: the partner block writes the other half of the line
+        // at about the same time, and the halves merge in the L2
+        typedef T v4 __attribute__((ext_vector_type(4)));
+        *reinterpret_cast<v4 *>(Yq + (long long)k * ny + y0 + c0) = v4{o[0], o[1], o[2], o[3]};
+    }
+}
+
+// Persistent half-length x-row kernel (dealiased rows): as k_xrow_p, with
+// row groups of R rows, the M-point transforms and the conversions above.
+template <typename T, int N, int R, int LU, int LR>
+__global__ __launch_bounds__(R * (N / 2) / 8, 2) void k_xrow_h(T *__restrict__ Yr, Dims d,
+                                                             const cpx<T> *__restrict__ twn,
+                                                             const cpx<T> *__restrict__ twm, int want_stats,
+                                                             double *__restrict__ partials, int nblk)
+{
+    constexpr int M = N / 2;
+    using TL = Tile<T, M, 8>;
+    constexpr int NT = R * M / 8, NW = NT / 64;
+    static_assert(TL::TPC == 64 && R == NW, "one wavefront per row");
+    constexpr int TILE = R * M, SR = xrowh_srows<N>();
+    __shared__ cpx<T> smem[TILE + 2 * SR * R];
+    __shared__ cpx<T> twl[M];
+    __shared__ double red[NW][3 * LU];
+    cpx<T> *const lds = smem;
+    auto stage = [&](int i) { return smem + TILE + (i & 1) * (SR * R); };
+    const int nyt = d.ny / R;
+    const int c = threadIdx.x / TL::TPC, j0 = threadIdx.x % TL::TPC;
+    const int mine = xrowh_stage_count<N, NT>(threadIdx.x >> 6);
+    for (int e = threadIdx.x; e < M; e += NT) twl[e] = twn[e];
+    typename TL::TwRegs tw;
+    TL::tw_regs(twm, j0, tw);
+    cpx<T> wk[8];
+#pragma unroll
+    for (int m = 0; m < 8; m++) wk[m] = twn[j0 + TL::TPC * m];
+    cpx<T> *Y = reinterpret_cast<cpx<T> *>(Yr);
+    const long long fstride = d.yfield;
+    const T h = (T)(0.5 / ((double)d.nx * (double)d.ny));
+    auto zoff = [&](int b) { return (long long)(b / nyt) * d.nxm * d.ny; };
+    auto yof = [&](int b) { return (b % nyt) * R; };
+    // Row groups 2j and 2j + 1 share their 128-byte k-row lines.  Blocks B
+    // and B + 8 run on one XCD (blocks are dealt to the 8 XCDs round robin);
+    // they take the two halves of the same line pairs in step, so one block's
+    // staged reads find the lines the other has just brought into the L2.
+    // Each block pair walks a contiguous range of line pairs.  The grid is a
+    // multiple of 16 blocks and nblk is even.
+    const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+    const int half = slot & 1, pair = (slot >> 1) * 8 + xcd;
+    const int npair = gridDim.x / 2, ngp = nblk / 2;
+    const int per = (ngp + npair - 1) / npair;
+    const int gp_end = min(ngp, (pair + 1) * per);
+    int gp = pair * per;
+
+    int b = 2 * gp + half;
+    if (gp < gp_end) {
+        xrowh_issue_stage<T, N, NT>(stage(0), Y + zoff(b), d.ny, yof(b), d.kxm);
+        if (LU > 1) xrowh_issue_stage<T, N, NT>(stage(1), Y + zoff(b) + fstride, d.ny, yof(b), d.kxm);
+    }
+    int par = 0;   // field f of the current group sits in stage(f + par)
+    for (; gp < gp_end; gp++, b += 2) {
+        const int bn = b + 2;
+        const bool has_next = gp + 1 < gp_end;
+        const int y0 = yof(b);
+        cpx<T> *const Yz = Y + zoff(b);
+        cpx<T> uf[LU][8];
+#pragma unroll
+        for (int f = 0; f < LU; f++) {
+            if (f + 1 < LU || has_next) wait_vmem_le(mine);
+            else wait_vmem();
+            lds_barrier();
+            cpx<T> *const sf = stage(f + par);
+            xrowh_pass0<T, N>(sf, c, j0, d.kxm, wk, uf[f]);
+            const int g = f + 2;   // next item into the buffer just read
+            if (g < LU || (has_next && g < LU + 2)) {
+                lds_barrier();
+                if (g < LU) xrowh_issue_stage<T, N, NT>(sf, Yz + g * fstride, d.ny, y0, d.kxm);
+                else xrowh_issue_stage<T, N, NT>(sf, Y + zoff(bn) + (g - LU) * fstride, d.ny, yof(bn), d.kxm);
+            }
+            TL::template run_regs<+1>(lds, tw, c, j0, uf[f]);
+        }
+        if (want_stats) xrow_stats<T, NT, LU, 8>(uf, red, partials + (long long)b * 3 * LU);
+#pragma unroll
+        for (int q = 0; q < LR; q++) {
+            cpx<T> po[8], pi[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const cpx<T> a = uf[prod_i(q)][k], bb = uf[prod_j(q)][k];
+                po[k] = {a.x * bb.x, a.y * bb.y};
+            }
+            TL::out_to_in(po, pi);
+            lds_barrier();       // the previous product's stores have read the tile
+            TL::template run_regs<-1>(lds, tw, c, j0, pi);
+            TL::store_natural(lds, c, j0, pi);
+            lds_barrier();
+            xrowh_store<T, N, R, NT>(lds, twl, Yz + q * fstride, d.ny, y0, d.kxm, h);
+        }
+        par ^= LU & 1;
+    }
+}
+
 template <typename T> struct XCP { static constexpr int CP = Row128<T>::C / 2; };
 // column pairs per x-row block: 128-byte k-rows, except FP32 at N >= 1024,
 // where 8 pairs (1024 threads) would cap the kernel at 128 VGPRs and spill;
-// 4 pairs (64-byte k-rows) keep 512 threads
+// 4 pairs (64-byte k-rows) keep 512 threads.  (Dealiased FP32 rows at
+// N = 1024 take the half-length kernel k_xrow_h instead.)
 template <typename T, int N> constexpr int xrow_cp() { return (sizeof(T) == 4 && N >= 1024) ? 4 : XCP<T>::CP; }
 
 template <typename T>
@@ -777,6 +989,22 @@ int launch_xrow(const Dims &d, T *Y, int want_stats, double *partials, DevDiag *
                 int *nblocks_out)
 {
     SAM_SWITCH_N(d.nx, {
+        if constexpr (sizeof(T) == 4 && N == 1024) {
+            // FP32 at N = 1024, dealiased rows: the half-length form, 8 rows per block
+            constexpr int R = 8, NT = R * (N / 2) / 8;
+            static_assert((R * (N / 2) + 2 * xrowh_srows<N>() * R + N / 2) * sizeof(cpx<T>) +
+                              (NT / 64) * 12 * sizeof(double) <= 160 * 1024, "x-row LDS");
+            if (d.kxm + 1 == N / 3 + 1 && d.ny % (2 * R) == 0) {
+                const unsigned nbh = (unsigned)((long long)d.nz * (d.ny / R));
+                if (nblocks_out) *nblocks_out = (int)nbh;
+                auto kern = d.lu == 4 ? k_xrow_h<T, N, R, 4, 9> : k_xrow_h<T, N, R, 3, 6>;
+                // block pairs on one XCD: a multiple of 16 blocks
+                const unsigned ng = (persistent_grid((const void *)kern, NT, nbh) + 15) / 16 * 16;
+                const cpx<T> *twn = (const cpx<T> *)twx;
+                hipLaunchKernelGGL(kern, dim3(ng), dim3(NT), 0, st, Y, d, twn, twn + N, want_stats, partials, (int)nbh);
+                return launched();
+            }
+        }
         constexpr int CP = xrow_cp<T, N>();
         const unsigned nb = (unsigned)((long long)d.nz * (d.ny / (2 * CP)));
         if (nblocks_out) *nblocks_out = (int)nb;

@@ -219,8 +219,9 @@ def test_double_staged_x_rows(oracle_mod, prec, n, tol):
 
 @pytest.mark.parametrize("prec,tol", [(S.PREC_FP64, TOL64), (S.PREC_SPLIT, 1e-5)], ids=["fp64", "split"])
 def test_long_x_rows(oracle_mod, prec, tol):
-    """nx = 1024 (FP32 takes 64-byte k-rows, 4 column pairs per block) on a
-    thin 1024 x 32 x 32 box: full stratified steps against the oracle"""
+    """nx = 1024 (FP32: each real row as one 512-point complex transform,
+    8 rows per block, k_xrow_h) on a thin 1024 x 32 x 32 box: full
+    stratified steps against the oracle"""
     nsteps = 2
     o, s = pair(oracle_mod, 1024, 32, 32, xl=64.0, yl=2.0, zl=2.0, i_strat=1, grav=16.0, lapse0=1.0, vis=5e-3,
                 i_cfl=1, cfl0=0.5, prec=prec)
