@@ -101,6 +101,7 @@ def ref():
         R.ref_vel_max.argtypes = [_dp, ctypes.c_double, _dp]
         R.ref_spectra.argtypes = [_dp, _dp, _dp]
         R.ref_div_check.argtypes = [_dp, _dp]
+        R.ref_cl_spectra.argtypes = [_dp, _dp, _dp]
         _ref = R
     return _ref
 
@@ -441,6 +442,17 @@ class ReferencePhys:
         return dict(ek=out[:lm], tk=out[lm:2 * lm], dk=out[2 * lm:3 * lm], vt=out[3 * lm:4 * lm],
                     sample=out[4 * lm:5 * lm], energy_w=out[5 * lm], div_rms=out[5 * lm + 1],
                     div_max=out[5 * lm + 2], n_div=out[5 * lm + 3])
+
+    def cl_spectra(self, u, r):
+        """cl_energy.f, then spectra.f with the Chollet-Lesieur model (i_les = 1)"""
+        uf = self._full(u, self.lu)
+        rf = self._full(r, self.lr) if r.shape[3] >= self.lr else self._full(
+            np.concatenate([r, np.zeros(r.shape[:3] + (self.lr - r.shape[3],), np.complex128)], axis=3), self.lr)
+        lm = self.lm
+        out = np.zeros(5 * lm + 6)
+        self.R.ref_cl_spectra(_p(uf), _p(rf), _p(out))
+        return dict(ek=out[:lm], tk=out[lm:2 * lm], dk=out[2 * lm:3 * lm], vt=out[3 * lm:4 * lm],
+                    sample=out[4 * lm:5 * lm], energy_w=out[5 * lm], e_cl=out[5 * lm + 4], k_cl=out[5 * lm + 5])
 
     def div_check(self, u):
         uf = self._full(u, self.lu)

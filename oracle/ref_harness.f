@@ -304,6 +304,35 @@ c energy_w div_rms div_max n_div
       return
       end
 
+c cl_energy.f and spectra.f with the Chollet-Lesieur model, as sam.f:128-132
+c and spectra.f:21,39-43 run them for i_les = 1 (init.f:88-93 sets the
+c model's constants): out = as ref_spectra, then E_cl, k_cl.  The setup
+c of ref_setup_phys (i_les = 0) is restored afterwards.
+      subroutine ref_cl_spectra( u, r, out ) bind(c)
+      use iso_c_binding
+      include 'sam.h'
+      complex u(Nze,Ny_min,Nx_min,Lu), r(Nze,Ny_min,Nx_min,Lr)
+      real(c_double) out(*)
+      real, allocatable :: work(:)
+      integer ierr_l
+      allocate( work(Lm*12) )
+      i_les = 1
+      call init( .false., ierr_l )
+      lapse = lapse0
+      call cl_energy( u )
+      call spectra( u, r, out(1+4*Lm), out(1), out(1+Lm),
+     &              out(1+2*Lm), out(1+3*Lm), out(1+5*Lm),
+     &              out(2+5*Lm), out(3+5*Lm), n_div, 0, work )
+      out(4+5*Lm) = n_div
+      out(5+5*Lm) = E_cl
+      out(6+5*Lm) = k_cl
+      i_les = 0
+      call init( .false., ierr_l )
+      lapse = lapse0
+      deallocate( work )
+      return
+      end
+
 c div_check.f of u: out = div_max div_rms n_div
       subroutine ref_div_check( u, out ) bind(c)
       use iso_c_binding

@@ -61,9 +61,10 @@ def transforms_fixture(name, nx, ny, nz, xl, yl, zl, n_dealias, i_strat, seed):
 
 def rhs_spectra_fixture(name, n, i_strat, i_cfl, seed):
     """rhs.f (first substep, with vel_max.f), spectra.f and div_check.f of
-    the reference on a seeded solenoidal state, and spectra.f/div_check.f
-    on a state with divergence (a random field), so that div_rms, div_max
-    and n_div are not zero"""
+    the reference on a seeded solenoidal state, spectra.f/div_check.f on a
+    state with divergence (a random field), so that div_rms, div_max and
+    n_div are not zero, and cl_energy.f + spectra.f with the
+    Chollet-Lesieur model (i_les = 1) on the solenoidal state"""
     from sam_amd import ic
     cfg = O.make_cfg(n, i_strat=i_strat, grav=16.0, lapse0=1.0, vis=5e-3, i_cfl=i_cfl, cfl0=0.5, dt0=0.01)
     R = O.ReferencePhys(cfg)
@@ -79,6 +80,7 @@ def rhs_spectra_fixture(name, n, i_strat, i_cfl, seed):
     w[:, :, 0] = u[:, :, 0]          # the kx = 0 plane stays Hermitian
     spw = R.spectra(w, r)
     dvw = R.div_check(w)
+    cl = R.cl_spectra(u, r)      # i_les = 1: cl_energy.f, then spectra.f with nu_t(k)
     keys = ("ek", "tk", "dk", "vt", "sample")
     out = dict(params=np.array([n, i_strat, i_cfl, seed]), u=u, r=r, stats=st, w=w)
     for tag, s_, d_ in (("", sp, dv), ("w_", spw, dvw)):
@@ -86,6 +88,9 @@ def rhs_spectra_fixture(name, n, i_strat, i_cfl, seed):
             out[f"{tag}{k}"] = s_[k]
         out[f"{tag}scalars"] = np.array([s_["energy_w"], s_["div_rms"], s_["div_max"], s_["n_div"]])
         out[f"{tag}div_check"] = np.array([d_["div_max"], d_["div_rms"], d_["n_div"]])
+    for k in ("ek", "tk", "dk", "vt"):
+        out[f"cl_{k}"] = cl[k]
+    out["cl_energy"] = np.array([cl["e_cl"], cl["k_cl"]])
     np.savez_compressed(os.path.join(HERE, f"rhs_{name}.npz"), **out)
 
 

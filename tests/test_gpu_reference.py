@@ -123,6 +123,23 @@ def test_rhs_spectra_div_match_reference(name):
     s.close()
 
 
+@pytest.mark.parametrize("name", ["16u", "32s"])
+def test_cl_diagnostics_match_reference(name):
+    """the Chollet-Lesieur diagnostics (sam_set_cl(h, 1), i_les = 1) of an
+    RHS pass: cl_energy.f and spectra.f's dk, vt of the reference's own
+    routines on the same state"""
+    f = np.load(os.path.join(GOLD, f"rhs_{name}.npz"))
+    n, strat, icfl, seed = (int(v) for v in f["params"])
+    s = S.Sam(S.make_params(n, i_strat=strat, grav=16.0, lapse=1.0, vis=5e-3, i_cfl=icfl, cfl0=0.5, dt0=0.01))
+    S.set_cl(s, S.CL_DIAG)
+    s.set_state(f["u"])
+    s.rhs(want_spectra=True)
+    sp = s.spectra()
+    s.close()
+    for k in ("ek", "tk", "dk", "vt"):
+        np.testing.assert_allclose(sp[k], f[f"cl_{k}"], rtol=1e-11, atol=1e-300)
+
+
 GOLD_T = [p for p in sorted(glob.glob(os.path.join(GOLD, "transforms_*.npz")))
           if min(int(v) for v in np.load(p)["dims"][:3]) >= 16]
 

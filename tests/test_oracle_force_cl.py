@@ -4,7 +4,8 @@
 force.f itself is pinned through the reference's own program: the force16
 run of tests/golden/sam_runs.npz (the reference's `sam` built with MPICH by
 oracle/Makefile, run with force.f after every step) is matched by the oracle
-in test_oracle_refrun.py.  No reference run covers cl_energy.f (i_les = 1).
+in test_oracle_refrun.py, and so are cl_energy.f and spectra.f with the
+Chollet-Lesieur model (the reference's own routines, tests/golden/rhs_*.npz).
 Both are also pinned here by independent evaluations:
 - forcing: after force.f the shell energies 2..k_force+1 of u, v, w equal
   the target exactly, every other shell, the mean and T' are untouched, and

@@ -157,6 +157,27 @@ def test_oracle_rhs_spectra_div_vs_reference(name):
     assert g["w_scalars"][1] > 1e-3 and g["w_div_check"][2] > 0   # the random field has divergence
 
 
+@pytest.mark.parametrize("name", ["16u", "32s"])
+def test_oracle_cl_vs_reference(name):
+    """the Chollet-Lesieur model (i_les = 1): cl_energy.f's band energy and
+    spectra.f's dk and vt with nu_t(k) = fac_cl (a + b exp(-c k_cl/|k|))
+    (sam.f:128-132, spectra.f:21,39-43, init.f:88-93), against the
+    reference's own routines on the same state"""
+    g = np.load(os.path.join(GOLD, f"rhs_{name}.npz"))
+    n, strat, icfl, seed = (int(v) for v in g["params"])
+    o = O.Oracle(O.make_cfg(n, i_strat=strat, grav=16.0, lapse0=1.0, vis=5e-3, i_cfl=icfl, cfl0=0.5, dt0=0.01,
+                            i_cl=1))
+    e_cl, fac = o.cl_energy(g["u"])
+    e_ref, k_cl = g["cl_energy"]
+    assert k_cl == int(n / 3.0)
+    assert abs(e_cl - e_ref) <= 1e-13 * e_ref
+    sp = o.spectra(g["u"], g["r"], fac)
+    for k in ("ek", "tk", "dk", "vt"):
+        np.testing.assert_allclose(sp[k], g[f"cl_{k}"], rtol=1e-12, atol=1e-300)
+    assert np.all(g["cl_vt"][1:][g["sample"][1:] > 0] > 0)            # the model is on in every shell
+    assert np.abs(g["cl_dk"] - g["dk"]).max() > 1e-3 * np.abs(g["dk"]).max()
+
+
 def test_divergence_brute_force():
     """spectra.f's divergence sums by brute force on a 16^3 field:
     div_rms = sqrt(sum over stored modes of |k.u|^2) (no kx factor 2),
