@@ -23,6 +23,13 @@ Cases (every field is u(Nz_min, Ny_min, Nx_min, Lu) complex):
   force16    fresh stratified start with force.f (i_force 1, k_force 2): 5 steps
   midrk16    n_skip_v 2, 4 steps: vel.000002 holds the state written inside
              step 2's last substep (sam.f:277-284), vel.000004 the final one
+
+tests/golden/sam_planes.npz (planes_fixture): a stratified 32^3 restart
+(seeded solenoidal u, v, w and T', N^2 = 16, nu = 5e-3, fixed dt 0.01, so
+lapse = 0 after the restart) run for 4 steps with n_skip_p 2 and one xy,
+one xz and two yz planes: the reference's time.out, mean.out and
+xy1/xz1/yz1/yz2.dat (write_planes.f, get_mean.f, x_deriv.f, y_deriv.f) as
+float64 arrays.
 """
 import json
 import os
@@ -146,5 +153,25 @@ def main():
     print("wrote", os.path.join(HERE, "sam_runs.npz"), sorted(out))
 
 
+def planes_fixture():
+    p = dict(Nx=32, Ny=32, Nz=32, xL=TWO_PI, yL=TWO_PI, zL=TWO_PI, vis=5e-3, i_restart=1, nt_restart=0,
+             i_strat=1, grav=16.0, lapse0=1.0, To=1.0, Pr=1.0, z0=0.0, i_cfl=0, dt0=0.01, nrk_max=3,
+             n_dealias=1, i_fs=0, n_skip_h=2, n_skip_p=2, k_xy_plane1=5, j_xz_plane1=3, i_yz_plane1=7,
+             i_yz_plane2=20)
+    u0 = ic.random_solenoidal(32, k0=4, seed=31, i_strat=1)
+    u0[..., 3] = 0.1 * ic.random_solenoidal(32, k0=3, seed=32)[..., 0]
+    files = {"vel.000000": lambda q: sio.write_vel(q, u0),
+             "header.000000": lambda q: sio.write_header(q, p, time=0.0, dt=0.01, nt=0)}
+    d = run("planes32", dict(p, n_steps=4), files)
+    out = {"params_json": np.array(json.dumps(p)), "u0": u0,
+           "time": np.loadtxt(os.path.join(d, "time.out"))}
+    for f in ("mean.out", "xy1.dat", "xz1.dat", "yz1.dat", "yz2.dat"):
+        out[f.split(".")[0]] = np.fromfile(os.path.join(d, f), dtype=np.float64)
+    np.savez_compressed(os.path.join(HERE, "sam_planes.npz"), **out)
+    print("wrote", os.path.join(HERE, "sam_planes.npz"), {k: v.shape for k, v in out.items()})
+
+
 if __name__ == "__main__":
-    main()
+    if "--planes" not in sys.argv[1:]:
+        main()
+    planes_fixture()

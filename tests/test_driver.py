@@ -218,6 +218,29 @@ def test_planes_files(tmp_path):
 
 @pytest.mark.gpu
 @needs_driver
+def test_planes_files_match_reference(tmp_path):
+    """the driver's time.out, mean.out and xy/xz/yz planes files equal the
+    reference's own files of the same run (a stratified 32^3 restart,
+    n_skip_p 2, 4 steps; tests/golden/sam_planes.npz) at 1e-12"""
+    import json
+    g = np.load(os.path.join(ROOT, "tests", "golden", "sam_planes.npz"))
+    p = json.loads(str(g["params_json"]))
+    u0 = g["u0"]
+    sio.write_vel(str(tmp_path / "vel.000000"), u0)
+    sio.write_header(str(tmp_path / "header.000000"), p, time=0.0, dt=0.01, nt=0)
+    r = run(tmp_path, **dict(p, n_steps=4))
+    assert r.returncode == 0, r.stdout + r.stderr
+    t = np.loadtxt(str(tmp_path / "time.out"))
+    assert np.allclose(t, g["time"], rtol=0, atol=1e-8)
+    for f in ("mean", "xy1", "xz1", "yz1", "yz2"):
+        got = np.fromfile(str(tmp_path / (f + (".out" if f == "mean" else ".dat"))), dtype=np.float64)
+        ref = g[f]
+        assert got.shape == ref.shape, f
+        assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max(), (f, np.abs(got - ref).max())
+
+
+@pytest.mark.gpu
+@needs_driver
 def test_mid_step_vel_files_match_reference(tmp_path):
     """vel_midstep 1: vel.NNNNNN at nt < nt_end holds what the reference
     writes inside a step (sam.f:277-284); the reference's own run with

@@ -178,6 +178,40 @@ def test_oracle_cl_vs_reference(name):
     assert np.abs(g["cl_dk"] - g["dk"]).max() > 1e-3 * np.abs(g["dk"]).max()
 
 
+def test_oracle_planes_vs_reference():
+    """write_planes.f / get_mean.f (with x_deriv.f, y_deriv.f): the planes
+    and means files of the reference's own stratified 32^3 run (n_skip_p 2,
+    frames at nt = 0, 2, 4; tests/golden/sam_planes.npz) against the
+    oracle's planes of the same states, with the pressure of the previous
+    substep (zero at nt = 0)"""
+    g = np.load(os.path.join(GOLD, "sam_planes.npz"))
+    p = json.loads(str(g["params_json"]))
+    n, nfr = p["Nx"], 3
+    o = O.Oracle(O.make_cfg(n, i_strat=1, grav=p["grav"], lapse0=p["lapse0"], lapse=0.0, vis=p["vis"], i_cfl=0,
+                            dt0=p["dt0"]))
+    u0 = g["u0"]
+    lu = u0.shape[3]
+    nf = 2 * lu
+    assert list(g["time"][:, 0]) == [0, 2, 4]
+    files = {"xy1": ("xy", 0), "xz1": ("xz", 0), "yz1": ("yz", 0), "yz2": ("yz", 1)}
+    for fr, nt in enumerate((0, 2, 4)):
+        if nt == 0:
+            u, pr = u0, np.zeros(u0.shape[:3], np.complex128, order="F")
+        else:
+            u, _, _, _, pr = o.step(u0, nt, want_p=True)
+        mean, xy, xz, yz = o.planes(u, pr, (p["k_xy_plane1"],), (p["j_xz_plane1"],),
+                                    (p["i_yz_plane1"], p["i_yz_plane2"]))
+        got = {"xy": xy, "xz": xz, "yz": yz}
+        for f, (kind, m) in files.items():
+            ref = g[f].reshape(nfr, nf, n, n)[fr]            # Fortran records (n, n) per field
+            for q in range(nf):
+                a = got[kind][:, :, q, m].T
+                assert np.abs(a - ref[q]).max() <= 1e-12 * np.abs(ref[q]).max(), (nt, f, q)
+        ref_mean = g["mean"].reshape(nfr, lu, n)[fr]
+        assert np.abs(mean.T - ref_mean).max() <= 1e-12 * np.abs(ref_mean).max(), nt
+        assert np.abs(ref_mean).max() > 0.1                   # non-trivial horizontal means
+
+
 def test_divergence_brute_force():
     """spectra.f's divergence sums by brute force on a 16^3 field:
     div_rms = sqrt(sum over stored modes of |k.u|^2) (no kx factor 2),
