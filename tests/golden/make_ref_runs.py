@@ -29,7 +29,8 @@ tests/golden/sam_planes.npz (planes_fixture): a stratified 32^3 restart
 lapse = 0 after the restart) run for 4 steps with n_skip_p 2 and one xy,
 one xz and two yz planes: the reference's time.out, mean.out and
 xy1/xz1/yz1/yz2.dat (write_planes.f, get_mean.f, x_deriv.f, y_deriv.f) as
-float64 arrays.
+float64 arrays, and its log files energy.out, umax.out, tstep.out,
+variance.out and spectra.out as text.
 """
 import json
 import os
@@ -167,6 +168,9 @@ def planes_fixture():
            "time": np.loadtxt(os.path.join(d, "time.out"))}
     for f in ("mean.out", "xy1.dat", "xz1.dat", "yz1.dat", "yz2.dat"):
         out[f.split(".")[0]] = np.fromfile(os.path.join(d, f), dtype=np.float64)
+    for f in ("energy.out", "umax.out", "tstep.out", "variance.out", "spectra.out"):
+        with open(os.path.join(d, f)) as fh:
+            out["log_" + f.split(".")[0]] = np.array(fh.read())
     np.savez_compressed(os.path.join(HERE, "sam_planes.npz"), **out)
     print("wrote", os.path.join(HERE, "sam_planes.npz"), {k: v.shape for k, v in out.items()})
 

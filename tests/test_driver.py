@@ -221,7 +221,9 @@ def test_planes_files(tmp_path):
 def test_planes_files_match_reference(tmp_path):
     """the driver's time.out, mean.out and xy/xz/yz planes files equal the
     reference's own files of the same run (a stratified 32^3 restart,
-    n_skip_p 2, 4 steps; tests/golden/sam_planes.npz) at 1e-12"""
+    n_skip_p 2, 4 steps; tests/golden/sam_planes.npz) at 1e-12, and its log
+    files (energy, umax, tstep, variance, spectra) equal the reference's at
+    their printed precision"""
     import json
     g = np.load(os.path.join(ROOT, "tests", "golden", "sam_planes.npz"))
     p = json.loads(str(g["params_json"]))
@@ -237,6 +239,22 @@ def test_planes_files_match_reference(tmp_path):
         ref = g[f]
         assert got.shape == ref.shape, f
         assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max(), (f, np.abs(got - ref).max())
+    # the log files (sam.f:252-271, spectra.f:92-111): same lines and layout,
+    # numbers equal at their printed precision (5 significant digits)
+    for f in ("energy", "umax", "tstep", "variance", "spectra"):
+        got = open(str(tmp_path / (f + ".out"))).read().splitlines()
+        ref = str(g["log_" + f]).splitlines()
+        assert len(got) == len(ref), f
+        for a, b in zip(got, ref):
+            ta, tb = a.split(), b.split()
+            assert len(ta) == len(tb) and len(a) == len(b), (f, a, b)
+            for x, y in zip(ta, tb):
+                try:
+                    xv, yv = float(x), float(y)
+                except ValueError:
+                    assert x == y, (f, a, b)
+                    continue
+                assert abs(xv - yv) <= 2e-4 * abs(yv) + 1e-12, (f, a, b)
 
 
 @pytest.mark.gpu
