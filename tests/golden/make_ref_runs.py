@@ -30,7 +30,7 @@ lapse = 0 after the restart) run for 4 steps with n_skip_p 2 and one xy,
 one xz and two yz planes: the reference's time.out, mean.out and
 xy1/xz1/yz1/yz2.dat (write_planes.f, get_mean.f, x_deriv.f, y_deriv.f) as
 float64 arrays, and its log files energy.out, umax.out, tstep.out,
-variance.out and spectra.out as text.
+variance.out and spectra.out and the final header.000004 as text.
 """
 import json
 import os
@@ -168,7 +168,7 @@ def planes_fixture():
            "time": np.loadtxt(os.path.join(d, "time.out"))}
     for f in ("mean.out", "xy1.dat", "xz1.dat", "yz1.dat", "yz2.dat"):
         out[f.split(".")[0]] = np.fromfile(os.path.join(d, f), dtype=np.float64)
-    for f in ("energy.out", "umax.out", "tstep.out", "variance.out", "spectra.out"):
+    for f in ("energy.out", "umax.out", "tstep.out", "variance.out", "spectra.out", "header.000004"):
         with open(os.path.join(d, f)) as fh:
             out["log_" + f.split(".")[0]] = np.array(fh.read())
     np.savez_compressed(os.path.join(HERE, "sam_planes.npz"), **out)

@@ -223,7 +223,8 @@ def test_planes_files_match_reference(tmp_path):
     reference's own files of the same run (a stratified 32^3 restart,
     n_skip_p 2, 4 steps; tests/golden/sam_planes.npz) at 1e-12, and its log
     files (energy, umax, tstep, variance, spectra) equal the reference's at
-    their printed precision"""
+    their printed precision, and its header.000004 carries every label of
+    the reference's with the same value"""
     import json
     g = np.load(os.path.join(ROOT, "tests", "golden", "sam_planes.npz"))
     p = json.loads(str(g["params_json"]))
@@ -255,6 +256,17 @@ def test_planes_files_match_reference(tmp_path):
                     assert x == y, (f, a, b)
                     continue
                 assert abs(xv - yv) <= 2e-4 * abs(yv) + 1e-12, (f, a, b)
+    # header.000004 (write_header.f): every label of the reference's, same value
+    ref_h = tmp_path / "ref_header"
+    ref_h.write_text(str(g["log_header"]))
+    hr, hg = sio.parse_labels(str(ref_h)), sio.parse_labels(str(tmp_path / "header.000004"))
+    missing = [k for k in hr if k not in hg]
+    assert not missing, missing
+    for k, v in hr.items():
+        if isinstance(v, str) or isinstance(hg[k], str):
+            assert str(hg[k]) == str(v), (k, hg[k], v)
+        else:
+            assert abs(hg[k] - v) <= 1e-12 * abs(v) + 1e-300, (k, hg[k], v)
 
 
 @pytest.mark.gpu
