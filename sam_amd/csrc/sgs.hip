@@ -222,12 +222,14 @@ __global__ __launch_bounds__(DMX * DMY) void k_dm_fused(const T *__restrict__ P,
     }
     __syncthreads();
 
-    T m2[DMF], m1[DMF];                             // x-y filtered planes p-2, p-1 of this column
+    T pa[DMF], pb[DMF], pc[DMF];                    // x-y filtered planes of this column, rotated
 #pragma unroll
-    for (int f = 0; f < DMF; f++) m2[f] = m1[f] = (T)0;
+    for (int f = 0; f < DMF; f++) pa[f] = pb[f] = pc[f] = (T)0;
     double num = 0.0, den = 0.0;
     const int ip[6] = {0, 1, 2, 0, 1, 0}, jp[6] = {0, 1, 2, 1, 2, 2};
-    for (int p = z0 - 1; p <= z0 + zc; p++) {
+    // one plane: m2, m1 the x-y filtered planes p-2, p-1 of this column, cur
+    // receives plane p; the caller rotates the three arrays (no copies)
+    auto plane = [&](const int p, const T (&m2)[DMF], const T (&m1)[DMF], T (&cur)[DMF]) {
         // ring slots of planes p-1, p, p+1
         const int sm1 = (p - 1 - (z0 - 2)) % 3, s0 = (p - (z0 - 2)) % 3, sp1 = (p + 1 - (z0 - 2)) % 3;
         T nxt[NPRE];
@@ -288,23 +290,22 @@ __global__ __launch_bounds__(DMX * DMY) void k_dm_fused(const T *__restrict__ P,
         // the ring slot of plane p-1 is free: plane p+2 goes there
         if (more) store(sm1, nxt);
         // stage B: y sums -> the x-y filtered plane p of this thread's column
-        T cur[DMF];
 #pragma unroll
-        for (int f = 0; f < DMF; f++) {
-            const C acc = xf[f][ty][tx] + (C)4 * xf[f][ty + 1][tx] + xf[f][ty + 2][tx];
-            cur[f] = acc * (C)(1.0 / 36.0);
-        }
+        for (int f = 0; f < DMF; f++) cur[f] = xf[f][ty][tx] + (C)4 * xf[f][ty + 1][tx] + xf[f][ty + 2][tx];
         if (p >= z0 + 1) {
-            // plane p - 1 is filtered: z sum of planes p-2, p-1, p
+            // plane p - 1 is filtered: z sum of planes p-2, p-1, p.  The sums
+            // are G = 216 x the filtered fields ((1,4,1)/6 in three
+            // directions), and L' = 216^2 L, M' = 216^2 M are formed from them:
+            // C = -1/2 <L:M>/<M:M> is unchanged by the common factor 216^4
             C fl[DMF];
 #pragma unroll
-            for (int f = 0; f < DMF; f++) fl[f] = (m2[f] + (C)4 * m1[f] + cur[f]) * (C)(1.0 / 6.0);
+            for (int f = 0; f < DMF; f++) fl[f] = m2[f] + (C)4 * m1[f] + cur[f];
             const C smh = smag6c(fl + 3);
             C pn = 0, pd = 0;
 #pragma unroll
             for (int n = 0; n < 6; n++) {
-                const C M = -fl[9 + n] + (C)4 * smh * fl[3 + n];
-                const C L = fl[15 + n] - fl[ip[n]] * fl[jp[n]];
+                const C M = (C)4 * smh * fl[3 + n] - (C)216 * fl[9 + n];
+                const C L = (C)216 * fl[15 + n] - fl[ip[n]] * fl[jp[n]];
                 const C w = n < 3 ? (C)1 : (C)2;
                 pn += w * L * M;
                 pd += w * M * M;
@@ -312,12 +313,12 @@ __global__ __launch_bounds__(DMX * DMY) void k_dm_fused(const T *__restrict__ P,
             num += (double)pn;     // the volume sums in FP64 (dm.f:145-147)
             den += (double)pd;
         }
-#pragma unroll
-        for (int f = 0; f < DMF; f++) {
-            m2[f] = m1[f];
-            m1[f] = cur[f];
-        }
         __syncthreads();           // xf is rewritten and the ring read by the next plane
+    };
+    for (int p = z0 - 1; p <= z0 + zc; p += 3) {
+        plane(p, pa, pb, pc);
+        if (p + 1 <= z0 + zc) plane(p + 1, pb, pc, pa);
+        if (p + 2 <= z0 + zc) plane(p + 2, pc, pa, pb);
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
