@@ -1,6 +1,6 @@
 """GPU parity: libsamhip (through its C ABI) against the CPU oracle on the
-same inputs.  FP64 bar: relative L2 <= 1e-12 (the drop-in tolerance for
-the drop-in), per stage and after N full RK3 steps.  FP32-split bar: the
+same inputs.  FP64 bar: relative L2 <= 1e-12 (the drop-in's tolerance),
+per stage and after N full RK3 steps.  FP32-split bar: the
 error is reported and bounded by FP32 rounding (<= 2e-6 relative per stage).
 """
 import numpy as np
