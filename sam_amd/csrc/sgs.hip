@@ -450,7 +450,7 @@ __global__ __launch_bounds__(256) void k_sgs_flux_pt(const T *__restrict__ P, co
 // plane z), each thread keeps its own column's values of planes z-1, z, z+1
 // in registers for the z differences.  The fields are read from HBM once
 // per block plus the x-y halo, instead of once per stencil point.
-constexpr int FY = 8;
+constexpr int FY = 16;
 
 template <typename T, int DMX>
 __global__ __launch_bounds__(DMX * FY) void k_sgs_flux(const T *__restrict__ P, const DevDiag *__restrict__ diag,
