@@ -148,14 +148,18 @@ template <int S, typename T> __device__ __forceinline__ void dft8(cpx<T> *v)
 
 // LDS ordering between passes.  When every column of the tile belongs to a
 // single wavefront (threads per column divide 64), the exchange only has to
-// order this wave's own LDS accesses: wait for them and stop the compiler
-// from moving LDS accesses across.  Otherwise a workgroup barrier for LDS
-// data only (not __syncthreads, which also waits for every outstanding
-// global access and so would drain prefetched loads and pending stores).
+// order this wave's own LDS accesses.  The LDS performs one wavefront's
+// accesses in issue order, so a read issued after a write sees it without a
+// wait (the AMDGPU memory model needs no s_waitcnt for wavefront-scope LDS
+// ordering); only the compiler must be kept from moving LDS accesses across,
+// and the reads are queued right behind the writes.  Otherwise a workgroup
+// barrier for LDS data only (not __syncthreads, which also waits for every
+// outstanding global access and so would drain prefetched loads and pending
+// stores).
 template <bool WAVE> __device__ __forceinline__ void tile_sync()
 {
     if constexpr (WAVE) {
-        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+        asm volatile("" ::: "memory");
     } else {
         asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
     }

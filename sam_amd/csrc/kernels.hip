@@ -308,6 +308,54 @@ __device__ __forceinline__ void xrow_store_pairs(const cpx<T> *lds, cpx<T> *__re
     }
 }
 
+// the value of the neighbouring lane (l ^ 1), through DPP (no LDS traffic)
+template <typename T> __device__ __forceinline__ cpx<T> lane_swap1(cpx<T> a)
+{
+    static_assert(sizeof(cpx<T>) % 4 == 0, "whole dwords");
+    constexpr int ND = sizeof(cpx<T>) / 4;
+    struct W { int v[ND]; };
+    W w = __builtin_bit_cast(W, a), r;
+#pragma unroll
+    for (int i = 0; i < ND; i++) r.v[i] = __builtin_amdgcn_update_dpp(0, w.v[i], 0xB1, 0xF, 0xF, true);   // quad_perm 1,0,3,2
+    return __builtin_bit_cast(cpx<T>, r);
+}
+
+// FP64 form of xrow_store_pairs whose store instructions write whole
+// 128-byte k-rows: item (k, c, h) with h fastest; lane pairs (h = 0, 1) read
+// Z_k and Z_{N-k} of column pair c once each, swap them through DPP, and
+// store X_k (h = 0) and Y_k (h = 1) side by side, so eight lanes fill one
+// line in one instruction.  (xrow_store_pairs stores a lane's 32 bytes as
+// two 16-byte halves, each instruction covering half of every line it
+// touches; with the nt policy the halves leave the L2 separately, and the
+// 512^3 FP64 x-row kernel took 2.32 ms against 2.25 ms with these stores.)
+template <typename T, int N, int CP, int NT>
+__device__ __forceinline__ void xrow_store_lines(const cpx<T> *lds, cpx<T> *__restrict__ Yq, int ny, int y0, int kxm,
+                                                 T h)
+{
+    static_assert(2 * CP * sizeof(cpx<T>) == 128 && NT % 2 == 0, "whole lines of 2 CP values, lane pairs");
+    constexpr int NI = (2 * CP * (N / 3 + 1) + NT - 1) / NT;
+    const int nitem = 2 * CP * (kxm + 1);
+    cpx<T> m[NI];
+#pragma unroll
+    for (int i = 0; i < NI; i++) {
+        const int e = threadIdx.x + i * NT, ee = e < nitem ? e : 0;
+        const int hh = ee & 1, c = (ee >> 1) % CP, k = (ee >> 1) / CP;
+        m[i] = lds[lds_idx<T, N>(c, hh ? (N - k) & (N - 1) : k)];
+    }
+#pragma unroll
+    for (int i = 0; i < NI; i++) {
+        const int e = threadIdx.x + i * NT;
+        const cpx<T> o = lane_swap1(m[i]);
+        if (e < nitem) {
+            const int hh = e & 1, c = (e >> 1) % CP, k = (e >> 1) / CP;
+            // h = 0: X_k = (Z_k + conj Z_{N-k})/2 ; h = 1: Y_k = -i (Z_k - conj Z_{N-k})/2
+            const cpx<T> v = hh ? cpx<T>{(m[i].y + o.y) * h, (m[i].x - o.x) * h}
+                                : cpx<T>{(m[i].x + o.x) * h, (m[i].y - o.y) * h};
+            st_nt(Yq + (long long)k * ny + y0 + 2 * c + hh, v);
+        }
+    }
+}
+
 __device__ __forceinline__ double wave_sum(double v)
 {
 #pragma unroll
@@ -665,7 +713,8 @@ __device__ __forceinline__ void xrow_products(cpx<T> *lds, cpx<T> *stg, const TW
         if constexpr (KEPT) xrow_store_kept<T, N, EPT>(tile, c, j0, pi);
         else TL::store_natural(tile, c, j0, pi);
         lds_barrier();
-        xrow_store_pairs<T, N, CP, NT>(tile, Yz + q * fstride, d.ny, y0, d.kxm, h);
+        if constexpr (KEPT && sizeof(T) == 8) xrow_store_lines<T, N, CP, NT>(tile, Yz + q * fstride, d.ny, y0, d.kxm, h);
+        else xrow_store_pairs<T, N, CP, NT>(tile, Yz + q * fstride, d.ny, y0, d.kxm, h);
     }
 }
 
