@@ -56,6 +56,80 @@ def kernel_bytes(n, bc, i_strat=1, zrhs_fused=False, nrk=3):
     }
 
 
+def kernel_rw(n, bc, i_strat=1, zrhs_fused=False, nrk=3):
+    """kernel_bytes split into (bytes read, bytes written) per launch"""
+    g = grid_counts(n, i_strat)
+    S, Z, Y, lu, lr = g["S"], g["Z"], g["Y"], g["lu"], g["lr"]
+    return {
+        "zb": (lu * S * 16, lu * Z * bc),
+        "yb": (lu * Z * bc, lu * Y * bc),
+        "xrow": (lu * Y * bc, lr * Y * bc),
+        "yf": (lr * Y * bc, lr * Z * bc),
+        "zrhs": (lr * Z * bc + (1 + (nrk - 1) / nrk) * lu * S * 16,
+                 2 * lu * S * 16 + (lu * Z * bc if zrhs_fused else 0)),
+    }
+
+
+# (arrays read, arrays written) per launch of the streaming-mix kernel
+# (sam_device_stream); a kernel's practical ceiling is the mix nearest its
+# own read:write proportions
+STREAM_MIXES = ((1, 0), (0, 1), (1, 1), (1, 2), (2, 1), (2, 3), (3, 2), (4, 5), (5, 4), (4, 9))
+
+
+def nearest_mix(rd, wr):
+    def key(m):
+        a, b = m
+        if a == 0 or b == 0 or rd == 0 or wr == 0:
+            return 0.0 if (a == 0) == (rd == 0) and (b == 0) == (wr == 0) else 1e9
+        return abs(math.log((a / b) / (rd / wr)))
+    return min(STREAM_MIXES, key=key)
+
+
+def stream_ceilings(S, mixes, nbytes=1 << 30, reps=8, trials=3):
+    """sustained GB/s (bytes read + written) of the streaming kernel at each
+    read:write mix, nt policy, on this device: the best of `trials` timings of
+    `reps` launches (each launch moves (a + b) GiB, so the call overhead is
+    well under 1%)"""
+    cal = S.Sam(make_params(S, 64, S.PREC_FP64))
+    out = {}
+    for a, b in mixes:
+        cal.device_stream(a, b, nbytes, 1)
+        best = 0.0
+        for _ in range(trials):
+            t0 = time.perf_counter()
+            cal.device_stream(a, b, nbytes, reps)
+            best = max(best, reps * (a + b) * nbytes / (time.perf_counter() - t0) / 1e9)
+        out[(a, b)] = best
+    cal.close()
+    return out
+
+
+def with_ceilings(per_kernel, rw, ceil):
+    """per-kernel mix, ceiling and fraction of it (in place)"""
+    for k, v in per_kernel.items():
+        if k not in rw:
+            continue
+        rd, wr = rw[k]
+        m = nearest_mix(rd, wr)
+        v["read_write_GB"] = [round(rd / 1e9, 3), round(wr / 1e9, 3)]
+        v["mix"] = f"{m[0]}:{m[1]}"
+        v["ceiling_GBps"] = round(ceil[m], 1)
+        v["frac_of_ceiling"] = round(v["GBps"] / ceil[m], 4)
+    return per_kernel
+
+
+def step_floor_ms(n, bc, prof, steps, ceil):
+    """the step's time if every kernel ran at its mix's ceiling (launch
+    counts of the run)"""
+    rw = kernel_rw(n, bc, zrhs_fused=zrhs_fused(prof))
+    t = 0.0
+    for k, (rd, wr) in rw.items():
+        cnt = prof.get(k, (0.0, 0))[1]
+        if cnt:
+            t += (rd + wr) / (ceil[nearest_mix(rd, wr)] * 1e9) * cnt
+    return t / steps * 1e3
+
+
 def zrhs_fused(prof):
     """the timed region's k_zrhs wrote the next z inverse: fewer k_zb than
     k_zrhs launches (k_zb runs only where the previous k_zrhs did not)"""
@@ -512,6 +586,20 @@ def main():
                        "frac_of_peak": round(10 * (2 << 30) / t_copy / HBM_PEAK, 4),
                        "step_frac_of_copy": round(sb / (r64["wall"] / args.steps) / (10 * (2 << 30) / t_copy), 4)}
     log(f"[bench] copy kernel {out['hbm_copy']['GBps']} GB/s")
+    # practical ceilings by read:write mix (the write rate is the lower one)
+    ceil = stream_ceilings(S, STREAM_MIXES)
+    out["hbm_mix_ceilings"] = {
+        "kernel": "k_stream: a arrays of 1 GiB read, b written per launch (nt, 16-byte accesses), best of 3 x 8 launches",
+        "GBps_by_read_write": {f"{a}:{b}": round(v, 1) for (a, b), v in ceil.items()}}
+    rw64 = kernel_rw(n, 16, zrhs_fused=zrhs_fused(r64["prof"]))
+    with_ceilings(out["kernels"], rw64, ceil)
+    kmix = nearest_mix(*rw64[roof["kernel"]])
+    roof["practical_ceiling"] = {"GBps": round(ceil[kmix], 1), "read_write_mix": f"{kmix[0]}:{kmix[1]}",
+                                 "frac": round(roof["achieved"] / ceil[kmix], 4)}
+    fl64 = step_floor_ms(n, 16, r64["prof"], args.steps, ceil)
+    out["step_hbm"]["floor_ms_at_mix_ceilings"] = round(fl64, 3)
+    out["step_hbm"]["frac_of_mix_ceilings"] = round(fl64 / ms_step, 4)
+    log(f"[bench] mix ceilings {out['hbm_mix_ceilings']['GBps_by_read_write']}; step floor {fl64:.2f} ms")
     if args.calibrate_copy:
         cal.device_copy(1 << 30, 3)
         out["copy_calibration_bytes"] = {"read": 1 << 30, "write": 1 << 30}
@@ -527,8 +615,16 @@ def main():
         e64, e32 = r64["spec"]["energy_w"], rsp["spec"]["energy_w"]
         d64, d32 = np.sum(r64["spec"]["dk"]), np.sum(rsp["spec"]["dk"])
         ek64, ek32 = r64["spec"]["ek"], rsp["spec"]["ek"]
+        rw32 = kernel_rw(n, 8, zrhs_fused=zrhs_fused(rsp["prof"]))
+        with_ceilings(pk32, rw32, ceil)
+        m32 = nearest_mix(*rw32[roof32["kernel"]])
+        roof32["practical_ceiling"] = {"GBps": round(ceil[m32], 1), "read_write_mix": f"{m32[0]}:{m32[1]}",
+                                       "frac": round(roof32["achieved"] / ceil[m32], 4)}
+        fl32 = step_floor_ms(n, 8, rsp["prof"], args.steps, ceil)
         out["fp32_split"] = {
             "value": round(n ** 3 * args.steps / rsp["wall"] / 1e6, 2),
+            "step_floor_ms_at_mix_ceilings": round(fl32, 3),
+            "step_frac_of_mix_ceilings": round(fl32 / ms_sp, 4),
             "ms_per_step": round(ms_sp, 3),
             "ms_per_step_median_single": round(rsp["median"] * 1e3, 3),
             "step_hbm_frac": round(sb32 / (rsp["wall"] / args.steps) / HBM_PEAK, 4),

@@ -36,7 +36,7 @@ EXPORTS = (
     "sam_z_trans_b", "sam_x2z", "sam_xy_trans_b", "sam_xy_trans_f", "sam_z2x", "sam_z_trans_f",
     "sam_products", "sam_vel_max", "sam_vel_max_noalloc", "sam_project", "sam_rk_update", "sam_div_check",
     "sam_eddy_visc", "sam_test_filter",
-    "sam_prof_enable", "sam_prof_read", "sam_sync", "sam_device_copy",
+    "sam_prof_enable", "sam_prof_read", "sam_sync", "sam_device_copy", "sam_device_stream",
     "sam_mlp_create", "sam_mlp_destroy", "sam_mlp_get_params", "sam_mlp_set_params",
     "sam_mlp_add_samples", "sam_mlp_add_samples_host", "sam_mlp_pool_size", "sam_mlp_get_samples",
     "sam_mlp_clear_pool", "sam_mlp_set_labels", "sam_mlp_last_coefficient", "sam_mlp_set_activation", "sam_mlp_get_activation",
@@ -141,6 +141,7 @@ def load():
     L.sam_prof_read.argtypes = [_hp, _dp, ctypes.POINTER(ctypes.c_longlong)]
     L.sam_sync.argtypes = [_hp]
     L.sam_device_copy.argtypes = [_hp, ctypes.c_longlong, ctypes.c_int]
+    L.sam_device_stream.argtypes = [_hp, ctypes.c_int, ctypes.c_int, ctypes.c_longlong, ctypes.c_int]
     _fp = ctypes.POINTER(ctypes.c_float)
     L.sam_mlp_create.argtypes = [ctypes.POINTER(_hp), ctypes.c_uint, ctypes.c_longlong]
     L.sam_mlp_destroy.argtypes = [_hp]
@@ -376,6 +377,9 @@ class Sam:
 
     def device_copy(self, nbytes, reps=1):
         _chk(self.L.sam_device_copy(self.h, int(nbytes), int(reps)))
+
+    def device_stream(self, nread, nwrite, nbytes, reps=1):
+        _chk(self.L.sam_device_stream(self.h, int(nread), int(nwrite), int(nbytes), int(reps)))
 
     def test_filter(self, f):
         f = np.asfortranarray(f, dtype=np.float64)
