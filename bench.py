@@ -589,7 +589,7 @@ def main():
     # practical ceilings by read:write mix (the write rate is the lower one)
     ceil = stream_ceilings(S, STREAM_MIXES)
     out["hbm_mix_ceilings"] = {
-        "kernel": "k_stream: a arrays of 1 GiB read, b written per launch (nt, 16-byte accesses), best of 3 x 8 launches",
+        "kernel": "k_stream: a GiB read from one array, b GiB written to another per launch, in 16 KiB chunks (nt, 16-byte accesses), best of 3 x 8 launches",
         "GBps_by_read_write": {f"{a}:{b}": round(v, 1) for (a, b), v in ceil.items()}}
     rw64 = kernel_rw(n, 16, zrhs_fused=zrhs_fused(r64["prof"]))
     with_ceilings(out["kernels"], rw64, ceil)

@@ -329,12 +329,13 @@ int  sam_sync(sam_handle *h);
 /* plain device-to-device copy of `bytes` bytes, `reps` times, on the
  * handle's stream: the calibration kernel for HBM counters */
 int  sam_device_copy(sam_handle *h, long long bytes, int reps);
-/* streaming mix: each launch reads `nread` and writes `nwrite` arrays of
- * `bytes` bytes (a multiple of 16 KiB; the arrays sit 4099 x 16 bytes apart
- * beyond their length), `reps` launches, synchronous; the
- * measured ceiling of a kernel with the same read:write proportions.
- * Supported (nread, nwrite): (1,0) (0,1) (1,1) (1,2) (2,1) (2,3) (3,2)
- * (4,5) (5,4) (4,9).  No reference counterpart (measurement only). */
+/* streaming mix: each launch reads nread x `bytes` bytes from one array and
+ * writes nwrite x `bytes` to another (one read and one write stream, 16 KiB
+ * chunks in the ratio nread:nwrite; `bytes` a multiple of 16 KiB), `reps`
+ * launches, synchronous: the sustained rate of a kernel with the same
+ * read:write proportions.  Supported (nread, nwrite): (1,0) (0,1) (1,1)
+ * (1,2) (2,1) (2,3) (3,2) (4,5) (5,4) (4,9).  No reference counterpart
+ * (measurement only). */
 int  sam_device_stream(sam_handle *h, int nread, int nwrite, long long bytes, int reps);
 
 #ifdef __cplusplus

@@ -1318,10 +1318,10 @@ extern "C" int sam_device_stream(sam_handle *h, int nread, int nwrite, long long
 {
     if (!h || bytes <= 0 || bytes % 16384 || reps < 1) return set_err(SAM_ERR_VALUE, "bad argument");
     if (!stream_mix_supported(nread, nwrite)) return set_err(SAM_ERR_VALUE, "unsupported read:write mix");
-    const long long n = bytes / 16, ld = stream_array_stride(n);
-    int rc = ensure(h->stage0, (size_t)(nread > 0 ? nread : 1) * (size_t)ld * 16);
+    const long long n = bytes / 16;
+    int rc = ensure(h->stage0, (size_t)(nread > 0 ? nread : 1) * (size_t)bytes);
     if (rc) return rc;
-    rc = ensure(h->stage1, (size_t)(nwrite > 0 ? nwrite : 1) * (size_t)ld * 16);
+    rc = ensure(h->stage1, (size_t)(nwrite > 0 ? nwrite : 1) * (size_t)bytes);
     if (rc) return rc;
     for (int r = 0; r < reps; r++) RCHK(launch_stream(nread, nwrite, h->stage0.p, h->stage1.p, n, h->st));
     HIPCHK(hipStreamSynchronize(h->st));

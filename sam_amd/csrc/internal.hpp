@@ -191,7 +191,6 @@ int launch_plane_gather(const Dims &d, const T *P, int kind, int pos, int src_f0
 // plain 16-byte-per-thread copy (counter calibration)
 int launch_copy16(const void *src, void *dst, long long n16, hipStream_t st);
 bool stream_mix_supported(int nr, int nw);
-long long stream_array_stride(long long n16);
 int launch_stream(int nr, int nw, const void *src, void *dst, long long n16, hipStream_t st);
 
 // max number of per-block partial rows the step's reductions may produce

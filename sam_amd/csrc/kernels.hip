@@ -2249,47 +2249,44 @@ int launch_copy16(const void *src, void *dst, long long n16, hipStream_t st)
     return launched();
 }
 
-// Streaming mixes for the practical HBM ceilings: NR arrays of n 16-byte
-// elements read and NW written per launch, in k_copy16's form (4 independent
-// elements per thread and iteration, nt policy).  The step kernels read and
-// write in different proportions (the x-row kernel writes 9 fields for the 4
-// it reads), and on gfx950 the sustained write rate is below the read rate,
-// so each kernel's ceiling is measured at its own read:write mix.
-// Arrays sit STREAM_PAD elements apart beyond their length, so the streams
-// do not all start on the same HBM channel (the step's fields are not
-// power-of-two sized either).
-constexpr long long STREAM_PAD = 4099;
+// Streaming mixes for the practical HBM ceilings: per launch NR units read
+// from one array and NW units written to another, each unit n 16-byte
+// elements, in k_copy16's form (4 independent elements per thread and
+// iteration, nt policy).  A block takes 16 KiB chunks: chunk c reads
+// elements [(c NR + r) 1024, +1024) and writes [(c NW + w) 1024, +1024), so
+// there is one read stream and one write stream whatever the mix.  The step
+// kernels read and write in different proportions (the x-row kernel writes 9
+// fields for the 4 it reads), and on gfx950 the sustained write rate is below
+// the read rate, so each kernel's ceiling is measured at its own mix.
 template <int NR, int NW>
-__global__ __launch_bounds__(256) void k_stream(const uint4 *__restrict__ s, uint4 *__restrict__ d, long long n)
+__global__ __launch_bounds__(256) void k_stream(const uint4 *__restrict__ s, uint4 *__restrict__ d, long long nchunk)
 {
-    const long long ld = n + STREAM_PAD;
     typedef unsigned v4u __attribute__((ext_vector_type(4)));
     const v4u *a = reinterpret_cast<const v4u *>(s);
     v4u *b = reinterpret_cast<v4u *>(d);
-    const long long stride = (long long)gridDim.x * 1024;
-    for (long long i = (long long)blockIdx.x * 1024 + threadIdx.x; i + 768 < n; i += stride) {
+    for (long long c = blockIdx.x; c < nchunk; c += gridDim.x) {
         v4u v[4] = {};
 #pragma unroll
         for (int r = 0; r < NR; r++)
 #pragma unroll
-            for (int j = 0; j < 4; j++) v[j] ^= __builtin_nontemporal_load(a + r * ld + i + 256 * j);
+            for (int j = 0; j < 4; j++)
+                v[j] ^= __builtin_nontemporal_load(a + (c * NR + r) * 1024 + 256 * j + threadIdx.x);
         if constexpr (NW == 0) {
             // keep the loads: a store no input can trigger (the XOR of four
             // array elements is compared with a value written nowhere)
             if (v[0].x == 0x5a3c96e1u && v[1].y == 0xe1963c5au && v[2].z == 0x0f0f0f0fu && v[3].w == 0xf0f0f0f0u)
-                b[i] = v[0];
+                b[threadIdx.x] = v[0];
         }
 #pragma unroll
         for (int w = 0; w < NW; w++)
 #pragma unroll
-            for (int j = 0; j < 4; j++) __builtin_nontemporal_store(v[j] + (unsigned)w, b + w * ld + i + 256 * j);
+            for (int j = 0; j < 4; j++)
+                __builtin_nontemporal_store(v[j] + (unsigned)w, b + (c * NW + w) * 1024 + 256 * j + threadIdx.x);
     }
 }
 
 // (nread, nwrite) pairs launch_stream supports
 #define SAM_STREAM_MIXES X(1, 0) X(0, 1) X(1, 1) X(1, 2) X(2, 1) X(2, 3) X(3, 2) X(4, 5) X(5, 4) X(4, 9)
-
-long long stream_array_stride(long long n16) { return n16 + STREAM_PAD; }
 
 bool stream_mix_supported(int nr, int nw)
 {
@@ -2301,12 +2298,12 @@ bool stream_mix_supported(int nr, int nw)
 
 int launch_stream(int nr, int nw, const void *src, void *dst, long long n16, hipStream_t st)
 {
-    long long nb = (n16 + 1023) / 1024;
-    if (nb > 32768) nb = 32768;
+    const long long nchunk = n16 / 1024;
+    const long long nb = nchunk < 32768 ? nchunk : 32768;
 #define X(a, b)                                                                                                    \
     if (nr == a && nw == b) {                                                                                      \
         hipLaunchKernelGGL((k_stream<a, b>), dim3((unsigned)nb), dim3(256), 0, st, (const uint4 *)src, (uint4 *)dst, \
-                           n16);                                                                                   \
+                           nchunk);                                                                                \
         return launched();                                                                                         \
     }
     SAM_STREAM_MIXES
