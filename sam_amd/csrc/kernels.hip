@@ -336,11 +336,15 @@ __device__ __forceinline__ void xrow_store_lines(const cpx<T> *lds, cpx<T> *__re
     constexpr int NI = (2 * CP * (N / 3 + 1) + NT - 1) / NT;
     const int nitem = 2 * CP * (kxm + 1);
     cpx<T> m[NI];
+    // the tile holds Z_p for p < N/2 and (im, re) of Z_p for p > N/2, and
+    // (im, re) of Z_0 at N/2 (xrow_store_kept<SWAPHI>): lane h = 0 reads Z_k,
+    // lane h = 1 the swapped Z_{N-k}, and after the swap both form their
+    // output with the same expression (no per-lane select)
 #pragma unroll
     for (int i = 0; i < NI; i++) {
         const int e = threadIdx.x + i * NT, ee = e < nitem ? e : 0;
         const int hh = ee & 1, c = (ee >> 1) % CP, k = (ee >> 1) / CP;
-        m[i] = lds[lds_idx<T, N>(c, hh ? (N - k) & (N - 1) : k)];
+        m[i] = lds[lds_idx<T, N>(c, hh ? (k ? N - k : N / 2) : k)];
     }
 #pragma unroll
     for (int i = 0; i < NI; i++) {
@@ -349,8 +353,7 @@ __device__ __forceinline__ void xrow_store_lines(const cpx<T> *lds, cpx<T> *__re
         if (e < nitem) {
             const int hh = e & 1, c = (e >> 1) % CP, k = (e >> 1) / CP;
             // h = 0: X_k = (Z_k + conj Z_{N-k})/2 ; h = 1: Y_k = -i (Z_k - conj Z_{N-k})/2
-            const cpx<T> v = hh ? cpx<T>{(m[i].y + o.y) * h, (m[i].x - o.x) * h}
-                                : cpx<T>{(m[i].x + o.x) * h, (m[i].y - o.y) * h};
+            const cpx<T> v = {(m[i].x + o.y) * h, (m[i].y - o.x) * h};
             st_nt(Yq + (long long)k * ny + y0 + 2 * c + hh, v);
         }
     }
@@ -578,7 +581,10 @@ __device__ __forceinline__ void xrow_pass0_from_stage(const cpx<T> *stg, int c, 
 // final forward outputs to LDS in natural order, leaving out the slots whose
 // positions all fall in the truncated band kx_max < k < N - kx_max (kx_max =
 // N/3): nothing reads them, so their butterfly outputs are never formed
-template <typename T, int N, int EPT = 8>
+// SWAPHI: positions above N/2 (read only as Z_{N-k}) hold (im, re), and
+// position N/2 (in the truncated band) holds Z_0 as (im, re) too; see
+// xrow_store_lines
+template <typename T, int N, int EPT = 8, bool SWAPHI = false>
 __device__ __forceinline__ void xrow_store_kept(cpx<T> *lds, int c, int j0, const cpx<T> *v)
 {
     using TL = Tile<T, N, EPT>;
@@ -587,7 +593,11 @@ __device__ __forceinline__ void xrow_store_kept(cpx<T> *lds, int c, int j0, cons
     for (int s = 0; s < EPT; s++) {
         const int lo = TL::kout(s) * TL::TPC, hi = lo + TL::TPC - 1;
         if (lo > KXM && hi < N - KXM) continue;
-        lds[lds_idx<T, N>(c, j0 + TL::kout(s) * TL::TPC)] = v[s];
+        const bool sw = SWAPHI && lo >= N / 2;
+        lds[lds_idx<T, N>(c, j0 + TL::kout(s) * TL::TPC)] = sw ? cpx<T>{v[s].y, v[s].x} : v[s];
+        if constexpr (SWAPHI) {
+            if (lo == 0 && j0 == 0) lds[lds_idx<T, N>(c, N / 2)] = cpx<T>{v[s].y, v[s].x};
+        }
     }
 }
 
@@ -710,7 +720,8 @@ __device__ __forceinline__ void xrow_products(cpx<T> *lds, cpx<T> *stg, const TW
             lds_barrier();
         }
         xrow_fft<-1, TL>(tile, tw, c, j0, pi);
-        if constexpr (KEPT) xrow_store_kept<T, N, EPT>(tile, c, j0, pi);
+        if constexpr (KEPT && sizeof(T) == 8) xrow_store_kept<T, N, EPT, true>(tile, c, j0, pi);
+        else if constexpr (KEPT) xrow_store_kept<T, N, EPT>(tile, c, j0, pi);
         else TL::store_natural(tile, c, j0, pi);
         lds_barrier();
         if constexpr (KEPT && sizeof(T) == 8) xrow_store_lines<T, N, CP, NT>(tile, Yz + q * fstride, d.ny, y0, d.kxm, h);
