@@ -180,18 +180,27 @@ template <typename T> struct Stencil {
     T c, xp, xm, yp, ym, zp, zm;
 };
 
+// the 16 points of a tile share y and z (nx >= 16, tiles aligned to 16):
+// their y and z neighbour offsets are wave-uniform and come from the scalar
+// unit; only the x neighbours (periodic at x = 0, nx - 1) differ per lane
 template <typename T>
-__device__ __forceinline__ void stencil_load(const T *__restrict__ F, int lx, int ly, int lz, unsigned p, Stencil<T> &s)
+__device__ __forceinline__ void stencil_load_tile(const T *__restrict__ F, int lx, int ly, int lz, unsigned t, int i,
+                                                  Stencil<T> &s)
 {
+    const unsigned tu = __builtin_amdgcn_readfirstlane(t);
     const unsigned mx = (1u << lx) - 1, my = (1u << ly) - 1, mz = (1u << lz) - 1;
-    const unsigned x = p & mx, y = (p >> lx) & my, z = p >> (lx + ly);
-    s.c = F[p];
-    s.xp = F[p - x + ((x + 1) & mx)];
-    s.xm = F[p - x + ((x - 1) & mx)];
-    s.yp = F[p + ((((y + 1) & my) - y) << lx)];
-    s.ym = F[p + ((((y - 1) & my) - y) << lx)];
-    s.zp = F[p + ((((z + 1) & mz) - z) << (lx + ly))];
-    s.zm = F[p + ((((z - 1) & mz) - z) << (lx + ly))];
+    const unsigned base = tu * 16, y = (base >> lx) & my, z = base >> (lx + ly);
+    const int oyp = (int)((((y + 1) & my) - y) << lx), oym = (int)((((y - 1) & my) - y) << lx);
+    const int ozp = (int)((((z + 1) & mz) - z) << (lx + ly)), ozm = (int)((((z - 1) & mz) - z) << (lx + ly));
+    const unsigned x = (base & mx) + (unsigned)i, p = base + (unsigned)i;
+    const T *Fp = F + p;
+    s.c = Fp[0];
+    s.xp = Fp[(int)(((x + 1) & mx) - x)];
+    s.xm = Fp[(int)(((x - 1) & mx) - x)];
+    s.yp = Fp[oyp];
+    s.ym = Fp[oym];
+    s.zp = Fp[ozp];
+    s.zm = Fp[ozm];
 }
 
 // Fused LES flux: features (centred differences), the network and the SGS
@@ -249,7 +258,7 @@ __global__ __launch_bounds__(256) void k_les_mlp_flux(const v4s *__restrict__ im
     unsigned t = blockIdx.x * 4 + w;
     const unsigned stride = gridDim.x * 4;
     Stencil<T> cur;
-    if (t < ntile) stencil_load(F, lx, ly, lz, t * 16 + i, cur);
+    if (t < ntile) stencil_load_tile(F, lx, ly, lz, t, i, cur);
     for (; t < ntile; t += stride) {
         const unsigned p = t * 16 + i;
         // this lane's 3 features and point value -> LDS
@@ -260,7 +269,7 @@ __global__ __launch_bounds__(256) void k_les_mlp_flux(const v4s *__restrict__ im
         cw[4 * i + g] = has_f ? cur.c : (T)0;
         // prefetch the next tile
         const bool more = t + stride < ntile;
-        if (more) stencil_load(F, lx, ly, lz, (t + stride) * 16 + i, cur);
+        if (more) stencil_load_tile(F, lx, ly, lz, t + stride, i, cur);
         tile_sync<true>();
         v4f xin = g < 3 ? *reinterpret_cast<const v4f *>(feat + 12 * i + 4 * g) : v4f{0, 0, 0, 0};
         const float r2 = point_r2(xin);
@@ -276,7 +285,8 @@ __global__ __launch_bounds__(256) void k_les_mlp_flux(const v4s *__restrict__ im
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             if (!ok[r]) continue;
-            qp[r][p] = (T)((double)cw[ca[r]] * (double)cw[cb[r]] + (double)o[r]);
+            if constexpr (sizeof(T) == 4) qp[r][p] = __builtin_fmaf(cw[ca[r]], cw[cb[r]], o[r]);
+            else qp[r][p] = (T)((double)cw[ca[r]] * (double)cw[cb[r]] + (double)o[r]);
         }
     }
 }

@@ -55,7 +55,7 @@ __device__ __forceinline__ float point_r2(v4f x)
     r += __shfl_xor(r, 32, 64);
     return r;
 }
-__device__ __forceinline__ float inv_radius(float r2) { return r2 > 0.f ? 1.0f / sqrtf(r2) : 0.f; }
+__device__ __forceinline__ float inv_radius(float r2) { return r2 > 0.f ? __builtin_amdgcn_rsqf(r2) : 0.f; }
 
 // Hidden activation, ACT = SAM_MLP_ACT_RELU or SAM_MLP_ACT_GELU.
 //
