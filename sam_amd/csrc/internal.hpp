@@ -125,12 +125,16 @@ int launch_diag_stats(const DevDiag *diag, double *stats, hipStream_t st);
 // out (device, 3 doubles): div_max, div_rms, n_div of the state U
 int launch_div_check(const Dims &d, const double *U, double *partials, double *out, hipStream_t st);
 
+// blocks of `kernel` (launched with `threads` threads) the current device
+// holds at once; cached per kernel and device
+long long resident_blocks(const void *kernel, int threads);
+
 // SGS (sgs.hip)
 template <typename T> int launch_strain_smag(const Dims &d, const T *P, T *smag, T *S6, double dxi, double dyi,
                                              double dzi, hipStream_t st);
 template <typename T> int launch_filter(const Dims &d, const T *in, T *out, T *tmp, int nvar, hipStream_t st);
 // dm.f's dynamic procedure in one pass: block sums of L:M and M:M (2 doubles
-// per block, dm_fused_rows(d) blocks) for launch_dm_coef
+// per block, at most dm_fused_rows(d) blocks) for launch_dm_coef
 template <typename T> int launch_dm_fused(const Dims &d, const T *P, double dxi, double dyi, double dzi,
                                           double *partials, hipStream_t st, int *nblocks_out);
 long long dm_fused_rows(const Dims &d);

@@ -39,10 +39,9 @@ static inline int launched()
     return e == hipSuccess ? SAM_OK : SAM_ERR_HIP;
 }
 
-// Grid of a persistent kernel: the blocks the device holds at once (capped by
-// the number of work items).  Occupancy per kernel and device is computed
-// once and cached.
-static unsigned persistent_grid(const void *kernel, int threads, long long nwork)
+// The blocks of a kernel the current device holds at once.  Occupancy per
+// kernel and device is computed once and cached.
+long long resident_blocks(const void *kernel, int threads)
 {
     static std::mutex mu;
     static std::map<std::pair<const void *, int>, long long> resident;
@@ -62,6 +61,14 @@ static unsigned persistent_grid(const void *kernel, int threads, long long nwork
         }
         nres = it->second;
     }
+    return nres;
+}
+
+// Grid of a persistent kernel: the resident blocks, capped by the number of
+// work items
+static unsigned persistent_grid(const void *kernel, int threads, long long nwork)
+{
+    const long long nres = resident_blocks(kernel, threads);
     return (unsigned)(nwork < nres ? nwork : nres);
 }
 
@@ -1616,14 +1623,38 @@ __global__ __launch_bounds__(256) void k_stats_phys(const T *__restrict__ P, lon
     const long long b1 = b0 + chunk < np ? b0 + chunk : np;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     double s[4] = {0, 0, 0, 0}, s2[4] = {0, 0, 0, 0}, mx[4] = {0, 0, 0, 0};
-    for (long long i = b0 + threadIdx.x; i < b1; i += 256) {
+    // 16-byte loads, every field's load of an iteration issued before any use
+    constexpr int V = 16 / sizeof(T);
+    typedef T vec __attribute__((ext_vector_type(V)));
+    if ((b1 - b0) % V == 0 && np % V == 0) {
+        for (long long i = b0 + (long long)threadIdx.x * V; i < b1; i += 256 * V) {
+            vec a[4];
 #pragma unroll
-        for (int f = 0; f < 4; f++) {
-            if (f < nvar) {
-                const double a = (double)P[f * np + i];
-                s[f] += a;
-                s2[f] += a * a;
-                mx[f] = fmax(mx[f], fabs(a));
+            for (int f = 0; f < 4; f++)
+                if (f < nvar) a[f] = *reinterpret_cast<const vec *>(P + f * np + i);
+#pragma unroll
+            for (int f = 0; f < 4; f++) {
+                if (f < nvar) {
+#pragma unroll
+                    for (int e = 0; e < V; e++) {
+                        const double x = (double)a[f][e];
+                        s[f] += x;
+                        s2[f] += x * x;
+                        mx[f] = fmax(mx[f], fabs(x));
+                    }
+                }
+            }
+        }
+    } else {
+        for (long long i = b0 + threadIdx.x; i < b1; i += 256) {
+#pragma unroll
+            for (int f = 0; f < 4; f++) {
+                if (f < nvar) {
+                    const double a = (double)P[f * np + i];
+                    s[f] += a;
+                    s2[f] += a * a;
+                    mx[f] = fmax(mx[f], fabs(a));
+                }
             }
         }
     }

@@ -57,10 +57,13 @@ __device__ __forceinline__ void strain_c(const T *__restrict__ P, const Grid3 &g
     s[5] = 0.25 * (wx + uz);
 }
 
+// FP32: the hardware square root (v_sqrt_f32, 1 ulp) without the library's
+// correctly rounded fix-up sequence; the argument is a sum of squares, >= 0
 template <typename C> __device__ __forceinline__ C smag6c(const C s[6])
 {
     const C sum = s[0] * s[0] + s[1] * s[1] + s[2] * s[2] + (C)2 * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5]);
-    return sqrt((C)2 * sum);
+    if constexpr (sizeof(C) == 4) return __builtin_amdgcn_sqrtf((C)2 * sum);
+    else return sqrt((C)2 * sum);
 }
 
 __device__ __forceinline__ double smag6(const double s[6])
@@ -168,7 +171,7 @@ template <typename T> int launch_filter(const Dims &d, const T *in, T *out, T *t
 constexpr int DMY = 8, DMF = 21;
 
 template <typename T, int DMX>
-__global__ __launch_bounds__(DMX * DMY) void k_dm_fused(const T *__restrict__ P, Grid3 g, int zc, double dxi, double dyi,
+__global__ __launch_bounds__(DMX * DMY, sizeof(T) == 4 ? 1 : 1) void k_dm_fused(const T *__restrict__ P, Grid3 g, int zc, double dxi, double dyi,
                                                          double dzi, double *__restrict__ partials)
 {
     using C = T;
@@ -222,14 +225,14 @@ __global__ __launch_bounds__(DMX * DMY) void k_dm_fused(const T *__restrict__ P,
     }
     __syncthreads();
 
-    T pa[DMF], pb[DMF], pc[DMF];                    // x-y filtered planes of this column, rotated
+    // this column's x-y filtered planes: m1 = plane p-1 and the partial z sum
+    // zs = (plane p-2) + 4 (plane p-1); plane p completes the sum of p-1
+    T zs[DMF], m1[DMF];
 #pragma unroll
-    for (int f = 0; f < DMF; f++) pa[f] = pb[f] = pc[f] = (T)0;
+    for (int f = 0; f < DMF; f++) zs[f] = m1[f] = (T)0;
     double num = 0.0, den = 0.0;
     const int ip[6] = {0, 1, 2, 0, 1, 0}, jp[6] = {0, 1, 2, 1, 2, 2};
-    // one plane: m2, m1 the x-y filtered planes p-2, p-1 of this column, cur
-    // receives plane p; the caller rotates the three arrays (no copies)
-    auto plane = [&](const int p, const T (&m2)[DMF], const T (&m1)[DMF], T (&cur)[DMF]) {
+    auto plane = [&](const int p) {
         // ring slots of planes p-1, p, p+1
         const int sm1 = (p - 1 - (z0 - 2)) % 3, s0 = (p - (z0 - 2)) % 3, sp1 = (p + 1 - (z0 - 2)) % 3;
         T nxt[NPRE];
@@ -289,17 +292,20 @@ __global__ __launch_bounds__(DMX * DMY) void k_dm_fused(const T *__restrict__ P,
         __syncthreads();
         // the ring slot of plane p-1 is free: plane p+2 goes there
         if (more) store(sm1, nxt);
-        // stage B: y sums -> the x-y filtered plane p of this thread's column
+        // stage B: y sums -> the x-y filtered plane p of this thread's column;
+        // plane p - 1 is then filtered: z sum of planes p-2, p-1, p.  The sums
+        // are G = 216 x the filtered fields ((1,4,1)/6 in three directions),
+        // and L' = 216^2 L, M' = 216^2 M are formed from them: C = -1/2
+        // <L:M>/<M:M> is unchanged by the common factor 216^4
+        C fl[DMF];
 #pragma unroll
-        for (int f = 0; f < DMF; f++) cur[f] = xf[f][ty][tx] + (C)4 * xf[f][ty + 1][tx] + xf[f][ty + 2][tx];
+        for (int f = 0; f < DMF; f++) {
+            const T cur = xf[f][ty][tx] + (C)4 * xf[f][ty + 1][tx] + xf[f][ty + 2][tx];
+            fl[f] = zs[f] + cur;
+            zs[f] = m1[f] + (C)4 * cur;
+            m1[f] = cur;
+        }
         if (p >= z0 + 1) {
-            // plane p - 1 is filtered: z sum of planes p-2, p-1, p.  The sums
-            // are G = 216 x the filtered fields ((1,4,1)/6 in three
-            // directions), and L' = 216^2 L, M' = 216^2 M are formed from them:
-            // C = -1/2 <L:M>/<M:M> is unchanged by the common factor 216^4
-            C fl[DMF];
-#pragma unroll
-            for (int f = 0; f < DMF; f++) fl[f] = m2[f] + (C)4 * m1[f] + cur[f];
             const C smh = smag6c(fl + 3);
             C pn = 0, pd = 0;
 #pragma unroll
@@ -316,9 +322,9 @@ __global__ __launch_bounds__(DMX * DMY) void k_dm_fused(const T *__restrict__ P,
         __syncthreads();           // xf is rewritten and the ring read by the next plane
     };
     for (int p = z0 - 1; p <= z0 + zc; p += 3) {
-        plane(p, pa, pb, pc);
-        if (p + 1 <= z0 + zc) plane(p + 1, pb, pc, pa);
-        if (p + 2 <= z0 + zc) plane(p + 2, pc, pa, pb);
+        plane(p);
+        if (p + 1 <= z0 + zc) plane(p + 1);
+        if (p + 2 <= z0 + zc) plane(p + 2);
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
@@ -341,20 +347,37 @@ __global__ __launch_bounds__(DMX * DMY) void k_dm_fused(const T *__restrict__ P,
     }
 }
 
-// planes per block: z chunks until the grid has 8192 blocks (about 32 per
-// CU), at least 8 planes (each chunk recomputes 2 product planes)
-static int dm_planes(const Dims &d, int dmx)
+// Planes per block.  A block computes zc + 2 planes of products for zc
+// outputs and the device holds R blocks at once, so a launch takes about
+// ceil(blocks / R) * (zc + 2) plane times: zc is the power of two in
+// [min(8, nz), 64] that minimises that (the smallest on a tie, for balance).
+// 128^3 FP32: 16 planes, 512 blocks in one round (8 planes: 1024 blocks in
+// two); 512^3: 64 planes.
+static int dm_min_planes(const Dims &d) { return d.nz < 8 ? d.nz : 8; }
+
+template <typename T> static int dm_planes(const Dims &d, int dmx)
 {
+    const void *kern = dmx == 32 ? (const void *)k_dm_fused<T, 32> : (const void *)k_dm_fused<T, 16>;
+    const long long res = resident_blocks(kern, dmx * DMY);
     const long long cols = (long long)(d.nx / dmx) * (d.ny / DMY);
-    int zc = d.nz;
-    while (zc > 8 && cols * (d.nz / zc) < 8192) zc >>= 1;
-    return zc;
+    int best = dm_min_planes(d);
+    long long best_cost = -1;
+    for (int zc = best; zc <= d.nz && zc <= 64; zc *= 2) {
+        const long long nblk = cols * (d.nz / zc);
+        const long long cost = (nblk + res - 1) / res * (zc + 2);
+        if (best_cost < 0 || cost < best_cost) {
+            best = zc;
+            best_cost = cost;
+        }
+    }
+    return best;
 }
 
+// an upper bound on the blocks of launch_dm_fused (for sizing partials)
 long long dm_fused_rows(const Dims &d)
 {
     const int dmx = d.nx >= 32 ? 32 : 16;
-    return (long long)(d.nx / dmx) * (d.ny / DMY) * (d.nz / dm_planes(d, dmx));
+    return (long long)(d.nx / dmx) * (d.ny / DMY) * (d.nz / dm_min_planes(d));
 }
 
 template <typename T>
@@ -364,7 +387,7 @@ int launch_dm_fused(const Dims &d, const T *P, double dxi, double dyi, double dz
     Grid3 g{d.nx, d.ny, d.nz, d.pfield};
     const int dmx = d.nx >= 32 ? 32 : 16;
     if (d.ny % DMY) return SAM_ERR_UNSUPPORTED;
-    const int zc = dm_planes(d, dmx);
+    const int zc = dm_planes<T>(d, dmx);
     const dim3 grid((unsigned)((d.nx / dmx) * (d.ny / DMY) * (d.nz / zc)));
     *nblocks_out = (int)grid.x;
     if (dmx == 32)
@@ -564,12 +587,19 @@ int launch_sgs_flux(const Dims &d, const T *P, const DevDiag *diag, T *Q, double
                            dxi, dyi, dzi, pr_t_inv, d.lu);
         return launched_sgs();
     }
-    const int dmx = d.nx >= 32 ? 32 : 16;
+    // 64-point rows: the x halo adds two 64-byte sectors to 256 bytes of a
+    // row instead of to 128 (FP32)
+    const int dmx = d.nx >= 64 ? 64 : d.nx >= 32 ? 32 : 16;
     const long long cols = (long long)(d.nx / dmx) * (d.ny / FY);
-    int zc = d.nz;                                 // z chunks until about 8192 blocks, at least 8 planes
-    while (zc > 8 && cols * (d.nz / zc) < 8192) zc >>= 1;
+    // z chunks until about 2^22 threads (8192 blocks of 512), at least 8 planes
+    const long long want = (8192ll * 512) / (dmx * FY);
+    int zc = d.nz;
+    while (zc > 8 && cols * (d.nz / zc) < want) zc >>= 1;
     const dim3 grid((unsigned)(cols * (d.nz / zc)));
-    if (dmx == 32)
+    if (dmx == 64)
+        hipLaunchKernelGGL((k_sgs_flux<T, 64>), grid, dim3(64 * FY), 0, st, P, diag, Q, g, zc, dxi, dyi, dzi,
+                           pr_t_inv, d.lu);
+    else if (dmx == 32)
         hipLaunchKernelGGL((k_sgs_flux<T, 32>), grid, dim3(32 * FY), 0, st, P, diag, Q, g, zc, dxi, dyi, dzi,
                            pr_t_inv, d.lu);
     else
