@@ -1681,7 +1681,10 @@ int launch_stats_phys(const Dims &d, const void *P, int is_double, int nvar, dou
                       hipStream_t st, int *nblocks_out)
 {
     const long long np = d.pfield;
-    const long long chunk = 2048;
+    // 2048 points per block, more while the grid keeps >= 4096 blocks (each
+    // block ends in 3 Lu reductions)
+    long long chunk = 2048;
+    while (np / (2 * chunk) >= 4096) chunk *= 2;
     const unsigned nb = (unsigned)((np + chunk - 1) / chunk);
     *nblocks_out = (int)nb;
     if (is_double)
