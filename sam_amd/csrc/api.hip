@@ -268,8 +268,8 @@ static int validate(const sam_params *p)
 
 // Workspace of sam_vel_max_noalloc, one copy per device for the life of
 // the process: the statistics reduction's partial rows (one row of 3 Lu <= 12
-// doubles per 2048 points, as k_stats_phys writes them), the rows of its
-// second level and the DevDiag it finalises into.  6.3 MB.
+// doubles per k_stats_phys block, at most one per 2048 points), the rows of
+// its second level and the DevDiag it finalises into.  6.3 MB.
 constexpr long long NOALLOC_ROWS = 65536;        // grids up to 2^27 points (512^3)
 constexpr long long NOALLOC_DIAG = NOALLOC_ROWS * 12 + FINALIZE_ROWS;
 constexpr long long NOALLOC_DOUBLES = NOALLOC_DIAG + (long long)(sizeof(DevDiag) / sizeof(double));
@@ -1182,7 +1182,7 @@ extern "C" int sam_vel_max_noalloc(sam_handle *h, const double *phys, double dt_
 {
     if (!h || !phys || !stats) return set_err(SAM_ERR_VALUE, "null argument");
     const Dims &d = h->d;
-    const long long rows = (d.pfield + 2047) / 2048;   // k_stats_phys's blocks
+    const long long rows = (d.pfield + 2047) / 2048;   // bounds k_stats_phys's blocks
     if (rows > NOALLOC_ROWS)
         return set_err(SAM_ERR_SIZE,
                        "sam_vel_max_noalloc: %lld points need %lld partial rows, above the workspace's %lld "

@@ -1806,7 +1806,7 @@ __global__ __launch_bounds__(256) void k_finalize_partial(const double *__restri
 int launch_finalize(const Dims &d, const StepConsts &k, const double *partials, int nblocks, DevDiag *diag,
                     int update_dt, hipStream_t st, double *rows)
 {
-    if (nblocks >= 8 * FIN_THREADS) {
+    if (nblocks >= 4 * FIN_THREADS) {
         // two levels: FIN_G blocks reduce chunks into rows (by default kept
         // past every other use of the partials buffer), then one block finishes
         if (!rows) rows = const_cast<double *>(partials) + (partials_capacity(d) - (long long)FIN_G * 12);
