@@ -105,6 +105,20 @@ def test_stage_products_velmax_project(oracle_mod):
     assert rel_l2(s.project(F, us), o.project(F, us)) < TOL64
 
 
+@pytest.mark.parametrize("shape", [(64, 64, 32), (256, 256, 256)], ids=["64x64x32", "256c"])
+def test_vel_max_statistics_chunking(oracle_mod, shape):
+    """vel_max.f's statistics at sizes where the per-block point chunk and
+    the finalize differ (64x64x32: 2048-point chunks, one finalize level;
+    256^3: 4096-point chunks, 4096 partial rows, two levels)"""
+    o, s = pair(oracle_mod, shape[0], shape[1], shape[2], i_strat=1, grav=16.0, lapse0=1.0, i_cfl=1, cfl0=0.5)
+    rng = np.random.default_rng(11)
+    phys = np.asfortranarray(rng.standard_normal((o.nx, o.ny, o.nz, o.lu)) * np.array([1.0, 2.0, 0.5, 3.0]))
+    st_ref = o.vel_max(phys, 0.01)
+    st = s.vel_max(phys, 0.01)
+    s.close()
+    np.testing.assert_allclose(st, st_ref, rtol=1e-12, atol=1e-15)
+
+
 @pytest.mark.parametrize("strat", [0, 1])
 def test_rhs_full(oracle_mod, strat):
     o, s = pair(oracle_mod, 32, i_strat=strat, grav=16.0, lapse0=1.0, vis=5e-3)
