@@ -48,20 +48,26 @@ __device__ __forceinline__ v4f mfma2(v4s a0, v4s a1, v4s b0, v4s b1, v4f c)
 
 // |x|^2 of a point's scaled features, held 4 per lane in lanes i, i+16, i+32
 // (lane i+48 holds zeros); every lane of the point gets the sum
+// (gfx950 row swaps: after v_permlane16_swap of two copies of r, the pair
+// holds r and its xor-16 partner in every lane, in some order; their sum is
+// the same either way, as after v_permlane32_swap for xor 32)
 __device__ __forceinline__ float point_r2(v4f x)
 {
     float r = x[0] * x[0] + x[1] * x[1] + x[2] * x[2] + x[3] * x[3];
-    r += __shfl_xor(r, 16, 64);
-    r += __shfl_xor(r, 32, 64);
-    return r;
+    const auto s16 = __builtin_amdgcn_permlane16_swap(__float_as_uint(r), __float_as_uint(r), false, false);
+    r = __uint_as_float(s16[0]) + __uint_as_float(s16[1]);
+    const auto s32 = __builtin_amdgcn_permlane32_swap(__float_as_uint(r), __float_as_uint(r), false, false);
+    return __uint_as_float(s32[0]) + __uint_as_float(s32[1]);
 }
 __device__ __forceinline__ float inv_radius(float r2) { return r2 > 0.f ? __builtin_amdgcn_rsqf(r2) : 0.f; }
 
 // Hidden activation, ACT = SAM_MLP_ACT_RELU or SAM_MLP_ACT_GELU.
 //
-// ReLU: one instruction per value, a signed max on the bits (a negative
-// float, sign bit set, is a negative integer; fmaxf would also quiet NaNs
-// first).  Derivative 1 for x > 0, else 0.
+// ReLU: a signed max on the bits (a negative float, sign bit set, is a
+// negative integer; fmaxf would also quiet NaNs first), one instruction per
+// FP32 value; where the result is rounded to BF16 anyway (act_bf4) it is
+// taken after the rounding, two values per instruction.  Derivative 1 for
+// x > 0, else 0.
 //
 // GELU, tanh form: 0.5 x (1 + tanh u) with u = c (x + a x^3), evaluated as
 // x * sigmoid(2u) = x / (1 + 2^(-2u log2 e)): one exp2 and one reciprocal,
@@ -127,9 +133,22 @@ struct Acts {
     v4f out;         // outputs 4g..4g+3
 };
 
+// ReLU after the BF16 rounding, on the packed 16-bit values: max(b, 0) as
+// signed integers zeroes exactly the negative (sign-bit) values, two per
+// instruction; rounding is monotonic, so this equals rounding max(z, 0)
 template <int ACT> __device__ __forceinline__ v4s act_bf4(v4f z)
 {
-    return bf4(v4f{act<ACT>(z[0]), act<ACT>(z[1]), act<ACT>(z[2]), act<ACT>(z[3])});
+    if constexpr (ACT == SAM_MLP_ACT_GELU) {
+        return bf4(v4f{act<ACT>(z[0]), act<ACT>(z[1]), act<ACT>(z[2]), act<ACT>(z[3])});
+    } else {
+        typedef unsigned v2u __attribute__((ext_vector_type(2)));
+        auto relu2 = [](unsigned w) {
+            unsigned r;
+            asm("v_pk_max_i16 %0, %1, 0" : "=v"(r) : "v"(w));
+            return r;
+        };
+        return __builtin_bit_cast(v4s, v2u{relu2(pk2(z[0], z[1])), relu2(pk2(z[2], z[3]))});
+    }
 }
 
 // Weight-tile sources of the forward pass: tile k of this lane from the LDS
