@@ -182,8 +182,12 @@ __global__ __launch_bounds__(DMX * DMY, sizeof(T) == 4 ? 1 : 1) void k_dm_fused(
     static_assert(NA <= NT, "one stage-A item per thread");
     constexpr int NU = 3 * UX * UY;               // u values of one plane
     constexpr int NPRE = (NU + NT - 1) / NT;      // per thread
-    __shared__ T us[3][3][UY][UX];
-    __shared__ T xf[DMF][PY][DMX];
+    // rows padded by one element: stage A's lanes (x segment s, row r) read
+    // and write at 2 s + r * stride, and with strides 36 and 32 the two rows
+    // of a half-wavefront fell on the same (even) banks, a 2-way conflict on
+    // every stage-A LDS access; with 37 and 33 they take even and odd banks
+    __shared__ T us[3][3][UY][UX + 1];
+    __shared__ T xf[DMF][PY][DMX + 1];
     __shared__ double red[2][NT / 64];
     const int tid = threadIdx.x;
     // tile of this block, dealt to the XCDs as in k_sgs_flux
