@@ -296,7 +296,7 @@ int  sam_set_mlp_refresh(sam_handle *h, int nsteps);
  * as dm.f does.  The control for the learned closure's CDelta^2 refresh. */
 int  sam_set_dyn_refresh(sam_handle *h, int nsteps);
 /* a-posteriori check of the learned closure on the current state, with the
- * dynamic model as judge: the shell spectra of the SGS energy transfer
+ * dynamic model as the arbiter: the shell spectra of the SGS energy transfer
  * T_m(k) = tk_m(k) - tk_none(k) (spectra.f's tk of the RHS with model m
  * minus without) are compared; err = ||T_mlp - T_dyn|| / ||T_dyn|| over
  * shells.  When err > tol the handle falls back to SAM_SGS_DYN

@@ -499,7 +499,7 @@ def attach_mlp(sam, mlp):
 
 
 def sgs_check(sam, tol):
-    """a-posteriori check of the learned closure (dynamic model as judge);
+    """a-posteriori check of the learned closure (the dynamic model as the arbiter);
     returns (err, switched_to_dynamic)"""
     e = ctypes.c_double()
     sw = ctypes.c_int()

@@ -1,12 +1,13 @@
-"""CPU checks of bench.py's bytes model: the survey's per-substep formula,
-the read/write split that selects each kernel's streaming mix, and the step
-floor built from it (no GPU)."""
+"""CPU checks of bench.py's bytes model: the step's bytes at the grid sizes
+worked out by hand for the fused 3-pass step, the read/write split that
+selects each kernel's streaming mix, and the step floor built from it (no
+GPU)."""
 import math
 
 import bench as B
 
 
-def test_step_bytes_match_the_survey_figures():
+def test_step_bytes_match_the_hand_counts():
     # 512^3 stratified: 112.3 GB/step FP64, 65.7 GB/step FP32-split (fused
     # 3-pass model); 1024^3 FP32-split 526.2 GB/step; 64^3 unstratified FP64
     # 0.16 GB/step

@@ -1,6 +1,6 @@
 """GPU: the learned closure end to end -- DNS -> box-filtered samples ->
 training on the GPU -> LES in the loop -- and what it gains over the LES
-without a model; the dynamic model as judge with fallback."""
+without a model; the dynamic model as the arbiter with fallback."""
 import numpy as np
 import pytest
 
